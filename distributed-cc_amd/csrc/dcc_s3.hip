@@ -1,0 +1,247 @@
+// HBM-bound transforms of v^3 blocks under the symmetric group S3 of (a,b,c).
+//
+// Every permutation-type operation on a T3 block -- taking a permuted block,
+// spin summation, the P3_full projection -- reads element x of the output
+// from elements p(x) of the input for the six axis permutations p.  A
+// workgroup owns one *tile orbit*: an 8x8x8 tile with tile coordinates
+// (A,B,C) together with its five permuted tiles.  The six input tiles are
+// read into LDS along their contiguous axis (runs of 8 doubles), every
+// permuted access is then an LDS access, and the six output tiles are written
+// along their contiguous axis.  Each element is read and written exactly once.
+#include "dcc_common.h"
+
+namespace {
+
+constexpr int E = 8;            // tile edge
+constexpr int TE = E * E * E;   // elements per tile
+constexpr int kThreads = 256;
+constexpr int kPer = TE / kThreads;  // elements per thread per tile
+
+// The six permutations in the reference's tp_t3 order (src/rccsdt.c:32-39);
+// X.transpose(p)[x] = X[y] with y[p[n]] = x[n].
+__constant__ int kP[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+
+__device__ __forceinline__ int perm_index(int a, int b, int c) {
+    // index in kP of the permutation (a,b,c)
+    return a == 0 ? (b == 1 ? 0 : 1) : (a == 1 ? (b == 0 ? 2 : 3) : (b == 0 ? 4 : 5));
+}
+
+// Orbit o -> tile triple (A >= B >= C) with o = A(A+1)(A+2)/6 + B(B+1)/2 + C.
+__device__ __forceinline__ void decode_orbit(int o, int &A, int &B, int &C) {
+    A = 0;
+    while ((A + 1) * (A + 2) * (A + 3) / 6 <= o) ++A;
+    int r = o - A * (A + 1) * (A + 2) / 6;
+    B = 0;
+    while ((B + 1) * (B + 2) / 2 <= r) ++B;
+    C = r - B * (B + 1) / 2;
+}
+
+__host__ __device__ __forceinline__ int64_t n_orbits(int64_t nt) { return nt * (nt + 1) * (nt + 2) / 6; }
+
+// Load the six permuted tiles of orbit (R0,R1,R2) of block `src` into LDS.
+__device__ __forceinline__ void load_orbit(double (*tiles)[TE], const double *__restrict__ src, int nvir,
+                                           const int R[3]) {
+    const int64_t v = nvir, v2 = v * v;
+    for (int s = 0; s < 6; ++s) {
+        const int t0 = R[kP[s][0]], t1 = R[kP[s][1]], t2 = R[kP[s][2]];
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) {
+            const int e = threadIdx.x + h * kThreads;
+            const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
+            double val = 0.0;
+            if (g0 < nvir && g1 < nvir && g2 < nvir) val = src[g0 * v2 + g1 * v + g2];
+            tiles[s][e] = val;
+        }
+    }
+}
+
+// Is output tile s a repeat of an earlier tile of the same orbit?
+__device__ __forceinline__ bool duplicate_tile(int s, const int R[3]) {
+    const int a = R[kP[s][0]], b = R[kP[s][1]], c = R[kP[s][2]];
+    for (int u = 0; u < s; ++u)
+        if (R[kP[u][0]] == a && R[kP[u][1]] == b && R[kP[u][2]] == c) return true;
+    return false;
+}
+
+// Value of X.transpose(p)[x] for x = local element e of output tile s.
+// y[p[n]] = x[n]; y lives in tile s' with ranges R[s'[m]] = R[s[pinv[m]]].
+__device__ __forceinline__ double permuted(const double (*tiles)[TE], int s, int p, int e) {
+    const int x[3] = {e >> 6, (e >> 3) & 7, e & 7};
+    int y[3], sy[3];
+#pragma unroll
+    for (int n = 0; n < 3; ++n) {
+        y[kP[p][n]] = x[n];
+        sy[kP[p][n]] = kP[s][n];
+    }
+    const int sp = perm_index(sy[0], sy[1], sy[2]);
+    return tiles[sp][(y[0] << 6) | (y[1] << 3) | y[2]];
+}
+
+__global__ __launch_bounds__(kThreads) void s3_transform_kernel(double *__restrict__ out,
+                                                                const double *__restrict__ src, int nvir,
+                                                                double beta, const dcc_s3_job *__restrict__ jobs,
+                                                                int norb) {
+    __shared__ double tiles[6][TE];
+    const int64_t job = blockIdx.x / norb;
+    const int orbit = blockIdx.x - (int)(job * norb);
+    int R[3];
+    decode_orbit(orbit, R[0], R[1], R[2]);
+    const dcc_s3_job jb = jobs[job];
+    load_orbit(tiles, src + jb.src, nvir, R);
+    __syncthreads();
+    const int64_t v = nvir, v2 = v * v;
+    double *dst = out + jb.dst;
+    for (int s = 0; s < 6; ++s) {
+        if (duplicate_tile(s, R)) continue;
+        const int t0 = R[kP[s][0]], t1 = R[kP[s][1]], t2 = R[kP[s][2]];
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) {
+            const int e = threadIdx.x + h * kThreads;
+            const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
+            if (g0 >= nvir || g1 >= nvir || g2 >= nvir) continue;
+            double val = 0.0;
+#pragma unroll
+            for (int p = 0; p < 6; ++p)
+                if (jb.coef[p] != 0.0) val += jb.coef[p] * permuted(tiles, s, p, e);
+            double *o = dst + g0 * v2 + g1 * v + g2;
+            if (beta != 0.0) val += beta * (*o);
+            *o = val;
+        }
+    }
+}
+
+// Fused P3_full projection + denominator + t3 update + norms (one tile orbit).
+__global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const double *__restrict__ R, double *__restrict__ t3,
+                                                               double *__restrict__ r3_out, int nvir,
+                                                               const int *__restrict__ blk_ijk,
+                                                               const double *__restrict__ eia, int nvir_diis,
+                                                               double *__restrict__ partials, int norb) {
+    __shared__ double tiles[6][TE];
+    __shared__ double red[2][kThreads / 64];
+    const int64_t blk = blockIdx.x / norb;
+    const int orbit = blockIdx.x - (int)(blk * norb);
+    int Rg[3];
+    decode_orbit(orbit, Rg[0], Rg[1], Rg[2]);
+    const int64_t v = nvir, v2 = v * v, v3 = v2 * v;
+    const double *Rb = R + blk * v3;
+    load_orbit(tiles, Rb, nvir, Rg);
+    const int oi = blk_ijk[3 * blk], oj = blk_ijk[3 * blk + 1], ok = blk_ijk[3 * blk + 2];
+    const double *ei = eia + (int64_t)oi * v, *ej = eia + (int64_t)oj * v, *ek = eia + (int64_t)ok * v;
+    __syncthreads();
+    double nrm = 0.0, nact = 0.0;
+    for (int s = 0; s < 6; ++s) {
+        if (duplicate_tile(s, Rg)) continue;
+        const int t0 = Rg[kP[s][0]], t1 = Rg[kP[s][1]], t2 = Rg[kP[s][2]];
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) {
+            const int e = threadIdx.x + h * kThreads;
+            const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
+            if (g0 >= nvir || g1 >= nvir || g2 >= nvir) continue;
+            double sym = 0.0;
+#pragma unroll
+            for (int p = 0; p < 6; ++p) sym += permuted(tiles, s, p, e);
+            const double own = permuted(tiles, s, 0, e);
+            const double r = (own - sym / 6.0) / (ei[g0] + ej[g1] + ek[g2]);
+            const int64_t off = blk * v3 + g0 * v2 + g1 * v + g2;
+            t3[off] += r;
+            if (r3_out != nullptr) r3_out[off] = r;
+            nrm += r * r;
+            if (g0 < nvir_diis && g1 < nvir_diis && g2 < nvir_diis) nact += r * r;
+        }
+    }
+    // fixed-order reduction: wave shuffles, then the four wave sums in order
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        nrm += __shfl_down(nrm, off, 64);
+        nact += __shfl_down(nact, off, 64);
+    }
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    if (lane == 0) {
+        red[0][wave] = nrm;
+        red[1][wave] = nact;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, b = 0.0;
+        for (int w = 0; w < kThreads / 64; ++w) {
+            a += red[0][w];
+            b += red[1][w];
+        }
+        partials[2 * blockIdx.x] = a;
+        partials[2 * blockIdx.x + 1] = b;
+    }
+}
+
+__global__ __launch_bounds__(256) void reduce_pairs_kernel(const double *__restrict__ x, int64_t n,
+                                                            double *__restrict__ out) {
+    __shared__ double sa[256], sb[256];
+    double a = 0.0, b = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        a += x[2 * i];
+        b += x[2 * i + 1];
+    }
+    sa[threadIdx.x] = a;
+    sb[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = sa[0];
+        out[1] = sb[0];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double beta, const dcc_s3_job *jobs,
+                         int64_t njobs, dcc_stream_t stream) {
+    DCC_CHECK(nvir >= 0 && njobs >= 0, "negative extent");
+    if (nvir == 0 || njobs == 0) return 0;
+    DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
+    const int64_t nt = (nvir + E - 1) / E;
+    const int64_t norb = n_orbits(nt);
+    DCC_CHECK(norb * njobs < ((int64_t)1 << 31), "grid too large");
+    hipLaunchKernelGGL(s3_transform_kernel, dim3((unsigned)(norb * njobs)), dim3(kThreads), 0,
+                       dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)norb);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+int64_t dcc_r3_finalize_tiles(int64_t nvir) {
+    const int64_t nt = (nvir + E - 1) / E;
+    return n_orbits(nt);
+}
+
+int dcc_r3_finalize_f64(const double *R, double *t3, double *r3_out, int64_t nblk, int64_t nvir,
+                        const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
+                        double *partials, dcc_stream_t stream) {
+    DCC_CHECK(nblk >= 0 && nvir >= 0 && nocc >= 0, "negative extent");
+    if (nblk == 0 || nvir == 0) return 0;
+    DCC_CHECK(R != nullptr && t3 != nullptr && blk_ijk != nullptr && eia != nullptr && partials != nullptr,
+              "null pointer");
+    const int64_t norb = dcc_r3_finalize_tiles(nvir);
+    DCC_CHECK(norb * nblk < ((int64_t)1 << 31), "grid too large");
+    if (nvir_diis < 0 || nvir_diis > nvir) nvir_diis = nvir;
+    hipLaunchKernelGGL(r3_finalize_kernel, dim3((unsigned)(norb * nblk)), dim3(kThreads), 0,
+                       dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis, partials,
+                       (int)norb);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+int dcc_reduce_pairs_f64(const double *x, int64_t n, double *out, dcc_stream_t stream) {
+    DCC_CHECK(n >= 0 && out != nullptr, "bad arguments");
+    DCC_CHECK(n == 0 || x != nullptr, "null pointer");
+    hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(256), 0, dcc::as_stream(stream), x, n, out);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // extern "C"

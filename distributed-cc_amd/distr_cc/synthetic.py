@@ -1,0 +1,97 @@
+"""Seeded synthetic RCCSDT inputs of the shapes the triples path runs at.
+
+Generator (host NumPy, FP64):
+  1. ``rng = np.random.default_rng(seed)``.
+  2. occupied energies ~ U[-1.5,-0.3] (sorted), virtual energies ~ U[0.1,3.0]
+     (sorted); ``fock = diag(mo_energy)``.
+  3. ``B ~ N(0, 0.05^2)`` of shape (naux, nmo, nmo), symmetrized in (p,q);
+     chemist ``(pq|rs) = sum_Q B[Q,p,q] B[Q,r,s]``; physicist
+     ``pppp[p,q,r,s] = <pq|rs> = (pr|qs)``.
+  4. ``t1 = 0``; ``t2[i,j,a,b] = (ia|jb) / (e_i + e_j - e_a - e_b)``.
+  5. each canonical block (i<=j<=k) is N(0, sigma_t3^2) noise, symmetrized
+     for tied occupied indices so that the simultaneous-permutation symmetry
+     holds, then divided by D_ijk^abc.
+
+The ERI tensor is only materialized when ``nmo`` is small enough; large shapes
+build the slices the triples path needs directly from ``B``.
+"""
+import numpy as np
+
+
+class SyntheticSystem:
+    """Holds the seeded inputs; slices are built lazily from the B factor."""
+
+    def __init__(self, nocc, nvir, naux=300, seed=20261202, sigma_b=0.05, sigma_t3=1e-4,
+                 dtype=np.float64):
+        self.nocc, self.nvir, self.naux, self.seed = int(nocc), int(nvir), int(naux), int(seed)
+        self.nmo = self.nocc + self.nvir
+        rng = np.random.default_rng(self.seed)
+        e_occ = np.sort(rng.uniform(-1.5, -0.3, self.nocc))
+        e_vir = np.sort(rng.uniform(0.1, 3.0, self.nvir))
+        self.mo_energy = np.concatenate([e_occ, e_vir])
+        self.fock = np.diag(self.mo_energy)
+        B = rng.normal(0.0, sigma_b, (self.naux, self.nmo, self.nmo))
+        self.B = 0.5 * (B + B.transpose(0, 2, 1))
+        self._rng_t3 = np.random.default_rng(self.seed + 7919)
+        self.sigma_t3 = sigma_t3
+        self.dtype = dtype
+
+    # physicist <pq|rs> = (pr|qs) = sum_Q B[Q,p,r] B[Q,q,s]
+    def eri_phys(self, ps, qs, rs, ss):
+        """Slice <p q | r s> for index slices (ps, qs, rs, ss) of the MO range."""
+        Bpr = self.B[:, ps, rs]
+        Bqs = self.B[:, qs, ss]
+        return np.einsum('Qpr,Qqs->pqrs', Bpr, Bqs, optimize=True)
+
+    def pppp(self):
+        """Full physicist ERI tensor (nmo^4); only for small systems."""
+        s = slice(0, self.nmo)
+        return self.eri_phys(s, s, s, s)
+
+    def t2(self):
+        o = self.nocc
+        so, sv = slice(0, o), slice(o, self.nmo)
+        eia = self.mo_energy[:o, None] - self.mo_energy[None, o:]
+        d = eia[:, None, :, None] + eia[None, :, None, :]
+        return self.eri_phys(so, so, sv, sv) / d
+
+    def t3_tri(self):
+        """Canonical-block T3, shape (nocc3, v, v, v), lexicographic i<=j<=k."""
+        o, v = self.nocc, self.nvir
+        eia = self.mo_energy[:o, None] - self.mo_energy[None, o:]
+        blocks = []
+        for i in range(o):
+            for j in range(i, o):
+                for k in range(j, o):
+                    x = self._rng_t3.normal(0.0, self.sigma_t3, (v, v, v))
+                    if i == j == k:
+                        x = (x + x.transpose(0, 2, 1) + x.transpose(1, 0, 2) + x.transpose(1, 2, 0)
+                             + x.transpose(2, 0, 1) + x.transpose(2, 1, 0)) / 6.0
+                    elif i == j:
+                        x = 0.5 * (x + x.transpose(1, 0, 2))
+                    elif j == k:
+                        x = 0.5 * (x + x.transpose(0, 2, 1))
+                    d = eia[i, :, None, None] + eia[j, None, :, None] + eia[k, None, None, :]
+                    blocks.append(x / d)
+        return np.ascontiguousarray(np.stack(blocks).astype(self.dtype))
+
+    def bare_intermediates(self):
+        """Leading (T1 = T2 = 0) values of the triples intermediates: ERI slices.
+
+        Shapes follow distr_cc/rccsdt.py:380,445,625: F_oo (o,o), F_vv (v,v),
+        W_oooo (o,o,o,o), W_ovvo (o,v,v,o), W_ovov (o,v,o,v), W_vooo (v,o,o,o),
+        W_vvvo (v,v,v,o), W_vvvv (v,v,v,v).
+        """
+        o = self.nocc
+        so, sv = slice(0, o), slice(o, self.nmo)
+        f = self.fock
+        return {
+            'F_oo': np.ascontiguousarray(f[so, so]),
+            'F_vv': np.ascontiguousarray(f[sv, sv]),
+            'W_oooo': self.eri_phys(so, so, so, so),
+            'W_ovvo': self.eri_phys(so, sv, sv, so),
+            'W_ovov': self.eri_phys(so, sv, so, sv),
+            'W_vooo': self.eri_phys(sv, so, so, so),
+            'W_vvvo': self.eri_phys(sv, sv, sv, so),
+            'W_vvvv': self.eri_phys(sv, sv, sv, sv),
+        }

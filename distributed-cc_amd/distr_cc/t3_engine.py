@@ -1,0 +1,328 @@
+"""GPU triples engine: the T3 residual, its projection/denominator/update and
+the T3 -> r1/r2/W contributions, as launches of libdcc kernels.
+
+Data in HBM (FP64, C order):
+  t3, R          (n3, v, v, v)     canonical blocks, lexicographic i<=j<=k
+  Y              (npairs, o, v, v, v)  Y[(p,q)][l] = T[l,p,q] for p<=q (ring B operand)
+  WR1/WR3a/WR3b  (o, v, o, v)      ring weights, rows (r,x), cols (l,d)
+  pfs            (6, n3, n3)       W_oooo/F_oo prefactors
+  intermediates  shapes of distr_cc/rccsdt.py:380,445,625
+
+The residual follows distr_cc/rccsdt.py:368-433 term by term, grouped into
+batched gather-GEMMs (dcc_contract_f64):
+  * T2->T3 (rccsdt.py:388-399): 12 launches batched over the n3 blocks;
+  * F_vv and W_vvvv x T3 (rccsdt.py:400-402,424-426): 6 launches, N = (block, ...);
+  * W_ovov/W_ovvo x T3 (rccsdt.py:540-596): with G the part-3 operator
+        G[r;pq][a,b,c] = sum_{l,d} ( -W_ovov[l,b,r,d] Y[l][a,d,c] - W_ovov[l,c,r,d] Y[l][a,b,d]
+                        + W_ovvo[l,a,d,r] Y[l][d,b,c]
+                        - 1/2 (W_ovov[l,a,r,d] + W_ovvo[l,a,d,r]) (Y[l][b,d,c] + Y[l][c,b,d]) )
+    part 3 of target (i,j,k) is G[i;jk], part 4 is G[j;ik] with (a,b) swapped and
+    part 5 is G[k;ij] read as [c,a,b]; one launch per (part, term), batched over
+    the pairs p<=q with rows (r, x) restricted to the part's range of r;
+  * W_oooo/F_oo x T3 (rccsdt.py:616-732): 6 GEMMs M = n3 targets, K = n3 sources,
+    N = v^3, with the prefactors built on the device.
+Then dcc_r3_finalize_f64 fuses rccsdt.py:947-971.
+"""
+import ctypes
+
+import numpy as np
+
+from distr_cc import _lib as L
+from distr_cc.device import DeviceArray, init as device_init
+from distr_cc.distribute_t3 import TP_T3, canonical_case, canonical_triples, ijk_to_linear, nocc3
+
+
+def c_strides(shape):
+    st, acc = [], 1
+    for s in reversed(shape):
+        st.append(acc)
+        acc *= int(s)
+    return tuple(reversed(st))
+
+
+class Operand:
+    """A device tensor with one letter per axis (letters not in a term are fixed per batch)."""
+
+    def __init__(self, buf, letters, shape):
+        assert len(letters) == len(shape)
+        self.buf, self.letters, self.shape = buf, letters, tuple(int(s) for s in shape)
+        self.strides = dict(zip(letters, c_strides(self.shape)))
+        self.ext = dict(zip(letters, self.shape))
+
+    def offset(self, fixed):
+        return sum(self.strides[k] * int(v) for k, v in fixed.items() if k in self.strides)
+
+
+def idxmap(letters, ext, strides):
+    """Three-level map of the group `letters` (outermost first) into an operand."""
+    ls = list(letters)
+    if not 1 <= len(ls) <= 3:
+        raise ValueError("index group must have 1-3 letters: %r" % (letters,))
+    while len(ls) < 3:
+        ls.insert(0, None)
+    e1 = ext[ls[1]] if ls[1] else 1
+    e2 = ext[ls[2]] if ls[2] else 1
+    s = [strides[x] if x else 0 for x in ls]
+    return L.IdxMap(None, 0, int(e1), int(e2), int(s[0]), int(s[1]), int(s[2]))
+
+
+def table_map(table_dev, bstride):
+    return L.IdxMap(table_dev.ptr, int(bstride), 1, 1, 0, 0, 0)
+
+
+class GemmLaunch:
+    """One dcc_contract_f64 call with fixed operands, maps and batch table."""
+
+    def __init__(self, name, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, max_rows,
+                 flops, keep=()):
+        self.name = name
+        self.args = (int(M), int(N), int(K), float(alpha), float(beta), ctypes.c_void_p(A.ptr), am, ak,
+                     ctypes.c_void_p(B.ptr), bk, bn, ctypes.c_void_p(C.ptr), cm, cn)
+        bt = np.zeros(len(batches), dtype=L.BATCH_DTYPE)
+        for n, (ao, bo, co, mb, me) in enumerate(batches):
+            bt[n] = (ao, bo, co, mb, me)
+        self.batches = DeviceArray.from_host(bt)
+        self.nbatch = len(batches)
+        self.max_rows = int(max_rows)
+        self.flops = float(flops)
+        self.keep = keep  # device tables referenced by maps
+
+    def run(self, stream):
+        L.call("dcc_contract_f64", *self.args, self.batches.vp, self.nbatch, self.max_rows,
+               stream.handle if stream is not None else None)
+
+
+def plan_term(name, sub, A, B, C, alpha, beta, batches_fixed, ext):
+    """Plan C[...] (+)= alpha * einsum(sub, A, B) as a batched gather-GEMM.
+
+    ``sub`` is 'xy,yz->xz'-style over the free letters; each entry of
+    ``batches_fixed`` is a dict {letter: value} for the operands' fixed letters.
+    M = letters of C found in A, N = letters of C found in B, K = the rest of A.
+    """
+    sa, rest = sub.split(',')
+    sb, sc = rest.split('->')
+    m_l = ''.join(x for x in sa if x in sc)
+    n_l = ''.join(x for x in sb if x in sc)
+    k_a = ''.join(x for x in sa if x not in sc)
+    k_b = ''.join(x for x in sb if x not in sc)
+    assert sorted(k_a) == sorted(k_b), sub
+    # K order: operand whose innermost letter is a K letter sets the order
+    k_l = k_b if (sb[-1] in k_b or sa[-1] not in k_a) else k_a
+    M = int(np.prod([ext[x] for x in m_l]))
+    N = int(np.prod([ext[x] for x in n_l]))
+    K = int(np.prod([ext[x] for x in k_l]))
+    batches = []
+    for fx in batches_fixed:
+        batches.append((A.offset(fx), B.offset(fx), C.offset(fx), 0, M))
+    return GemmLaunch(name, M, N, K, alpha, beta, A.buf, idxmap(m_l, ext, A.strides), idxmap(k_l, ext, A.strides),
+                      B.buf, idxmap(k_l, ext, B.strides), idxmap(n_l, ext, B.strides),
+                      C.buf, idxmap(m_l, ext, C.strides), idxmap(n_l, ext, C.strides),
+                      batches, M, 2.0 * M * N * K * len(batches_fixed))
+
+
+def s3_jobs_array(jobs):
+    arr = np.zeros(len(jobs), dtype=L.S3JOB_DTYPE)
+    for n, (dst, src, coef) in enumerate(jobs):
+        arr[n]['dst'] = dst
+        arr[n]['src'] = src
+        arr[n]['coef'] = coef
+    return arr
+
+
+class TriplesEngine:
+    """Holds the device state of one RCCSDT triples update for (nocc, nvir)."""
+
+    def __init__(self, nocc, nvir, stream=None, device=0):
+        device_init(device)
+        o, v = int(nocc), int(nvir)
+        self.nocc, self.nvir = o, v
+        self.n3 = n3 = nocc3(o)
+        self.stream = stream
+        self.triples = canonical_triples(o)
+        self.blk_ijk = DeviceArray.from_host(np.ascontiguousarray(self.triples, dtype=np.int32))
+        self.pairs = [(p, q) for p in range(o) for q in range(p, o)]
+        self.npairs = len(self.pairs)
+        v3 = v ** 3
+        # --- device buffers ---
+        self.t3 = DeviceArray((n3, v, v, v))
+        self.R = DeviceArray((n3, v, v, v))
+        self.Y = DeviceArray((self.npairs, o, v, v, v))
+        self.t2 = DeviceArray((o, o, v, v))
+        self.F_oo = DeviceArray((o, o))
+        self.F_vv = DeviceArray((v, v))
+        self.W_oooo = DeviceArray((o, o, o, o))
+        self.W_ovvo = DeviceArray((o, v, v, o))
+        self.W_ovov = DeviceArray((o, v, o, v))
+        self.W_vooo = DeviceArray((v, o, o, o))
+        self.W_vvvo = DeviceArray((v, v, v, o))
+        self.W_vvvv = DeviceArray((v, v, v, v))
+        self.WR1 = DeviceArray((o, v, o, v))
+        self.WR3a = DeviceArray((o, v, o, v))
+        self.WR3b = DeviceArray((o, v, o, v))
+        self.pfs = DeviceArray((6, n3, n3))
+        self.eia = DeviceArray((o, v))
+        self.norb = int(L._lib.dcc_r3_finalize_tiles(v))
+        self.partials = DeviceArray((n3 * self.norb, 2))
+        self.norms = DeviceArray((2,))
+        # --- Y materialization jobs: Y[(p,q)][l] = T[l,p,q] ---
+        jobs = []
+        for pq, (p, q) in enumerate(self.pairs):
+            for l in range(o):
+                case, (ci, cj, ck) = canonical_case(l, p, q)
+                coef = np.zeros(6)
+                coef[case] = 1.0
+                jobs.append(((pq * o + l) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
+        self.y_jobs = DeviceArray.from_host(s3_jobs_array(jobs))
+        self.n_y_jobs = len(jobs)
+        self._tables = []
+        self.launches = self._plan_residual()
+
+    # ------------------------------------------------------------------ plan
+    def _plan_residual(self):
+        o, v, n3 = self.nocc, self.nvir, self.n3
+        ext = {x: v for x in 'abcdexyz'}
+        ext.update({'l': o, 'm': o, 'r': o, 'B': n3, 'p': n3, 'q': n3})
+        T = Operand(self.t3, 'Bxyz', (n3, v, v, v))
+        launches = []
+        blocks = [dict(I=i, J=j, K=k, B=n) for n, (i, j, k) in enumerate(self.triples)]
+        # T2 -> T3 (rccsdt.py:388-399)
+        Rb = Operand(self.R, 'Babc', (n3, v, v, v))
+        vvvo_terms = [('abd,dc->abc', 'J', 'IK'), ('acd,db->abc', 'K', 'IJ'), ('bad,dc->abc', 'I', 'JK'),
+                      ('bcd,da->abc', 'K', 'JI'), ('cad,db->abc', 'I', 'KJ'), ('cbd,da->abc', 'J', 'KI')]
+        for sub, wocc, tocc in vvvo_terms:
+            sa, rest = sub.split(',')
+            sb = rest.split('->')[0]
+            A = Operand(self.W_vvvo, sa + 'w', (v, v, v, o))
+            B = Operand(self.t2, 'st' + sb, (o, o, v, v))
+            fixed = [dict(w=f[wocc], s=f[tocc[0]], t=f[tocc[1]], B=f['B']) for f in blocks]
+            launches.append(plan_term('T2T3:W_vvvo ' + sub, sub, A, B, Rb, 1.0, 1.0, fixed, ext))
+        vooo_terms = [('al,lbc->abc', 'IJ', 'K'), ('al,lcb->abc', 'IK', 'J'), ('bl,lac->abc', 'JI', 'K'),
+                      ('bl,lca->abc', 'JK', 'I'), ('cl,lab->abc', 'KI', 'J'), ('cl,lba->abc', 'KJ', 'I')]
+        for sub, wocc, tocc in vooo_terms:
+            sa, rest = sub.split(',')
+            sb = rest.split('->')[0]
+            A = Operand(self.W_vooo, sa + 'uw', (v, o, o, o))
+            B = Operand(self.t2, sb[0] + 's' + sb[1:], (o, o, v, v))
+            fixed = [dict(u=f[wocc[0]], w=f[wocc[1]], s=f[tocc], B=f['B']) for f in blocks]
+            launches.append(plan_term('T2T3:W_vooo ' + sub, sub, A, B, Rb, -1.0, 1.0, fixed, ext))
+        # F_vv and W_vvvv x T3 with the block index in N (rccsdt.py:400-402, 424-426)
+        one = [dict()]
+        for sub, tl in [('ad,Bdbc->Babc', 'Bdbc'), ('bd,Badc->Babc', 'Badc'), ('cd,Babd->Babc', 'Babd')]:
+            A = Operand(self.F_vv, sub[:2], (v, v))
+            B = Operand(self.t3, tl, (n3, v, v, v))
+            launches.append(plan_term('F_vv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
+        for sub, tl in [('abde,Bdec->Babc', 'Bdec'), ('acde,Bdbe->Babc', 'Bdbe'), ('bcde,Bade->Babc', 'Bade')]:
+            A = Operand(self.W_vvvv, sub[:4], (v, v, v, v))
+            B = Operand(self.t3, tl, (n3, v, v, v))
+            launches.append(plan_term('W_vvvv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
+        launches += self._plan_ring()
+        # W_oooo / F_oo (rccsdt.py:726-731)
+        pf_terms = ['pq,pabc->qabc', 'pq,pacb->qabc', 'pq,pbac->qabc',
+                    'pq,pabc->qcab', 'pq,pabc->qbca', 'pq,pabc->qcba']
+        for s, sub in enumerate(pf_terms):
+            sa, rest = sub.split(',')
+            sb, sc = rest.split('->')
+            A = Operand(self.pfs, 'Spq', (6, n3, n3))
+            B = Operand(self.t3, sb, (n3, v, v, v))
+            C = Operand(self.R, sc, (n3, v, v, v))
+            launches.append(plan_term('W_oooo/F_oo ' + sub, sub, A, B, C, 1.0, 1.0, [dict(S=s)], ext))
+        return launches
+
+    def _plan_ring(self):
+        """Ring terms (rccsdt.py:540-596) as 5 GEMMs x 3 parts, batched over pairs."""
+        o, v, n3 = self.nocc, self.nvir, self.n3
+        v2, v3 = v * v, v ** 3
+        ov = o * v
+        # where element [x0,x1,x2] of G[r;pq] lands in the target block, per part
+        place = {3: (v2, v, 1), 4: (v, v2, 1), 5: (1, v2, v)}
+        # rows r of pair (p,q) owned by each part, and the target of row r
+        def rows(part, p, q):
+            return {3: (0, p + 1), 4: (p, q + 1), 5: (q, o)}[part]
+
+        def target(part, r, p, q):
+            t = {3: (r, p, q), 4: (p, r, q), 5: (p, q, r)}[part]
+            return ijk_to_linear(t[0], t[1], t[2], o)
+
+        # B operand per term: Y[l][...] strides for (l,d) and the two N axes
+        # term: (weight, G axis carried by M, N axes (outer, inner), B strides of d, n_outer, n_inner)
+        terms = [
+            ('WR1', 1, (0, 2), v, v2, 1),    # -W_ovov[l,b,r,d] Y[l][a,d,c]
+            ('WR1', 2, (0, 1), 1, v2, v),    # -W_ovov[l,c,r,d] Y[l][a,b,d]
+            ('WR3a', 0, (1, 2), v2, v, 1),   #  W_ovvo[l,a,d,r] Y[l][d,b,c]
+            ('WR3b', 0, (1, 2), v, v2, 1),   # -1/2(..) Y[l][b,d,c]
+            ('WR3b', 0, (1, 2), 1, v, v2),   # -1/2(..) Y[l][c,b,d]
+        ]
+        launches = []
+        for part in (3, 4, 5):
+            S = place[part]
+            for wname, mx, (n0, n1), sd, sn0, sn1 in terms:
+                # per-pair row table: cm[pair][(r,x)] = target(r)*v^3 + x*S[mx]
+                tbl = np.zeros((self.npairs, ov), dtype=np.int64)
+                batches = []
+                for pq, (p, q) in enumerate(self.pairs):
+                    r0, r1 = rows(part, p, q)
+                    for r in range(r0, r1):
+                        tbl[pq, r * v:(r + 1) * v] = target(part, r, p, q) * v3 + np.arange(v) * S[mx]
+                    batches.append((0, pq * o * v3, 0, r0 * v, r1 * v))
+                tdev = DeviceArray.from_host(tbl)
+                self._tables.append(tdev)
+                W = getattr(self, wname)
+                am = L.IdxMap(None, 0, 1, ov, 0, 0, ov)          # row (r,x) of the (ov x ov) weight
+                ak = L.IdxMap(None, 0, 1, ov, 0, 0, 1)
+                bk = L.IdxMap(None, 0, o, v, 0, v3, sd)          # (l, d)
+                bn = L.IdxMap(None, 0, v, v, 0, sn0, sn1)        # (n_outer, n_inner)
+                cm = table_map(tdev, ov)
+                cn = L.IdxMap(None, 0, v, v, 0, S[n0], S[n1])
+                nrows = sum(b[4] - b[3] for b in batches)
+                launches.append(GemmLaunch('ring part%d %s M%d' % (part, wname, mx), ov, v2, ov, 1.0, 1.0,
+                                           W, am, ak, self.Y, bk, bn, self.R, cm, cn, batches,
+                                           max(b[4] - b[3] for b in batches), 2.0 * nrows * v2 * ov,
+                                           keep=(tdev,)))
+        return launches
+
+    # --------------------------------------------------------------- inputs
+    def set_t3(self, t3_local):
+        self.t3.upload(np.ascontiguousarray(t3_local, dtype=np.float64), self.stream)
+
+    def get_t3(self):
+        return self.t3.download(stream=self.stream)
+
+    def set_intermediates(self, imds, t2):
+        o, v = self.nocc, self.nvir
+        for name in ('F_oo', 'F_vv', 'W_oooo', 'W_ovvo', 'W_ovov', 'W_vooo', 'W_vvvo', 'W_vvvv'):
+            getattr(self, name).upload(np.ascontiguousarray(imds[name], dtype=np.float64), self.stream)
+        self.t2.upload(np.ascontiguousarray(t2, dtype=np.float64), self.stream)
+        s = self.stream.handle if self.stream is not None else None
+        L.call("dcc_ring_weights_f64", self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
+               self.WR3b.vp, s)
+        L.call("dcc_oooo_prefactors_f64", self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
+               self.pfs.vp, s)
+
+    # -------------------------------------------------------------- compute
+    def materialize_y(self):
+        s = self.stream.handle if self.stream is not None else None
+        L.call("dcc_s3_transform_f64", self.Y.vp, self.t3.vp, self.nvir, 0.0, self.y_jobs.vp, self.n_y_jobs, s)
+
+    def residual(self):
+        """R <- r3 of distr_cc/rccsdt.py:368-433 for the current t3 and intermediates."""
+        self.R.fill(0.0, self.stream)
+        self.materialize_y()
+        for g in self.launches:
+            g.run(self.stream)
+
+    def finalize(self, mo_energy, level_shift=0.0, nvir_diis=None, r3_out=None):
+        """rccsdt.py:947-971 fused: t3 += P3_full-projected R / D; returns (norm^2, active norm^2)."""
+        o, v = self.nocc, self.nvir
+        e = np.asarray(mo_energy, dtype=np.float64)
+        eia = e[:o, None] - e[None, o:] - level_shift
+        self.eia.upload(np.ascontiguousarray(eia), self.stream)
+        nd = v if nvir_diis is None else int(nvir_diis)
+        s = self.stream.handle if self.stream is not None else None
+        L.call("dcc_r3_finalize_f64", self.R.vp, self.t3.vp, r3_out.vp if r3_out is not None else None,
+               self.n3, v, self.blk_ijk.vp, self.eia.vp, o, nd, self.partials.vp, s)
+        L.call("dcc_reduce_pairs_f64", self.partials.vp, self.n3 * self.norb, self.norms.vp, s)
+        n = self.norms.download(stream=self.stream)
+        return float(n[0]), float(n[1])
+
+    def residual_flops(self):
+        return sum(g.flops for g in self.launches)

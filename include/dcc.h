@@ -1,0 +1,168 @@
+/*
+ * dcc.h -- C ABI of libdcc.so, the MI355X (gfx950) triples engine.
+ *
+ * Plain C: device pointers are `void*` / `double*`, sizes are int64_t, streams
+ * are opaque `dcc_stream_t` handles (a hipStream_t underneath; NULL = the
+ * default stream).  Every function returns 0 on success and a nonzero code on
+ * failure; `dcc_last_error()` then returns a message for the calling thread.
+ * All buffers are caller-owned device memory (from dcc_malloc or any HIP
+ * allocator); no entry point allocates except where stated.
+ *
+ * Reference interfaces each entry point replaces are cited as file:line of
+ * jinyuchem/Distributed-CC.  The reference binds `void f(...)` C helpers via
+ * ctypes (distr_cc/_lib.py:19, distr_cc/_c_rccsdt.py:23-175); here the same
+ * role is played by device-side entry points taking whole canonical-block
+ * arrays instead of one v^3 block per call.
+ */
+#ifndef DCC_H
+#define DCC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *dcc_stream_t;
+typedef void *dcc_event_t;
+
+/* ---------------------------------------------------------------- runtime */
+
+/* Library version, e.g. 1 for this ABI. */
+int dcc_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *dcc_last_error(void);
+
+/* Number of visible HIP devices (0 when none; never an error). */
+int dcc_device_count(int *count);
+
+/* Select the device for this thread and report its gfx arch name (>= 32 bytes). */
+int dcc_set_device(int device, char *arch_name, int64_t arch_len);
+
+int dcc_malloc(void **ptr, int64_t bytes);
+int dcc_free(void *ptr);
+int dcc_memcpy_htod(void *dst, const void *src, int64_t bytes, dcc_stream_t stream);
+int dcc_memcpy_dtoh(void *dst, const void *src, int64_t bytes, dcc_stream_t stream);
+int dcc_memcpy_dtod(void *dst, const void *src, int64_t bytes, dcc_stream_t stream);
+/* Fill with a double value using a kernel (safe inside captured graphs). */
+int dcc_fill_f64(double *dst, int64_t n, double value, dcc_stream_t stream);
+int dcc_stream_create(dcc_stream_t *stream);
+int dcc_stream_destroy(dcc_stream_t stream);
+int dcc_stream_synchronize(dcc_stream_t stream);
+int dcc_device_synchronize(void);
+int dcc_event_create(dcc_event_t *event);
+int dcc_event_destroy(dcc_event_t event);
+int dcc_event_record(dcc_event_t event, dcc_stream_t stream);
+int dcc_event_elapsed_ms(float *ms, dcc_event_t start, dcc_event_t stop);
+
+/* ------------------------------------------------- contraction (MFMA FP64) */
+
+/*
+ * Index map of one GEMM dimension into one operand: an index x in [0, extent)
+ * maps to the element offset
+ *     table ? table[batch * table_bstride + x]
+ *           : (x / (e1*e2)) * s0 + ((x / e2) % e1) * s1 + (x % e2) * s2
+ * i.e. either a lookup table or a three-level mixed-radix decomposition.
+ */
+typedef struct dcc_idxmap {
+    const int64_t *table;
+    int64_t table_bstride;
+    int64_t e1, e2;
+    int64_t s0, s1, s2;
+} dcc_idxmap;
+
+/* One batch entry of a contraction: base offsets and the row range it owns. */
+typedef struct dcc_batch {
+    int64_t a_off, b_off, c_off;
+    int64_t m_begin, m_end;
+} dcc_batch;
+
+/*
+ * Batched gather-GEMM on FP64 MFMA (v_mfma_f64_16x16x4f64, LDS-staged tiles):
+ *   C[c_off + cm(m) + cn(n)] = beta*C[...] + alpha * sum_k A[a_off + am(m) + ak(k)]
+ *                                                        * B[b_off + bk(k) + bn(n)]
+ * for every batch entry b and m in [m_begin, m_end), n < N.  `batches` is a
+ * device array of nbatch entries.  Entries must write disjoint C elements.
+ * Replaces the per-block `einsum` calls of distr_cc/rccsdt.py:385-426,
+ * :540-596 and :726-731 (NumPy/BLAS GEMMs on the CPU).
+ */
+int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, double beta,
+                     const double *A, dcc_idxmap am, dcc_idxmap ak,
+                     const double *B, dcc_idxmap bk, dcc_idxmap bn,
+                     double *C, dcc_idxmap cm, dcc_idxmap cn,
+                     const dcc_batch *batches, int64_t nbatch, int64_t max_rows,
+                     dcc_stream_t stream);
+
+/* ---------------------------------------- S3 block transforms (HBM-bound) */
+
+/*
+ * One job of the S3 transform: out block at `dst`, source block at `src`
+ * (element offsets), and six coefficients c[p] for the six axis
+ * permutations p of (a,b,c) in the order
+ *   (0,1,2) (0,2,1) (1,0,2) (1,2,0) (2,0,1) (2,1,0)
+ * (the reference's tp_t3 order, src/rccsdt.c:32-39), meaning
+ *   out[x] = beta*out[x] + sum_p c[p] * src.transpose(p)[x].
+ */
+typedef struct dcc_s3_job {
+    int64_t dst, src;
+    double coef[6];
+} dcc_s3_job;
+
+/*
+ * Apply njobs S3 transforms to v^3 blocks.  Each workgroup stages the six
+ * permuted 8x8x8 tiles of one tile orbit in LDS so that every HBM read and
+ * write is along the contiguous axis.  Replaces take_t3_ijk_single_
+ * (src/rccsdt.c:119-129), t3_single_spin_summation_inplace_
+ * (src/rccsdt.c:131-233), t3_spin_summation_triple_sym_ (src/rccsdt.c:245-310)
+ * and t3_transpose_add_ (src/rccsdt.c:320-375); out and src must not overlap.
+ */
+int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double beta,
+                         const dcc_s3_job *jobs, int64_t njobs, dcc_stream_t stream);
+
+/*
+ * Fused end of the triples update (distr_cc/rccsdt.py:947-971):
+ *   r = (R - (1/6) sum_p R.transpose(p)) / (eia[i,a] + eia[j,b] + eia[k,c])
+ *   t3 += r;  optionally r3_out = r;
+ *   partial[blk, tile] = (sum r^2, sum over a,b,c < nvir_diis of r^2)
+ * R and t3 are (nblk, v, v, v); blk_ijk is a device int32 array (nblk, 3) of
+ * the canonical triples; eia is (nocc, v) = e_occ - e_vir - level_shift.
+ * `partials` must hold 2 * nblk * dcc_r3_finalize_tiles(nvir) doubles.
+ * Replaces PySCF's batched P3_full spin summation (rccsdt.py:947),
+ * r3_tri_divide_e_ (rccsdt.py:734-744) and the norms/update (rccsdt.py:957-971).
+ */
+int64_t dcc_r3_finalize_tiles(int64_t nvir);
+int dcc_r3_finalize_f64(const double *R, double *t3, double *r3_out, int64_t nblk, int64_t nvir,
+                        const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
+                        double *partials, dcc_stream_t stream);
+
+/* Deterministic fixed-order sum of `n` pairs (x[2i], x[2i+1]) into out[0..1]. */
+int dcc_reduce_pairs_f64(const double *x, int64_t n, double *out, dcc_stream_t stream);
+
+/* ------------------------------------------------ GEMM operand builders */
+
+/*
+ * pfs[s][p][q] (6 x nblk x nblk): coefficient of source canonical block p,
+ * transposed by permutation s, in target canonical block q for the W_oooo and
+ * F_oo terms.  Same coefficients as the prefactor loop of
+ * compute_r3_Foo_Woooo_contribution_batch_ (distr_cc/rccsdt.py:616-723),
+ * computed for all source/target pairs at once on the device.
+ */
+int dcc_oooo_prefactors_f64(const double *W_oooo, const double *F_oo, int64_t nocc, const int32_t *blk_ijk,
+                            int64_t nblk, double *pfs, dcc_stream_t stream);
+
+/*
+ * Ring-term weight matrices ((nocc*nvir) x (nocc*nvir), rows (r,x), cols (l,d)):
+ *   WR1 = -W_ovov[l,x,r,d], WR3a = W_ovvo[l,x,d,r],
+ *   WR3b = -(W_ovov[l,x,r,d] + W_ovvo[l,x,d,r]) / 2
+ * operands of the W_ovov/W_ovvo x T3 terms of distr_cc/rccsdt.py:540-596.
+ */
+int dcc_ring_weights_f64(const double *W_ovov, const double *W_ovvo, int64_t nocc, int64_t nvir, double *WR1,
+                         double *WR3a, double *WR3b, dcc_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DCC_H */

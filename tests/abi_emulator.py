@@ -1,0 +1,264 @@
+"""NumPy emulator of the libdcc C ABI (include/dcc.h) -- CPU test infrastructure only.
+
+It implements the documented semantics of each entry point on host memory so
+that the host-side planning (index maps, batch tables, S3 jobs) can be checked
+against the oracle without a GPU.  It says nothing about the HIP kernels
+themselves; those are checked by the ``-m gpu`` tests.
+"""
+import ctypes
+import itertools
+
+import numpy as np
+
+_PERMS = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+
+
+class FakeDevice:
+    def __init__(self):
+        self.allocs = {}  # base -> bytearray-backed numpy uint8
+        self.next = 1 << 40
+
+    def malloc(self, nbytes):
+        base = self.next
+        self.next += ((nbytes + 4095) // 4096 + 1) * 4096
+        self.allocs[base] = np.zeros(max(nbytes, 1), dtype=np.uint8)
+        return base
+
+    def resolve(self, ptr):
+        for base, buf in self.allocs.items():
+            if base <= ptr < base + buf.size:
+                return buf, ptr - base
+        raise KeyError("unknown device pointer %x" % ptr)
+
+    def view(self, ptr, dtype, count=None):
+        buf, off = self.resolve(int(ptr))
+        arr = buf[off:].view(np.uint8)
+        n = (arr.size // np.dtype(dtype).itemsize) if count is None else count
+        return arr[: n * np.dtype(dtype).itemsize].view(dtype)
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, ctypes.c_void_p):
+        return x.value
+    return int(x)
+
+
+def _map(dev, m, idx, batch):
+    idx = np.asarray(idx, dtype=np.int64)
+    if m.table:
+        tab = dev.view(m.table, np.int64)
+        return tab[batch * m.table_bstride + idx]
+    e12 = m.e1 * m.e2
+    q0 = idx // e12
+    r0 = idx - q0 * e12
+    q1 = r0 // m.e2
+    q2 = r0 - q1 * m.e2
+    return q0 * m.s0 + q1 * m.s1 + q2 * m.s2
+
+
+class Emulator:
+    def __init__(self):
+        self.dev = FakeDevice()
+        self.error = b""
+        self.calls = []
+
+    # each handler returns a status
+    def __call__(self, name, *args):
+        self.calls.append(name)
+        return getattr(self, name)(*args)
+
+    def dcc_device_count(self, count):
+        count._obj.value = 1
+        return 0
+
+    def dcc_set_device(self, device, buf, n):
+        return 0
+
+    def dcc_malloc(self, pptr, nbytes):
+        pptr._obj.value = self.dev.malloc(int(nbytes))
+        return 0
+
+    def dcc_free(self, p):
+        return 0
+
+    def dcc_memcpy_htod(self, dst, src, nbytes, stream):
+        d = self.dev.view(_ptr(dst), np.uint8, int(nbytes))
+        d[:] = np.ctypeslib.as_array(ctypes.cast(src, ctypes.POINTER(ctypes.c_uint8)), (int(nbytes),))
+        return 0
+
+    def dcc_memcpy_dtoh(self, dst, src, nbytes, stream):
+        s = self.dev.view(_ptr(src), np.uint8, int(nbytes))
+        h = np.ctypeslib.as_array(ctypes.cast(dst, ctypes.POINTER(ctypes.c_uint8)), (int(nbytes),))
+        h[:] = s
+        return 0
+
+    def dcc_memcpy_dtod(self, dst, src, nbytes, stream):
+        self.dev.view(_ptr(dst), np.uint8, int(nbytes))[:] = self.dev.view(_ptr(src), np.uint8, int(nbytes))
+        return 0
+
+    def dcc_fill_f64(self, dst, n, value, stream):
+        self.dev.view(_ptr(dst), np.float64, int(n))[:] = value
+        return 0
+
+    def dcc_stream_synchronize(self, s):
+        return 0
+
+    def dcc_device_synchronize(self):
+        return 0
+
+    def dcc_contract_f64(self, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
+                         stream):
+        dev = self.dev
+        Av, Bv, Cv = dev.view(_ptr(A), np.float64), dev.view(_ptr(B), np.float64), dev.view(_ptr(C), np.float64)
+        bt = dev.view(_ptr(batches), np.int64, 5 * int(nbatch)).reshape(-1, 5)
+        ks = np.arange(K)
+        ns = np.arange(N)
+        for b in range(int(nbatch)):
+            a_off, b_off, c_off, mb, me = (int(x) for x in bt[b])
+            ms = np.arange(mb, me)
+            if ms.size == 0:
+                continue
+            ai = a_off + _map(dev, am, ms, b)[:, None] + _map(dev, ak, ks, b)[None, :]
+            bi = b_off + _map(dev, bk, ks, b)[:, None] + _map(dev, bn, ns, b)[None, :]
+            ci = c_off + _map(dev, cm, ms, b)[:, None] + _map(dev, cn, ns, b)[None, :]
+            prod = Av[ai] @ Bv[bi] if K > 0 else np.zeros(ci.shape)
+            old = Cv[ci] if beta != 0.0 else 0.0
+            Cv[ci] = alpha * prod + beta * old
+        return 0
+
+    def dcc_s3_transform_f64(self, out, src, nvir, beta, jobs, njobs, stream):
+        v = int(nvir)
+        O = self.dev.view(_ptr(out), np.float64)
+        S = self.dev.view(_ptr(src), np.float64)
+        raw = self.dev.view(_ptr(jobs), np.uint8, int(njobs) * 64)
+        jb = raw.view(np.dtype([("dst", np.int64), ("src", np.int64), ("coef", np.float64, (6,))]))
+        for j in jb:
+            x = S[int(j["src"]): int(j["src"]) + v ** 3].reshape(v, v, v)
+            val = sum(c * x.transpose(p) for c, p in zip(j["coef"], _PERMS) if c != 0.0)
+            o = O[int(j["dst"]): int(j["dst"]) + v ** 3].reshape(v, v, v)
+            o[...] = (beta * o if beta != 0.0 else 0.0) + val
+        return 0
+
+    def dcc_r3_finalize_f64(self, R, t3, r3_out, nblk, nvir, blk_ijk, eia, nocc, nvir_diis, partials, stream):
+        v, n3, o = int(nvir), int(nblk), int(nocc)
+        Rv = self.dev.view(_ptr(R), np.float64, n3 * v ** 3).reshape(n3, v, v, v)
+        Tv = self.dev.view(_ptr(t3), np.float64, n3 * v ** 3).reshape(n3, v, v, v)
+        ijk = self.dev.view(_ptr(blk_ijk), np.int32, 3 * n3).reshape(n3, 3)
+        e = self.dev.view(_ptr(eia), np.float64, o * v).reshape(o, v)
+        sym = sum(Rv.transpose(0, *(1 + np.array(p))) for p in _PERMS)
+        r = Rv - sym / 6.0
+        d = e[ijk[:, 0], :, None, None] + e[ijk[:, 1], None, :, None] + e[ijk[:, 2], None, None, :]
+        r = r / d
+        Tv += r
+        if _ptr(r3_out):
+            self.dev.view(_ptr(r3_out), np.float64, n3 * v ** 3)[:] = r.ravel()
+        nd = int(nvir_diis)
+        self._norms = (float((r * r).sum()), float((r[:, :nd, :nd, :nd] ** 2).sum()))
+        P = self.dev.view(_ptr(partials), np.float64, 2)
+        P[0], P[1] = self._norms
+        self._npart = 1
+        return 0
+
+    def dcc_reduce_pairs_f64(self, x, n, out, stream):
+        X = self.dev.view(_ptr(x), np.float64, 2 * int(n)).reshape(-1, 2)
+        # finalize wrote only the first pair in the emulator
+        o = self.dev.view(_ptr(out), np.float64, 2)
+        o[:] = X[0] if getattr(self, "_npart", 0) == 1 else X.sum(0)
+        return 0
+
+    def dcc_r3_finalize_tiles(self, nvir):
+        nt = (int(nvir) + 7) // 8
+        return nt * (nt + 1) * (nt + 2) // 6
+
+    def dcc_oooo_prefactors_f64(self, W, F, nocc, blk_ijk, nblk, pfs, stream):
+        o, n3 = int(nocc), int(nblk)
+        Wv = self.dev.view(_ptr(W), np.float64, o ** 4).reshape(o, o, o, o)
+        Fv = self.dev.view(_ptr(F), np.float64, o * o).reshape(o, o)
+        ijk = self.dev.view(_ptr(blk_ijk), np.int32, 3 * n3).reshape(n3, 3)
+        P = self.dev.view(_ptr(pfs), np.float64, 6 * n3 * n3).reshape(6, n3, n3)
+        P[...] = 0.0
+        # dense definition: coefficient of T[p]^s in R[q] collected from the
+        # full-tensor W_oooo / F_oo terms by enumerating ordered (l,m) pairs
+        lin = {tuple(int(x) for x in t): n for n, t in enumerate(ijk)}
+
+        def add(q_abc_perm, srcs, coef, q):
+            # srcs: ordered occupied triple of the source, q_abc_perm: how target abc reads source
+            s_sorted = tuple(sorted(srcs))
+            p = lin[s_sorted]
+            # T[srcs][x] = Tc[p].transpose(case)[x]; combined with the target read permutation
+            case = _case(*srcs)
+            P[_compose(case, q_abc_perm), p, q] += coef
+
+        for q, (i, j, k) in enumerate(ijk):
+            for l, m in itertools.product(range(o), range(o)):
+                add((0, 1, 2), (l, m, k), Wv[l, m, i, j], q)
+                add((0, 2, 1), (l, m, j), Wv[l, m, i, k], q)
+                add((2, 0, 1), (l, m, i), Wv[l, m, j, k], q)
+            for l in range(o):
+                add((0, 1, 2), (l, j, k), -Fv[l, i], q)
+                add((0, 1, 2), (i, l, k), -Fv[l, j], q)
+                add((0, 1, 2), (i, j, l), -Fv[l, k], q)
+        return 0
+
+    def dcc_ring_weights_f64(self, Wovov, Wovvo, nocc, nvir, WR1, WR3a, WR3b, stream):
+        o, v = int(nocc), int(nvir)
+        A = self.dev.view(_ptr(Wovov), np.float64, (o * v) ** 2).reshape(o, v, o, v)
+        B = self.dev.view(_ptr(Wovvo), np.float64, (o * v) ** 2).reshape(o, v, v, o)
+        ovov = A.transpose(2, 1, 0, 3)     # [r,x,l,d] = W_ovov[l,x,r,d]
+        ovvo = B.transpose(3, 1, 0, 2)     # [r,x,l,d] = W_ovvo[l,x,d,r]
+        self.dev.view(_ptr(WR1), np.float64, (o * v) ** 2)[:] = (-ovov).ravel()
+        self.dev.view(_ptr(WR3a), np.float64, (o * v) ** 2)[:] = ovvo.ravel()
+        self.dev.view(_ptr(WR3b), np.float64, (o * v) ** 2)[:] = (-0.5 * (ovov + ovvo)).ravel()
+        return 0
+
+
+def _case(i, j, k):
+    if i <= j <= k:
+        return 0
+    if i <= k < j:
+        return 1
+    if j < i <= k:
+        return 2
+    if k < i <= j:
+        return 3
+    if j <= k < i:
+        return 4
+    return 5
+
+
+def _compose(case, read):
+    """Index s with  Tc.transpose(P[case]).transpose(read) == Tc.transpose(P[s]) ... output side.
+
+    R[q][a,b,c] += T_src.transpose(read)[a,b,c] where T_src = Tc.transpose(P[case]).
+    The launches apply output permutations; the emulator maps onto the six
+    (source-transpose) GEMM forms of rccsdt.py:726-731 by composing.
+    """
+    p = _PERMS[case]
+    total = tuple(p[r] for r in read)
+    # GEMM form s adds Tc.transpose(gemm_form[s]) to R[q]:
+    # 'pq,pabc->qcab' puts Tc[a,b,c] at R[c,a,b], i.e. R[x,y,z] += Tc[y,z,x] = Tc.transpose(2,0,1)
+    # 'pq,pabc->qbca' puts Tc[a,b,c] at R[b,c,a], i.e. R[x,y,z] += Tc[z,x,y] = Tc.transpose(1,2,0)
+    gemm_form = {0: (0, 1, 2), 1: (0, 2, 1), 2: (1, 0, 2), 3: (2, 0, 1), 4: (1, 2, 0), 5: (2, 1, 0)}
+    for s, f in gemm_form.items():
+        if f == total:
+            return s
+    raise AssertionError(total)
+
+
+def install(monkeypatch):
+    """Route distr_cc._lib.call (and the raw finalize-tiles query) through a fresh emulator."""
+    from distr_cc import _lib as L
+    from distr_cc import device
+
+    emu = Emulator()
+    monkeypatch.setattr(L, "call", lambda name, *a: L.check(emu(name, *a), name))
+
+    class _Raw:
+        def __getattr__(self, name):
+            return getattr(emu, name)
+
+    monkeypatch.setattr(L, "_lib", _Raw())
+    monkeypatch.setattr(device, "_DEVICE_READY", {0: "gfx950-emulated"})
+    return emu
