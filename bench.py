@@ -1,0 +1,186 @@
+#!/usr/bin/env python
+"""Benchmark of the FP64 RCCSDT triples update on one MI355X.
+
+Workload (default): the synthetic plain-FP64 T3 residual at nocc=10, nvir=90
+(220 canonical blocks of 90^3, naux=300 B-factor ERIs, seed 20261202).  One
+step = one full triples update on the device: the T3 residual (T2->T3,
+F_vv, W_vvvv, W_ovov/W_ovvo, W_oooo/F_oo contractions) followed by the fused
+P3_full projection, denominator, norms and t3 += r3, with every input resident
+in HBM before the timed region.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "distributed-cc_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "RCCSDT iteration time, mixed vs FP64 at |ΔE|≤1 µEh; % MFMA peak; (Q) time"
+PEAK_FP64_MFMA_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def algorithmic_flops(o, v):
+    """Flops of one T3 residual with canonical-block savings (reference einsum count)."""
+    o3u = o * (o + 1) * (o + 2) // 6
+    return o3u * (6 * v ** 5 + 24 * o * v ** 4 + 18 * v ** 4 + 12 * o * v ** 3 + 6 * o * o * v ** 3
+                  + 6 * o * v ** 3) + o ** 3 * (4 * v ** 4 + (4 * o + 2) * v ** 3)
+
+
+def cpu_baseline(o, v, sysm, im, t2, t3, budget_s=25.0):
+    """Time the oracle (reference algorithm, NumPy/BLAS FP64) on a bounded sample of the
+    same residual: n and 2n target blocks, extrapolated linearly to all blocks."""
+    from oracle import rccsdt_tri as tri
+    n3 = t3.shape[0]
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+    except Exception:
+        threads = len(os.sched_getaffinity(0))
+
+    eia = sysm.mo_energy[:o, None] - sysm.mo_energy[None, o:]
+    triples = tri.canonical_triples(o)
+
+    def run(n):
+        t0 = time.perf_counter()
+        r3 = tri.compute_r3_tri(o, v, im, t2, t3, targets=range(n))
+        proj = tri.project_p3_full(r3[:n])
+        for idx in range(n):
+            i, j, k = triples[idx]
+            d = eia[i, :, None, None] + eia[j, None, :, None] + eia[k, None, None, :]
+            _ = t3[idx] + proj[idx] / d
+        return time.perf_counter() - t0
+
+    n1 = 1
+    t1 = run(n1)
+    n2 = max(3, min(n3, int(n1 + max(1.0, (budget_s * 0.7 - t1) / max(t1, 1e-3)))))
+    t2s = run(n2)
+    slope = max((t2s - t1) / (n2 - n1), 1e-9)
+    fixed = max(t1 - slope * n1, 0.0)
+    est = fixed + slope * n3
+    return {"value": est, "unit": "s", "cores": int(threads), "kind": "port",
+            "sample": "oracle block-loop residual+update timed on %d and %d of %d target blocks "
+                      "(%.1f s, %.1f s), extrapolated linearly to all blocks" % (n1, n2, n3, t1, t2s)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nocc", type=int, default=10)
+    ap.add_argument("--nvir", type=int, default=90)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--breakdown", action="store_true", help="print per-launch timings to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the triples path on one GPU; --gpus %d is not supported "
+              "(sharding over canonical blocks is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    from distr_cc.device import Event, Stream, init
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+
+    o, v = args.nocc, args.nvir
+    init(0)
+    print("bench: building synthetic system o=%d v=%d" % (o, v), file=sys.stderr, flush=True)
+    sysm = SyntheticSystem(o, v, naux=300, seed=20261202)
+    im = sysm.bare_intermediates()
+    t2, t3 = sysm.t2(), sysm.t3_tri()
+    stream = Stream()
+    eng = TriplesEngine(o, v, stream=stream)
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    stream.synchronize()
+
+    # dominant kernel: the three W_vvvv x T3 GEMMs (65% of the algorithmic flops)
+    dom = [g for g in eng.launches if g.name.startswith("W_vvvv")]
+    ev = {}
+
+    def timer(g, what):
+        if g in dom or args.breakdown:
+            e = Event()
+            e.record(stream)
+            ev.setdefault(id(g), []).append((what, e))
+
+    print("bench: warmup %d" % args.warmup, file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        eng.residual()
+        eng.finalize(sysm.mo_energy)
+    stream.synchronize()
+    ev.clear()
+    e0, e1 = Event(), Event()
+    print("bench: timing %d steps" % args.steps, file=sys.stderr, flush=True)
+    w0 = time.perf_counter()
+    e0.record(stream)
+    norms = None
+    for _ in range(args.steps):
+        eng.residual(timer=timer)
+        norms = eng.finalize(sysm.mo_energy)
+    e1.record(stream)
+    stream.synchronize()
+    wall = time.perf_counter() - w0
+    ms = e0.elapsed_ms(e1) / args.steps
+
+    durs = {}
+    for g in eng.launches:
+        lst = ev.get(id(g), [])
+        starts = [e for w, e in lst if w == "start"]
+        ends = [e for w, e in lst if w == "end"]
+        durs[g.name] = [a.elapsed_ms(b) for a, b in zip(starts, ends)]
+    dom_ms = [d for g in dom for d in durs[g.name]]
+    dom_avg_ms = float(np.mean(dom_ms))
+    dom_flops = float(np.mean([g.flops for g in dom]))
+    achieved = dom_flops / (dom_avg_ms * 1e-3) / 1e12
+    if args.breakdown:
+        tot = 0.0
+        for g in eng.launches:
+            d = np.mean(durs[g.name]) if durs[g.name] else float("nan")
+            tot += d
+            print("  %-36s %9.3f ms  %7.2f TF/s" % (g.name, d, g.flops / (d * 1e-3) / 1e12), file=sys.stderr)
+        print("  sum of GEMM launches %.3f ms of %.3f ms per step" % (tot, ms), file=sys.stderr)
+
+    F = algorithmic_flops(o, v)
+    out = {
+        "metric": METRIC,
+        "value": ms / 1e3,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain-FP64 RCCSDT T3 residual + projection/denominator/update, "
+                               "nocc=%d nvir=%d, %d canonical blocks" % (o, v, eng.n3),
+                   "nocc": o, "nvir": v, "naux": 300, "seed": 20261202, "precision": "fp64",
+                   "residual_tflops": F / (ms * 1e-3) / 1e12,
+                   "residual_frac_fp64_peak": F / (ms * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,
+                   "r3_norm_sq": norms[0], "wall_s": wall},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_MFMA_TFLOPS, "traffic": None,
+                     "kernel": "contract_f64_kernel (W_vvvv x T3 launches)",
+                     "flops_per_launch": dom_flops, "avg_launch_ms": dom_avg_ms},
+    }
+    if not args.no_cpu_baseline:
+        print("bench: CPU baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(o, v, sysm, im, t2, t3)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

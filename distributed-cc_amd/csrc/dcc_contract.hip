@@ -12,8 +12,14 @@
 //
 // Tile: BM x BN per workgroup of 256 threads (4 wave64), each wave owning a
 // (BM/2) x (BN/2) sub-tile as (BM/32) x (BN/32) MFMA 16x16 blocks; BK = 16
-// with a double-buffered LDS pipeline (one barrier per K step).
+// with a double-buffered LDS pipeline (one barrier per K step).  Every thread
+// loads its A and B elements at one fixed k row of the tile, so the K index
+// map is evaluated once per operand and then advanced by BK with carries --
+// no integer division inside the K loop.
 #include "dcc_common.h"
+
+#include <cstdio>
+#include <cstdlib>
 
 namespace {
 
@@ -34,15 +40,52 @@ __device__ __forceinline__ int64_t map_off(const dcc_idxmap &mp, uint32_t x, int
     return (int64_t)q0 * mp.s0 + (int64_t)q1 * mp.s1 + (int64_t)q2 * mp.s2;
 }
 
+// K index map in incremental form: offset of k, advanced by kBK per step.
+struct KMap {
+    int64_t s0, s1, s2;   // strides of the three levels
+    int64_t step;         // kBK * s2
+    int64_t carry1;       // s1 - e2 * s2  (q2 wraps into q1)
+    int64_t carry0;       // s0 - e1 * s1  (q1 wraps into q0)
+    uint32_t e1, e2;
+};
+
+struct KIter {
+    uint32_t q1, q2;
+    int64_t off;
+};
+
+__device__ __forceinline__ KIter kinit(const KMap &m, uint32_t k) {
+    const uint32_t e12 = m.e1 * m.e2;
+    const uint32_t q0 = k / e12;
+    const uint32_t r0 = k - q0 * e12;
+    KIter it;
+    it.q1 = r0 / m.e2;
+    it.q2 = r0 - it.q1 * m.e2;
+    it.off = (int64_t)q0 * m.s0 + (int64_t)it.q1 * m.s1 + (int64_t)it.q2 * m.s2;
+    return it;
+}
+
+__device__ __forceinline__ void kadvance(const KMap &m, KIter &it) {
+    it.q2 += kBK;
+    it.off += m.step;
+    while (it.q2 >= m.e2) {  // at most ceil(kBK / e2) passes
+        it.q2 -= m.e2;
+        it.off += m.carry1;
+        if (++it.q1 >= m.e1) {
+            it.q1 = 0;
+            it.off += m.carry0;
+        }
+    }
+}
+
 struct ContractArgs {
     int M, N, K;
     double alpha, beta;
     const double *A;
-    dcc_idxmap am, ak;
     const double *B;
-    dcc_idxmap bk, bn;
     double *C;
-    dcc_idxmap cm, cn;
+    dcc_idxmap am, bn, cm, cn;
+    KMap ak, bk;
     const dcc_batch *batches;
     int tiles_m, tiles_n;
     int64_t total_tiles;
@@ -59,7 +102,7 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t L, int64_t total) {
 }
 
 template <int BM, int BN, bool A_KFAST, bool B_NFAST>
-__global__ __launch_bounds__(kThreads) void contract_f64_kernel(ContractArgs p) {
+__global__ __launch_bounds__(kThreads) void contract_f64_kernel(const ContractArgs p) {
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int LDA = BM + 1, LDB = BN + 1;
@@ -88,62 +131,51 @@ __global__ __launch_bounds__(kThreads) void contract_f64_kernel(ContractArgs p) 
     const int64_t m_end = bt.m_end;
     const int n0 = tn * BN;
 
-    // ---- loader coordinates (fixed per thread) ----
-    int a_mm[A_PER_T], a_kk[A_PER_T];
+    // ---- loader coordinates: one k row per thread and operand ----
+    // K-fast: 16 consecutive lanes walk k; M/N-fast: 16 consecutive lanes walk m/n.
+    const int a_kk = A_KFAST ? (tid % kBK) : (tid / 16);
+    const int a_m_base = A_KFAST ? (tid / kBK) : (tid % 16);
+    const int b_kk = B_NFAST ? (tid / 16) : (tid % kBK);
+    const int b_n_base = B_NFAST ? (tid % 16) : (tid / kBK);
     int64_t a_row[A_PER_T];
-    bool a_ok[A_PER_T];
+    uint32_t a_ok = 0;
 #pragma unroll
     for (int r = 0; r < A_PER_T; ++r) {
-        if (A_KFAST) {
-            a_kk[r] = tid % kBK;
-            a_mm[r] = tid / kBK + r * (kThreads / kBK);
-        } else {
-            a_mm[r] = tid % BM;
-            a_kk[r] = tid / BM + r * (kThreads / BM);
-        }
-        const int64_t m = m0 + a_mm[r];
-        a_ok[r] = m < m_end;
-        a_row[r] = a_ok[r] ? bt.a_off + map_off(p.am, (uint32_t)m, batch) : 0;
+        const int64_t m = m0 + a_m_base + 16 * r;
+        const bool ok = m < m_end;
+        a_ok |= (ok ? 1u : 0u) << r;
+        a_row[r] = ok ? bt.a_off + map_off(p.am, (uint32_t)m, batch) : 0;
     }
-    int b_nn[B_PER_T], b_kk[B_PER_T];
     int64_t b_col[B_PER_T];
-    bool b_ok[B_PER_T];
+    uint32_t b_ok = 0;
 #pragma unroll
     for (int r = 0; r < B_PER_T; ++r) {
-        if (B_NFAST) {
-            b_nn[r] = tid % BN;
-            b_kk[r] = tid / BN + r * (kThreads / BN);
-        } else {
-            b_kk[r] = tid % kBK;
-            b_nn[r] = tid / kBK + r * (kThreads / kBK);
-        }
-        const int n = n0 + b_nn[r];
-        b_ok[r] = n < p.N;
-        b_col[r] = b_ok[r] ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
+        const int n = n0 + b_n_base + 16 * r;
+        const bool ok = n < p.N;
+        b_ok |= (ok ? 1u : 0u) << r;
+        b_col[r] = ok ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
     }
+    KIter ka = kinit(p.ak, (uint32_t)a_kk);
+    KIter kb = kinit(p.bk, (uint32_t)b_kk);
 
     double a_reg[A_PER_T], b_reg[B_PER_T];
     auto load_tiles = [&](int k0) {
+        const bool ak_ok = k0 + a_kk < p.K;
+        const bool bk_ok = k0 + b_kk < p.K;
 #pragma unroll
-        for (int r = 0; r < A_PER_T; ++r) {
-            const int k = k0 + a_kk[r];
-            double v = 0.0;
-            if (a_ok[r] && k < p.K) v = p.A[a_row[r] + map_off(p.ak, (uint32_t)k, batch)];
-            a_reg[r] = v;
-        }
+        for (int r = 0; r < A_PER_T; ++r)
+            a_reg[r] = (ak_ok && ((a_ok >> r) & 1u)) ? p.A[a_row[r] + ka.off] : 0.0;
 #pragma unroll
-        for (int r = 0; r < B_PER_T; ++r) {
-            const int k = k0 + b_kk[r];
-            double v = 0.0;
-            if (b_ok[r] && k < p.K) v = p.B[b_col[r] + map_off(p.bk, (uint32_t)k, batch)];
-            b_reg[r] = v;
-        }
+        for (int r = 0; r < B_PER_T; ++r)
+            b_reg[r] = (bk_ok && ((b_ok >> r) & 1u)) ? p.B[b_col[r] + kb.off] : 0.0;
+        kadvance(p.ak, ka);
+        kadvance(p.bk, kb);
     };
     auto store_tiles = [&](int buf) {
 #pragma unroll
-        for (int r = 0; r < A_PER_T; ++r) As[buf][a_kk[r]][a_mm[r]] = a_reg[r];
+        for (int r = 0; r < A_PER_T; ++r) As[buf][a_kk][a_m_base + 16 * r] = a_reg[r];
 #pragma unroll
-        for (int r = 0; r < B_PER_T; ++r) Bs[buf][b_kk[r]][b_nn[r]] = b_reg[r];
+        for (int r = 0; r < B_PER_T; ++r) Bs[buf][b_kk][b_n_base + 16 * r] = b_reg[r];
     };
 
     const int wave = tid / 64, lane = tid % 64;
@@ -182,29 +214,43 @@ __global__ __launch_bounds__(kThreads) void contract_f64_kernel(ContractArgs p) 
 
     // ---- epilogue: for v_mfma_f64_16x16x4f64 lane holds D[lane/16 + 4*r][lane%16], r = 0..3 ----
     int64_t ccol[FN];
-    bool cok[FN];
+    uint32_t cok = 0;
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
         const int n = n0 + wn * WN + j * 16 + lrow;
-        cok[j] = n < p.N;
-        ccol[j] = cok[j] ? map_off(p.cn, (uint32_t)n, batch) : 0;
+        const bool ok = n < p.N;
+        cok |= (ok ? 1u : 0u) << j;
+        ccol[j] = ok ? map_off(p.cn, (uint32_t)n, batch) : 0;
     }
+    // all C reads are issued before any store so their latencies overlap
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
+        int64_t crow[4];
+        uint32_t rok = 0;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int64_t m = m0 + wm * WM + i * 16 + lk + 4 * r;
-            if (m >= m_end) continue;
-            const int64_t crow = bt.c_off + map_off(p.cm, (uint32_t)m, batch);
+            const bool ok = m < m_end;
+            rok |= (ok ? 1u : 0u) << r;
+            crow[r] = ok ? bt.c_off + map_off(p.cm, (uint32_t)m, batch) : 0;
+        }
+        double old[4][FN];
+        if (p.beta != 0.0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    old[r][j] = (((rok >> r) & (cok >> j)) & 1u) ? p.C[crow[r] + ccol[j]] : 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
-                if (!cok[j]) continue;
-                double *cp = p.C + crow + ccol[j];
+                if (!(((rok >> r) & (cok >> j)) & 1u)) continue;
                 double v = p.alpha * acc[i][j][r];
-                if (p.beta != 0.0) v += p.beta * (*cp);
-                *cp = v;
+                if (p.beta != 0.0) v += p.beta * old[r][j];
+                p.C[crow[r] + ccol[j]] = v;
             }
-        }
     }
 }
 
@@ -216,8 +262,29 @@ int64_t inner_stride(const dcc_idxmap &m) {
     return m.s0;
 }
 
+KMap make_kmap(const dcc_idxmap &m) {
+    KMap k;
+    k.s0 = m.s0;
+    k.s1 = m.s1;
+    k.s2 = m.s2;
+    k.e1 = (uint32_t)m.e1;
+    k.e2 = (uint32_t)m.e2;
+    k.step = (int64_t)kBK * m.s2;
+    k.carry1 = m.s1 - m.e2 * m.s2;
+    k.carry0 = m.s0 - m.e1 * m.s1;
+    return k;
+}
+
 template <int BM, int BN>
-int launch(const ContractArgs &args, bool a_kfast, bool b_nfast, hipStream_t stream) {
+int launch(ContractArgs &args, int64_t max_rows, bool a_kfast, bool b_nfast, hipStream_t stream) {
+    args.tiles_m = (int)((max_rows + BM - 1) / BM);
+    args.tiles_n = (int)((args.N + BN - 1) / BN);
+    const int64_t nbatch = args.total_tiles;  // caller passes nbatch here
+    args.total_tiles = nbatch * args.tiles_m * args.tiles_n;
+    if (args.total_tiles >= ((int64_t)1 << 31)) {
+        dcc::set_error("dcc_contract_f64: too many tiles");
+        return 2;
+    }
     const dim3 grid((unsigned)args.total_tiles), block(kThreads);
     if (a_kfast && b_nfast)
         hipLaunchKernelGGL((contract_f64_kernel<BM, BN, true, true>), grid, block, 0, stream, args);
@@ -248,6 +315,7 @@ extern "C" int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, d
     DCC_CHECK(valid_map(am) && valid_map(ak) && valid_map(bk) && valid_map(bn) && valid_map(cm) &&
                   valid_map(cn),
               "invalid index map");
+    DCC_CHECK(ak.table == nullptr && bk.table == nullptr, "K index maps must be mixed-radix maps");
     if (M == 0 || N == 0 || nbatch == 0 || max_rows == 0) return 0;
     DCC_CHECK(A != nullptr && B != nullptr && C != nullptr && batches != nullptr, "null pointer");
 
@@ -258,27 +326,32 @@ extern "C" int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, d
     args.alpha = alpha;
     args.beta = beta;
     args.A = A;
-    args.am = am;
-    args.ak = ak;
     args.B = B;
-    args.bk = bk;
-    args.bn = bn;
     args.C = C;
+    args.am = am;
+    args.bn = bn;
     args.cm = cm;
     args.cn = cn;
+    args.ak = make_kmap(ak);
+    args.bk = make_kmap(bk);
     args.batches = batches;
+    args.total_tiles = nbatch;
 
     // choose the contiguous index of each operand as the lane-fastest one
     const bool a_kfast = inner_stride(ak) == 1 || inner_stride(am) != 1;
     const bool b_nfast = inner_stride(bn) == 1 || inner_stride(bk) != 1;
-
-    constexpr int BM = 64, BN = 64;
-    args.tiles_m = (int)((max_rows + BM - 1) / BM);
-    args.tiles_n = (int)((N + BN - 1) / BN);
-    args.total_tiles = (int64_t)nbatch * args.tiles_m * args.tiles_n;
-    DCC_CHECK(args.total_tiles < ((int64_t)1 << 31), "too many tiles");
-    if (K == 0) {
-        // C = beta*C: handled by the same kernel with an empty K loop
+    hipStream_t s = dcc::as_stream(stream);
+    // Tile shape: DCC_CONTRACT_TILE=128x128|128x64|64x128|64x64 overrides the default.
+    static const char *tile_env = getenv("DCC_CONTRACT_TILE");
+    // 64x64 keeps four waves per SIMD resident, which hides HBM latency better
+    // than the larger register tiles (measured on MI355X for every triples GEMM).
+    int tbm = 64, tbn = 64;
+    if (tile_env != nullptr && sscanf(tile_env, "%dx%d", &tbm, &tbn) != 2) {
+        tbm = 64;
+        tbn = 64;
     }
-    return launch<BM, BN>(args, a_kfast, b_nfast, dcc::as_stream(stream));
+    if (tbm == 128 && tbn == 128) return launch<128, 128>(args, max_rows, a_kfast, b_nfast, s);
+    if (tbm == 128 && tbn == 64) return launch<128, 64>(args, max_rows, a_kfast, b_nfast, s);
+    if (tbm == 64 && tbn == 128) return launch<64, 128>(args, max_rows, a_kfast, b_nfast, s);
+    return launch<64, 64>(args, max_rows, a_kfast, b_nfast, s);
 }

@@ -32,7 +32,6 @@ class SyntheticSystem:
         self.fock = np.diag(self.mo_energy)
         B = rng.normal(0.0, sigma_b, (self.naux, self.nmo, self.nmo))
         self.B = 0.5 * (B + B.transpose(0, 2, 1))
-        self._rng_t3 = np.random.default_rng(self.seed + 7919)
         self.sigma_t3 = sigma_t3
         self.dtype = dtype
 
@@ -59,11 +58,12 @@ class SyntheticSystem:
         """Canonical-block T3, shape (nocc3, v, v, v), lexicographic i<=j<=k."""
         o, v = self.nocc, self.nvir
         eia = self.mo_energy[:o, None] - self.mo_energy[None, o:]
+        rng = np.random.default_rng(self.seed + 7919)
         blocks = []
         for i in range(o):
             for j in range(i, o):
                 for k in range(j, o):
-                    x = self._rng_t3.normal(0.0, self.sigma_t3, (v, v, v))
+                    x = rng.normal(0.0, self.sigma_t3, (v, v, v))
                     if i == j == k:
                         x = (x + x.transpose(0, 2, 1) + x.transpose(1, 0, 2) + x.transpose(1, 2, 0)
                              + x.transpose(2, 0, 1) + x.transpose(2, 1, 0)) / 6.0

@@ -155,6 +155,8 @@ class TriplesEngine:
         self.W_ovov = DeviceArray((o, v, o, v))
         self.W_vooo = DeviceArray((v, o, o, o))
         self.W_vvvo = DeviceArray((v, v, v, o))
+        self.W_ovvv = DeviceArray((o, v, v, v))   # W_vvvo[a,b,d,j] stored as [j,a,b,d]
+        self.W_oovo = DeviceArray((o, o, v, o))   # W_vooo[a,l,i,j] stored as [i,j,a,l]
         self.W_vvvv = DeviceArray((v, v, v, v))
         self.WR1 = DeviceArray((o, v, o, v))
         self.WR3a = DeviceArray((o, v, o, v))
@@ -192,7 +194,7 @@ class TriplesEngine:
         for sub, wocc, tocc in vvvo_terms:
             sa, rest = sub.split(',')
             sb = rest.split('->')[0]
-            A = Operand(self.W_vvvo, sa + 'w', (v, v, v, o))
+            A = Operand(self.W_ovvv, 'w' + sa, (o, v, v, v))
             B = Operand(self.t2, 'st' + sb, (o, o, v, v))
             fixed = [dict(w=f[wocc], s=f[tocc[0]], t=f[tocc[1]], B=f['B']) for f in blocks]
             launches.append(plan_term('T2T3:W_vvvo ' + sub, sub, A, B, Rb, 1.0, 1.0, fixed, ext))
@@ -201,7 +203,7 @@ class TriplesEngine:
         for sub, wocc, tocc in vooo_terms:
             sa, rest = sub.split(',')
             sb = rest.split('->')[0]
-            A = Operand(self.W_vooo, sa + 'uw', (v, o, o, o))
+            A = Operand(self.W_oovo, 'uw' + sa, (o, o, v, o))
             B = Operand(self.t2, sb[0] + 's' + sb[1:], (o, o, v, v))
             fixed = [dict(u=f[wocc[0]], w=f[wocc[1]], s=f[tocc], B=f['B']) for f in blocks]
             launches.append(plan_term('T2T3:W_vooo ' + sub, sub, A, B, Rb, -1.0, 1.0, fixed, ext))
@@ -292,6 +294,8 @@ class TriplesEngine:
         for name in ('F_oo', 'F_vv', 'W_oooo', 'W_ovvo', 'W_ovov', 'W_vooo', 'W_vvvo', 'W_vvvv'):
             getattr(self, name).upload(np.ascontiguousarray(imds[name], dtype=np.float64), self.stream)
         self.t2.upload(np.ascontiguousarray(t2, dtype=np.float64), self.stream)
+        self.W_ovvv.upload(np.ascontiguousarray(np.asarray(imds['W_vvvo']).transpose(3, 0, 1, 2)), self.stream)
+        self.W_oovo.upload(np.ascontiguousarray(np.asarray(imds['W_vooo']).transpose(2, 3, 0, 1)), self.stream)
         s = self.stream.handle if self.stream is not None else None
         L.call("dcc_ring_weights_f64", self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
                self.WR3b.vp, s)
@@ -303,12 +307,20 @@ class TriplesEngine:
         s = self.stream.handle if self.stream is not None else None
         L.call("dcc_s3_transform_f64", self.Y.vp, self.t3.vp, self.nvir, 0.0, self.y_jobs.vp, self.n_y_jobs, s)
 
-    def residual(self):
-        """R <- r3 of distr_cc/rccsdt.py:368-433 for the current t3 and intermediates."""
+    def residual(self, timer=None):
+        """R <- r3 of distr_cc/rccsdt.py:368-433 for the current t3 and intermediates.
+
+        ``timer`` (optional) is called as timer(launch, 'start'|'end') around
+        every GEMM launch, e.g. to record HIP events on the engine's stream.
+        """
         self.R.fill(0.0, self.stream)
         self.materialize_y()
         for g in self.launches:
+            if timer is not None:
+                timer(g, 'start')
             g.run(self.stream)
+            if timer is not None:
+                timer(g, 'end')
 
     def finalize(self, mo_energy, level_shift=0.0, nvir_diis=None, r3_out=None):
         """rccsdt.py:947-971 fused: t3 += P3_full-projected R / D; returns (norm^2, active norm^2)."""

@@ -91,15 +91,21 @@ def intermediates_t3_add_t3_tri(nocc, nvir, t1_eris, t3_local, W_vooo, W_vvvo):
     return W_vooo, W_vvvo
 
 
-def compute_r3_tri(nocc, nvir, imds, t2, t3_local, batch_size=100):
+def compute_r3_tri(nocc, nvir, imds, t2, t3_local, batch_size=100, targets=None):
     """distr_cc/rccsdt.py:368-433 -> r3 in canonical-block form (before projection).
 
     ``imds`` is a mapping with F_oo, F_vv, W_oooo, W_ovvo, W_ovov, W_vooo,
-    W_vvvo, W_vvvv (the shapes of rccsdt.py:380,445,625).
+    W_vvvo, W_vvvv (the shapes of rccsdt.py:380,445,625).  ``targets``
+    optionally restricts the computed target blocks to a subset of canonical
+    indices (the other blocks of the result are left zero); it exists so a
+    bounded sample of a large residual can be timed.
     """
     F_vv, W_vooo, W_vvvo, W_vvvv = imds['F_vv'], imds['W_vooo'], imds['W_vvvo'], imds['W_vvvv']
-    r3 = np.empty_like(t3_local)
+    r3 = np.zeros_like(t3_local)
+    tset = None if targets is None else set(int(x) for x in targets)
     for idx, (i, j, k) in enumerate(canonical_triples(nocc)):       # :385-402
+        if tset is not None and idx not in tset:
+            continue
         r = r3[idx]
         t = t3_local[idx]
         einsum('abd,dc->abc', W_vvvo[..., j], t2[i, k], r, 1.0, 0.0)
@@ -117,8 +123,10 @@ def compute_r3_tri(nocc, nvir, imds, t2, t3_local, batch_size=100):
         einsum('ad,dbc->abc', F_vv, t, r, 1.0, 1.0)
         einsum('bd,adc->abc', F_vv, t, r, 1.0, 1.0)
         einsum('cd,abd->abc', F_vv, t, r, 1.0, 1.0)
-    compute_oooo_oovv_contraction(nocc, nvir, imds, t3_local, r3, batch_size)   # :414
+    compute_oooo_oovv_contraction(nocc, nvir, imds, t3_local, r3, batch_size, tset)   # :414
     for idx in range(r3.shape[0]):                                  # :421-426
+        if tset is not None and idx not in tset:
+            continue
         t = t3_local[idx]
         r = r3[idx]
         einsum('abde,dec->abc', W_vvvv, t, r, 1.0, 1.0)
@@ -127,7 +135,7 @@ def compute_r3_tri(nocc, nvir, imds, t2, t3_local, batch_size=100):
     return r3
 
 
-def compute_oooo_oovv_contraction(nocc, nvir, imds, t3_local, r3, batch_size=100):
+def compute_oooo_oovv_contraction(nocc, nvir, imds, t3_local, r3, batch_size=100, tset=None):
     """distr_cc/rccsdt.py:435-614 at one rank (t3_data = t3_local[batch], :523-526)."""
     W_ovvo, W_ovov = np.ascontiguousarray(imds['W_ovvo']), imds['W_ovov']
     ijk_list = np.array(canonical_triples(nocc), dtype=np.int32).reshape(-1, 3)
@@ -138,12 +146,14 @@ def compute_oooo_oovv_contraction(nocc, nvir, imds, t3_local, r3, batch_size=100
         b1 = min(b0 + batch_size, n3)
         batch_ijk = ijk_list[b0:b1]
         t3_data = t3_local[b0:b1]
-        compute_r3_Foo_Woooo_contribution_batch(nocc, r3, t3_data, batch_ijk, imds)
+        compute_r3_Foo_Woooo_contribution_batch(nocc, r3, t3_data, batch_ijk, imds, tset)
         for idx, (o00, o10, o20) in enumerate(batch_ijk):
             t = t3_data[idx]
             t_2 = t3_transpose_add(t)                                 # :534
             p0, p1, p2 = t3_triple_sym(t)                             # :535
             for tgt, (i, j, k) in enumerate(targets):
+                if tset is not None and tgt not in tset:
+                    continue
                 r = r3[tgt]
                 # part 3 (:540-558)
                 if j == o00 and k == o10:
@@ -196,13 +206,15 @@ def compute_oooo_oovv_contraction(nocc, nvir, imds, t3_local, r3, batch_size=100
     return r3
 
 
-def compute_r3_Foo_Woooo_contribution_batch(nocc, r3, t3_data, batch_ijk, imds):
+def compute_r3_Foo_Woooo_contribution_batch(nocc, r3, t3_data, batch_ijk, imds, tset=None):
     """distr_cc/rccsdt.py:616-732 (six prefactor matrices + six GEMMs)."""
     F_oo, W_oooo = imds['F_oo'], imds['W_oooo']
     targets = canonical_triples(nocc)
     pfs = np.zeros((6, len(batch_ijk), len(targets)))
     for idx, (o00, o10, o20) in enumerate(batch_ijk):
         for q, (i, j, k) in enumerate(targets):
+            if tset is not None and q not in tset:
+                continue
             # part 0 (:632-650)
             if k == o20:
                 pfs[0, idx, q] += W_oooo[o00, o10, i, j]
@@ -263,12 +275,19 @@ def compute_r3_Foo_Woooo_contribution_batch(nocc, r3, t3_data, batch_ijk, imds):
                 pfs[1, idx, q] -= F_oo[o10, k]
             if o00 != o10 and i == o10 and j == o20:
                 pfs[4, idx, q] -= F_oo[o00, k]
-    einsum('pq,pabc->qabc', pfs[0], t3_data, r3, 1.0, 1.0)             # :726-731
-    einsum('pq,pacb->qabc', pfs[1], t3_data, r3, 1.0, 1.0)
-    einsum('pq,pbac->qabc', pfs[2], t3_data, r3, 1.0, 1.0)
-    einsum('pq,pabc->qcab', pfs[3], t3_data, r3, 1.0, 1.0)
-    einsum('pq,pabc->qbca', pfs[4], t3_data, r3, 1.0, 1.0)
-    einsum('pq,pabc->qcba', pfs[5], t3_data, r3, 1.0, 1.0)
+    if tset is None:
+        out, sel = r3, pfs
+    else:  # only the sampled target rows are nonzero in pfs
+        qs = np.array(sorted(tset), dtype=np.int64)
+        out, sel = r3[qs], pfs[:, :, qs]
+    einsum('pq,pabc->qabc', sel[0], t3_data, out, 1.0, 1.0)            # :726-731
+    einsum('pq,pacb->qabc', sel[1], t3_data, out, 1.0, 1.0)
+    einsum('pq,pbac->qabc', sel[2], t3_data, out, 1.0, 1.0)
+    einsum('pq,pabc->qcab', sel[3], t3_data, out, 1.0, 1.0)
+    einsum('pq,pabc->qbca', sel[4], t3_data, out, 1.0, 1.0)
+    einsum('pq,pabc->qcba', sel[5], t3_data, out, 1.0, 1.0)
+    if tset is not None:
+        r3[qs] = out
     return r3
 
 
