@@ -1,0 +1,162 @@
+// Kernels of the (Q)/[Q] energy (distr_cc/rccsdt_q.py:943-1177).
+#include "dcc_common.h"
+
+namespace {
+
+// out[a,b,c,i,j,k] (a,b,c < vpad) = T[ijk][abc] for a,b,c < nvir, 0 in the padding.
+// T3 is the canonical i<=j<=k block array; the ordered (i,j,k) maps onto its
+// canonical block with the simultaneous permutation of (a,b,c).
+__global__ void t3_to_abc_kernel(const double *__restrict__ t3, int no, int nv, int vp, double *__restrict__ out) {
+    const int64_t o = no, v = nv, p = vp;
+    const int64_t total = p * p * p * o * o * o;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    const int k = (int)(t % o), j = (int)((t / o) % o), i = (int)((t / (o * o)) % o);
+    const int64_t abc = t / (o * o * o);
+    const int c = (int)(abc % p), b = (int)((abc / p) % p), a = (int)(abc / (p * p));
+    double val = 0.0;
+    if (a < nv && b < nv && c < nv) {
+        // sort (i,j,k) carrying (a,b,c) along
+        int oi[3] = {i, j, k}, vi[3] = {a, b, c};
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int u = 0; u < 2 - s; ++u)
+                if (oi[u] > oi[u + 1]) {
+                    int x = oi[u]; oi[u] = oi[u + 1]; oi[u + 1] = x;
+                    x = vi[u]; vi[u] = vi[u + 1]; vi[u + 1] = x;
+                }
+        const int64_t n = no;
+        const int64_t ci = oi[0], cj = oi[1], ck = oi[2];
+        const int64_t ni = n - ci;
+        const int64_t lin = (n * (n + 1) * (n + 2) - ni * (ni + 1) * (ni + 2)) / 6 +
+                            (cj - ci) * (2 * n - cj - ci + 1) / 2 + (ck - cj);
+        val = t3[lin * v * v * v + ((int64_t)vi[0] * v + vi[1]) * v + vi[2]];
+    }
+    out[t] = val;
+}
+
+// factor of the restricted a<=b<=c<=d sum (distr_cc/rccsdt_q.py:410-428)
+__device__ __forceinline__ double q_factor(int a, int b, int c, int d) {
+    if (a < b && b < c && c < d) return 24.0;
+    if ((a == b && b < c && c < d) || (a < b && b == c && c < d) || (a < b && b < c && c == d)) return 12.0;
+    if (a == b && b < c && c == d) return 6.0;
+    return 0.0;
+}
+
+constexpr int kQThreads = 256;
+constexpr int kQMaxO4 = 6561;  // nocc <= 9: two o^4 FP64 buffers in LDS
+
+// One workgroup per (task, a, b, c, d): t4 = P4_444[(t4 + z4_p) / D] * factor on
+// the o^4 occupied block, then partial sums <t4, z4_p> and <t4, z4_sq> with
+// z4_sq built on the fly from <ab|ij> (vvoo) and t2 (vvoo layout).
+__global__ __launch_bounds__(kQThreads) void q_finalize_kernel(
+    const double *__restrict__ t4, const double *__restrict__ z4, const int *__restrict__ tasks, int bs, int no,
+    int nv, int vp, const double *__restrict__ e_occ, const double *__restrict__ e_vir,
+    const double *__restrict__ vvoo, const double *__restrict__ t2, double *__restrict__ partials) {
+    extern __shared__ double lds[];
+    __shared__ double red[2][kQThreads / 64];
+    const int o = no, o2 = o * o, o3 = o2 * o, o4 = o3 * o;
+    double *b0 = lds, *b1 = lds + o4;
+    const int bs4 = bs * bs * bs * bs;
+    const int64_t task = blockIdx.x / bs4;
+    const int e = (int)(blockIdx.x % bs4);
+    const int ia = e / (bs * bs * bs), ib = (e / (bs * bs)) % bs, ic = (e / bs) % bs, id = e % bs;
+    const int a = tasks[4 * task] + ia, b = tasks[4 * task + 1] + ib, c = tasks[4 * task + 2] + ic,
+              d = tasks[4 * task + 3] + id;
+    double paren = 0.0, bracket = 0.0;
+    const double fac = (a < nv && b < nv && c < nv && d < nv) ? q_factor(a, b, c, d) : 0.0;
+    if (fac != 0.0) {  // uniform over the workgroup
+        const int64_t base = blockIdx.x * (int64_t)o4;
+        const double ev = e_vir[a] + e_vir[b] + e_vir[c] + e_vir[d];
+        for (int x = threadIdx.x; x < o4; x += kQThreads) {
+            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            const double den = e_occ[i] + e_occ[j] + e_occ[k] + e_occ[l] - ev;
+            const double num = t4[base + x] + z4[base + x];
+            b0[x] = fabs(den) > 1e-15 ? num / den : 0.0;
+        }
+        __syncthreads();
+        // (2 - P_ab - P_ac - P_ad) on the occupied positions
+        for (int x = threadIdx.x; x < o4; x += kQThreads) {
+            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            b1[x] = 2.0 * b0[x] - b0[((j * o + i) * o + k) * o + l] - b0[((k * o + j) * o + i) * o + l] -
+                    b0[((l * o + j) * o + k) * o + i];
+        }
+        __syncthreads();
+        // (2 - P_bc - P_bd)
+        for (int x = threadIdx.x; x < o4; x += kQThreads) {
+            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            b0[x] = 2.0 * b1[x] - b1[((i * o + k) * o + j) * o + l] - b1[((i * o + l) * o + k) * o + j];
+        }
+        __syncthreads();
+        const int64_t v = vp;
+        auto VV = [&](int p, int q, int r, int s) { return vvoo[((p * v + q) * o + r) * o + s]; };
+        auto T2 = [&](int p, int q, int r, int s) { return t2[((p * v + q) * o + r) * o + s]; };
+        for (int x = threadIdx.x; x < o4; x += kQThreads) {
+            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            // (2 - P_cd), then the factor
+            const double t = fac * (2.0 * b0[x] - b0[((i * o + j) * o + l) * o + k]);
+            const double zsq = VV(a, b, i, j) * T2(c, d, k, l) + VV(a, c, i, k) * T2(b, d, j, l) +
+                               VV(a, d, i, l) * T2(b, c, j, k) + VV(b, c, j, k) * T2(a, d, i, l) +
+                               VV(b, d, j, l) * T2(a, c, i, k) + VV(c, d, k, l) * T2(a, b, i, j);
+            paren += t * z4[base + x];
+            bracket += t * zsq;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        paren += __shfl_down(paren, off, 64);
+        bracket += __shfl_down(bracket, off, 64);
+    }
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    if (lane == 0) {
+        red[0][wave] = paren;
+        red[1][wave] = bracket;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double p = 0.0, q = 0.0;
+        for (int w = 0; w < kQThreads / 64; ++w) {
+            p += red[0][w];
+            q += red[1][w];
+        }
+        partials[2 * blockIdx.x] = p;
+        partials[2 * blockIdx.x + 1] = q;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int dcc_t3_to_abc_f64(const double *t3_tri, int64_t nocc, int64_t nvir, int64_t vpad, double *out,
+                      dcc_stream_t stream) {
+    DCC_CHECK(nocc >= 0 && nvir >= 0 && vpad >= nvir, "bad extents");
+    const int64_t total = vpad * vpad * vpad * nocc * nocc * nocc;
+    if (total == 0) return 0;
+    DCC_CHECK(t3_tri != nullptr && out != nullptr, "null pointer");
+    DCC_CHECK((total + 255) / 256 < ((int64_t)1 << 31), "grid too large");
+    hipLaunchKernelGGL(t3_to_abc_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       dcc::as_stream(stream), t3_tri, (int)nocc, (int)nvir, (int)vpad, out);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+int dcc_q_finalize_f64(const double *t4, const double *z4, const int32_t *tasks, int64_t ntask, int64_t blksize,
+                       int64_t nocc, int64_t nvir, int64_t vpad, const double *e_occ, const double *e_vir,
+                       const double *vvoo, const double *t2, double *partials, dcc_stream_t stream) {
+    DCC_CHECK(ntask >= 0 && blksize > 0 && nocc > 0 && nvir > 0 && vpad >= nvir, "bad extents");
+    const int64_t o4 = nocc * nocc * nocc * nocc;
+    DCC_CHECK(o4 <= kQMaxO4, "nocc > 9 is not supported by the (Q) finalize kernel");
+    const int64_t nwg = ntask * blksize * blksize * blksize * blksize;
+    if (nwg == 0) return 0;
+    DCC_CHECK(t4 && z4 && tasks && e_occ && e_vir && vvoo && t2 && partials, "null pointer");
+    DCC_CHECK(nwg < ((int64_t)1 << 31), "grid too large");
+    hipLaunchKernelGGL(q_finalize_kernel, dim3((unsigned)nwg), dim3(kQThreads), (size_t)(2 * o4 * sizeof(double)),
+                       dcc::as_stream(stream), t4, z4, tasks, (int)blksize, (int)nocc, (int)nvir, (int)vpad, e_occ,
+                       e_vir, vvoo, t2, partials);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // extern "C"

@@ -71,6 +71,9 @@ _SIGS = {
     "dcc_reduce_pairs_f64": ([c_vp, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_oooo_prefactors_f64": ([c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_ring_weights_f64": ([c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp], ctypes.c_int),
+    "dcc_t3_to_abc_f64": ([c_vp, c_i64, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
+    "dcc_q_finalize_f64": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
+                            c_vp], ctypes.c_int),
 }
 
 for _name, (_args, _res) in _SIGS.items():

@@ -1,0 +1,171 @@
+"""GPU (Q)/[Q] energy of RCCSDT (the computation of distr_cc/rccsdt_q.py:227-1191).
+
+Layout in HBM (FP64, virtual dimension zero-padded to vp = ceil(v/bs)*bs):
+  T_abc    (vp, vp, vp, o, o, o)   dense T3 in (a,b,c,i,j,k) order -- the
+                                   one-GPU form of the IJK -> ABC redistribution
+  W_vvvvoo (vp, vp, vp, vp, o, o)  built once for all tasks (the reference
+  W_vvoooo (vp, vp, o, o, o, o)    rebuilds block slices 12x / 6x per task)
+  z4, t4   (ntask_chunk, bs, bs, bs, bs, o, o, o, o)
+Each of the 24 z4_p einsums and 24 t4 einsums of a task (rccsdt_q.py:1014-1129)
+is one batched gather-GEMM over a chunk of tasks; a fused kernel then applies
+t4 += z4_p, the denominator, P4_444, the factor and both dot products
+(rccsdt_q.py:1131-1153), writing per-workgroup partials that are summed in a
+fixed order.
+"""
+import numpy as np
+
+from distr_cc import _lib as L
+from distr_cc.device import DeviceArray, init as device_init
+from distr_cc.t3_engine import Operand, plan_einsum
+
+# (subscripts, A operand, B operand, alpha) in the order of rccsdt_q.py:1017-1070
+Z4_TERMS = [
+    ('cdel,abeijk->abcdijkl', 'vvvo', 'T', 1.0), ('dcek,abeijl->abcdijkl', 'vvvo', 'T', 1.0),
+    ('dmli,abcmjk->abcdijkl', 'vooo', 'T', -1.0), ('dmlj,abcimk->abcdijkl', 'vooo', 'T', -1.0),
+    ('cmki,abdmjl->abcdijkl', 'vooo', 'T', -1.0), ('cmkj,abdiml->abcdijkl', 'vooo', 'T', -1.0),
+    ('bdel,aceikj->abcdijkl', 'vvvo', 'T', 1.0), ('dbej,aceikl->abcdijkl', 'vvvo', 'T', 1.0),
+    ('bmji,acdmkl->abcdijkl', 'vooo', 'T', -1.0), ('bmjk,acdiml->abcdijkl', 'vooo', 'T', -1.0),
+    ('dmlk,acbimj->abcdijkl', 'vooo', 'T', -1.0),
+    ('bcek,adeilj->abcdijkl', 'vvvo', 'T', 1.0), ('cbej,adeilk->abcdijkl', 'vvvo', 'T', 1.0),
+    ('bmjl,adcimk->abcdijkl', 'vooo', 'T', -1.0), ('cmkl,adbimj->abcdijkl', 'vooo', 'T', -1.0),
+    ('adel,bcejki->abcdijkl', 'vvvo', 'T', 1.0), ('daei,bcejkl->abcdijkl', 'vvvo', 'T', 1.0),
+    ('amij,bcdmkl->abcdijkl', 'vooo', 'T', -1.0), ('amik,bcdjml->abcdijkl', 'vooo', 'T', -1.0),
+    ('acek,bdejli->abcdijkl', 'vvvo', 'T', 1.0), ('caei,bdejlk->abcdijkl', 'vvvo', 'T', 1.0),
+    ('amil,bdcjmk->abcdijkl', 'vooo', 'T', -1.0),
+    ('abej,cdekli->abcdijkl', 'vvvo', 'T', 1.0), ('baei,cdeklj->abcdijkl', 'vvvo', 'T', 1.0),
+]
+# rccsdt_q.py:1075-1129
+T4_TERMS = [
+    ('abcejk,edil->abcdijkl', 'Wv', 't2', 1.0), ('abdejl,ecik->abcdijkl', 'Wv', 't2', 1.0),
+    ('acdekl,ebij->abcdijkl', 'Wv', 't2', 1.0), ('baceik,edjl->abcdijkl', 'Wv', 't2', 1.0),
+    ('badeil,ecjk->abcdijkl', 'Wv', 't2', 1.0), ('bcdekl,eaji->abcdijkl', 'Wv', 't2', 1.0),
+    ('cabeij,edkl->abcdijkl', 'Wv', 't2', 1.0), ('cadeil,ebkj->abcdijkl', 'Wv', 't2', 1.0),
+    ('cbdejl,eaki->abcdijkl', 'Wv', 't2', 1.0), ('dabeij,eclk->abcdijkl', 'Wv', 't2', 1.0),
+    ('daceik,eblj->abcdijkl', 'Wv', 't2', 1.0), ('dbcejk,eali->abcdijkl', 'Wv', 't2', 1.0),
+    ('abmijk,cdml->abcdijkl', 'Wo', 't2', -1.0), ('abmijl,dcmk->abcdijkl', 'Wo', 't2', -1.0),
+    ('acmikj,bdml->abcdijkl', 'Wo', 't2', -1.0), ('acmikl,dbmj->abcdijkl', 'Wo', 't2', -1.0),
+    ('admilj,bcmk->abcdijkl', 'Wo', 't2', -1.0), ('admilk,cbmj->abcdijkl', 'Wo', 't2', -1.0),
+    ('bcmjki,adml->abcdijkl', 'Wo', 't2', -1.0), ('bcmjkl,dami->abcdijkl', 'Wo', 't2', -1.0),
+    ('bdmjli,acmk->abcdijkl', 'Wo', 't2', -1.0), ('bdmjlk,cami->abcdijkl', 'Wo', 't2', -1.0),
+    ('cdmkli,abmj->abcdijkl', 'Wo', 't2', -1.0), ('cdmklj,bami->abcdijkl', 'Wo', 't2', -1.0),
+]
+# dense W builders (rccsdt_q.py:390-408) over the full virtual range
+WV_TERMS = [('abef,fcjk->abcejk', 'vvvv', 't2', 0.5), ('acef,fbkj->abcejk', 'vvvv', 't2', 0.5),
+            ('mbej,acmk->abcejk', 'ovvo', 't2', -1.0), ('mcek,abmj->abcejk', 'ovvo', 't2', -1.0),
+            ('maje,bcmk->abcejk', 'ovov', 't2', -1.0), ('make,cbmj->abcejk', 'ovov', 't2', -1.0)]
+WO_TERMS = [('mnki,abnj->abmijk', 'oooo', 't2', -0.5), ('mnkj,bani->abmijk', 'oooo', 't2', -0.5)]
+
+
+def create_full_task_list(nvir, blksize):
+    """Task order of rccsdt_q.py:302-314 (virtual block quadruples A<=B<=C<=D)."""
+    tasks = []
+    for d0 in range(0, nvir, blksize):
+        d1 = min(d0 + blksize, nvir)
+        for c0 in range(0, d1, blksize):
+            c1 = min(c0 + blksize, d1)
+            for b0 in range(0, c1, blksize):
+                b1 = min(b0 + blksize, c1)
+                for a0 in range(0, b1, blksize):
+                    tasks.append((a0, b0, c0, d0))
+    return tasks
+
+
+def _pad(x, axes_v, vp, o):
+    shape = tuple(vp if ax in axes_v else s for ax, s in enumerate(x.shape))
+    out = np.zeros(shape)
+    out[tuple(slice(0, s) for s in x.shape)] = x
+    return np.ascontiguousarray(out)
+
+
+class QEngine:
+    """Device state of one (Q)/[Q] evaluation."""
+
+    def __init__(self, nocc, nvir, blksize=4, stream=None, device=0, max_chunk_bytes=1 << 30):
+        device_init(device)
+        self.o, self.v, self.bs = int(nocc), int(nvir), int(blksize)
+        if self.bs < 1:
+            raise ValueError("blksize must be positive")
+        self.vp = -(-self.v // self.bs) * self.bs
+        self.stream = stream
+        o, vp, bs = self.o, self.vp, self.bs
+        per_task = bs ** 4 * o ** 4 * 8
+        self.chunk = max(1, min(int(max_chunk_bytes // per_task), 4096))
+
+    def _s(self):
+        return self.stream.handle if self.stream is not None else None
+
+    def kernel(self, pppp, t2_oovv, t3_dev, mo_energy, job_idx=0, n_jobs=1):
+        """(e_[Q], e_(Q)) for the canonical-block T3 already on the device (``t3_dev``)."""
+        o, v, vp, bs = self.o, self.v, self.vp, self.bs
+        e = np.asarray(mo_energy, dtype=np.float64)
+        pppp = np.asarray(pppp)
+        t2q = np.asarray(t2_oovv).transpose(2, 3, 0, 1)
+        dev = {}
+        dev['vvvo'] = DeviceArray.from_host(_pad(pppp[o:, o:, o:, :o], (0, 1, 2), vp, o))
+        dev['vooo'] = DeviceArray.from_host(_pad(pppp[o:, :o, :o, :o], (0,), vp, o))
+        dev['vvoo'] = DeviceArray.from_host(_pad(pppp[o:, o:, :o, :o], (0, 1), vp, o))
+        dev['ovvo'] = DeviceArray.from_host(_pad(pppp[:o, o:, o:, :o], (1, 2), vp, o))
+        dev['ovov'] = DeviceArray.from_host(_pad(pppp[:o, o:, :o, o:], (1, 3), vp, o))
+        dev['vvvv'] = DeviceArray.from_host(_pad(pppp[o:, o:, o:, o:], (0, 1, 2, 3), vp, o))
+        dev['oooo'] = DeviceArray.from_host(np.ascontiguousarray(pppp[:o, :o, :o, :o]))
+        dev['t2'] = DeviceArray.from_host(_pad(t2q, (0, 1), vp, o))
+        dev['e_occ'] = DeviceArray.from_host(np.ascontiguousarray(e[:o]))
+        dev['e_vir'] = DeviceArray.from_host(np.concatenate([e[o:], np.zeros(vp - v)]))
+        dev['T'] = DeviceArray((vp, vp, vp, o, o, o))
+        L.call("dcc_t3_to_abc_f64", t3_dev.vp, o, v, vp, dev['T'].vp, self._s())
+        dev['Wv'] = DeviceArray((vp, vp, vp, vp, o, o))
+        dev['Wo'] = DeviceArray((vp, vp, o, o, o, o))
+        shapes = {'vvvo': 'vvvo', 'vooo': 'vooo', 'vvoo': 'vvoo', 'ovvo': 'ovvo', 'ovov': 'ovov',
+                  'vvvv': 'vvvv', 'oooo': 'oooo', 't2': 'vvoo', 'T': 'vvvooo', 'Wv': 'vvvvoo', 'Wo': 'vvoooo'}
+
+        def op(name, letters):
+            shp = tuple(vp if ch == 'v' else o for ch in shapes[name])
+            return Operand(dev[name], letters, shp)
+
+        full = {x: vp for x in 'abcdef'}
+        full.update({x: o for x in 'ijklmn'})
+        # dense W_vvvvoo / W_vvoooo (beta = 0 on the first term of each)
+        for n, (sub, an, bn, alpha) in enumerate(WV_TERMS):
+            sa, rest = sub.split(',')
+            sb, sc = rest.split('->')
+            plan_einsum('Q W_vvvvoo', sub, op(an, sa), op(bn, sb), op('Wv', sc), alpha, 0.0 if n == 0 else 1.0,
+                        full, [({}, {}, {})]).run(self.stream)
+        for n, (sub, an, bn, alpha) in enumerate(WO_TERMS):
+            sa, rest = sub.split(',')
+            sb, sc = rest.split('->')
+            plan_einsum('Q W_vvoooo', sub, op(an, sa), op(bn, sb), op('Wo', sc), alpha, 0.0 if n == 0 else 1.0,
+                        full, [({}, {}, {})]).run(self.stream)
+
+        tasks = create_full_task_list(v, bs)
+        per = (len(tasks) + n_jobs - 1) // n_jobs
+        tasks = tasks[job_idx * per: min(job_idx * per + per, len(tasks))]
+        ext = {x: bs for x in 'abcd'}
+        ext.update({'e': vp, 'f': vp})
+        ext.update({x: o for x in 'ijklmn'})
+        paren = bracket = 0.0
+        for c0 in range(0, len(tasks), self.chunk):
+            chunk = tasks[c0:c0 + self.chunk]
+            nt = len(chunk)
+            ext['H'] = nt
+            z4 = DeviceArray((nt, bs, bs, bs, bs, o, o, o, o))
+            t4 = DeviceArray((nt, bs, bs, bs, bs, o, o, o, o))
+            blocks = [dict(a=a0, b=b0, c=cc, d=d0) for (a0, b0, cc, d0) in chunk]
+            for out, terms in ((z4, Z4_TERMS), (t4, T4_TERMS)):
+                C = Operand(out, 'Habcdijkl', (nt, bs, bs, bs, bs, o, o, o, o))
+                for n, (sub, an, bn, alpha) in enumerate(terms):
+                    sa, rest = sub.split(',')
+                    sb = rest.split('->')[0]
+                    batches = [(blk, blk, {'H': h}) for h, blk in enumerate(blocks)]
+                    plan_einsum('Q ' + sub, sub, op(an, sa), op(bn, sb), C, alpha, 0.0 if n == 0 else 1.0,
+                                ext, batches).run(self.stream)
+            tk = DeviceArray.from_host(np.array(chunk, dtype=np.int32).reshape(-1, 4))
+            nwg = nt * bs ** 4
+            part = DeviceArray((nwg, 2))
+            L.call("dcc_q_finalize_f64", t4.vp, z4.vp, tk.vp, nt, bs, o, v, vp, dev['e_occ'].vp, dev['e_vir'].vp,
+                   dev['vvoo'].vp, dev['t2'].vp, part.vp, self._s())
+            tot = DeviceArray((2,))
+            L.call("dcc_reduce_pairs_f64", part.vp, nwg, tot.vp, self._s())
+            p, b = tot.download(stream=self.stream)
+            paren += p
+            bracket += b
+        return bracket / 12.0, (paren + bracket) / 12.0

@@ -22,6 +22,16 @@ batched gather-GEMMs (dcc_contract_f64):
   * W_oooo/F_oo x T3 (rccsdt.py:616-732): 6 GEMMs M = n3 targets, K = n3 sources,
     N = v^3, with the prefactors built on the device.
 Then dcc_r3_finalize_f64 fuses rccsdt.py:947-971.
+
+T3 -> r1/r2 (rccsdt.py:253-321) and T3 -> W_vooo/W_vvvo (rccsdt.py:323-366)
+run per middle occupied index q over a slice Zq[p][r] = P3_201(T[p,q,r])
+built by one S3 transform, with all ordered (p,q,r) covered exactly once:
+  r2[q,l,b,c] += sum_{r,p,a} (-E_ooov[r,p,l,a] + 1/2 d_lr f_ov[p,a]) Z[p,q,r][a,b,c]
+  r2[q,r,b,c] += sum_{p,a,d} E_vovv[c,p,a,d] Z[p,q,r][d,b,a]
+  r1[i,a]     += sum_{k,b,c} (E_oovv[q,k,b,c] - 1/2 E_oovv[q,k,c,b]) Z[i,q,k][a,b,c]
+  W_vooo[a,m,q,r] += sum_{p,d,e} E_oovv[m,p,d,e] Z[p,q,r][e,a,d]
+  W_vvvo[a,b,d,q] -= sum_{r,p,e} E_oovv[r,p,d,e] Z[p,q,r][e,b,a]
+(E_* are slices of the T1-dressed <pq|rs>; r1 uses P3_422 = 2 P3_201 - (P3_201)^{acb}.)
 """
 import ctypes
 
@@ -92,32 +102,81 @@ class GemmLaunch:
                stream.handle if stream is not None else None)
 
 
-def plan_term(name, sub, A, B, C, alpha, beta, batches_fixed, ext):
-    """Plan C[...] (+)= alpha * einsum(sub, A, B) as a batched gather-GEMM.
+def _fuse(letters, ops, ext):
+    """Split an ordered letter list into runs that are contiguous in every operand of ``ops``."""
+    groups = []
+    for x in letters:
+        if groups:
+            prev = groups[-1][-1]
+            if all(op.strides[prev] == op.strides[x] * ext[x] for op in ops):
+                groups[-1].append(x)
+                continue
+        groups.append([x])
+    return groups
 
-    ``sub`` is 'xy,yz->xz'-style over the free letters; each entry of
-    ``batches_fixed`` is a dict {letter: value} for the operands' fixed letters.
-    M = letters of C found in A, N = letters of C found in B, K = the rest of A.
+
+def _group_map(groups, ext, op):
+    """dcc_idxmap of up to three fused groups (outermost first) into operand ``op``."""
+    if not groups:
+        return L.IdxMap(None, 0, 1, 1, 0, 0, 0), 1
+    exts = [int(np.prod([ext[x] for x in g])) for g in groups]
+    strides = [op.strides[g[-1]] for g in groups]
+    while len(exts) < 3:
+        exts.insert(0, 1)
+        strides.insert(0, 0)
+    return L.IdxMap(None, 0, exts[1], exts[2], strides[0], strides[1], strides[2]), int(np.prod(exts))
+
+
+def plan_einsum(name, sub, A, B, C, alpha, beta, ext, batches, max_levels=3):
+    """Plan C (+)= alpha * einsum(sub, A, B) as one batched gather-GEMM launch.
+
+    ``sub`` names the free/contracted letters ('xy,yz->xz' style; no letter may
+    appear in all three operands).  ``batches`` is a list of (fixA, fixB, fixC)
+    dicts giving, per operand, values of letters that are fixed for that batch
+    entry (block offsets, occupied labels, ...); they only shift base offsets.
+    M = letters of C taken from A (in A's order), N = letters of C taken from B
+    (in B's order), K = letters shared by A and B.  Letters contiguous in both
+    operands of a dimension are fused; if a dimension still needs more than
+    ``max_levels`` levels, its outermost groups are moved into the batch list.
     """
     sa, rest = sub.split(',')
     sb, sc = rest.split('->')
-    m_l = ''.join(x for x in sa if x in sc)
-    n_l = ''.join(x for x in sb if x in sc)
-    k_a = ''.join(x for x in sa if x not in sc)
-    k_b = ''.join(x for x in sb if x not in sc)
-    assert sorted(k_a) == sorted(k_b), sub
-    # K order: operand whose innermost letter is a K letter sets the order
+    assert not (set(sa) & set(sb) & set(sc)), sub
+    m_l = [x for x in sa if x in sc]
+    n_l = [x for x in sb if x in sc]
+    k_a = [x for x in sa if x not in sc]
+    k_b = [x for x in sb if x not in sc]
+    assert sorted(k_a) == sorted(k_b) and sorted(m_l + n_l) == sorted(sc), sub
     k_l = k_b if (sb[-1] in k_b or sa[-1] not in k_a) else k_a
-    M = int(np.prod([ext[x] for x in m_l]))
-    N = int(np.prod([ext[x] for x in n_l]))
-    K = int(np.prod([ext[x] for x in k_l]))
-    batches = []
-    for fx in batches_fixed:
-        batches.append((A.offset(fx), B.offset(fx), C.offset(fx), 0, M))
-    return GemmLaunch(name, M, N, K, alpha, beta, A.buf, idxmap(m_l, ext, A.strides), idxmap(k_l, ext, A.strides),
-                      B.buf, idxmap(k_l, ext, B.strides), idxmap(n_l, ext, B.strides),
-                      C.buf, idxmap(m_l, ext, C.strides), idxmap(n_l, ext, C.strides),
-                      batches, M, 2.0 * M * N * K * len(batches_fixed))
+    gm, gn, gk = _fuse(m_l, (A, C), ext), _fuse(n_l, (B, C), ext), _fuse(k_l, (A, B), ext)
+    if len(gk) > max_levels:
+        raise ValueError("%s: contracted index needs %d levels" % (sub, len(gk)))
+    peel = []
+    while len(gm) > max_levels:
+        peel += gm.pop(0)
+    while len(gn) > max_levels:
+        peel += gn.pop(0)
+    entries = list(batches)
+    for x in peel:
+        grown = []
+        for fa, fb, fc in entries:
+            for val in range(ext[x]):
+                grown.append(tuple(dict(f, **{x: f.get(x, 0) + val}) for f in (fa, fb, fc)))
+        entries = grown
+    am, M = _group_map(gm, ext, A)
+    ak, K = _group_map(gk, ext, A)
+    bk, _ = _group_map(gk, ext, B)
+    bn, N = _group_map(gn, ext, B)
+    cm, _ = _group_map(gm, ext, C)
+    cn, _ = _group_map(gn, ext, C)
+    bt = [(A.offset(fa), B.offset(fb), C.offset(fc), 0, M) for fa, fb, fc in entries]
+    return GemmLaunch(name, M, N, K, alpha, beta, A.buf, am, ak, B.buf, bk, bn, C.buf, cm, cn, bt, M,
+                      2.0 * M * N * K * len(bt))
+
+
+def plan_term(name, sub, A, B, C, alpha, beta, batches_fixed, ext):
+    """plan_einsum with the same fixed-letter values applied to all three operands."""
+    return plan_einsum(name, sub, A, B, C, alpha, beta, ext, [(f, f, f) for f in batches_fixed])
 
 
 def s3_jobs_array(jobs):
@@ -178,6 +237,7 @@ class TriplesEngine:
         self.n_y_jobs = len(jobs)
         self._tables = []
         self.launches = self._plan_residual()
+        self._lower = None
 
     # ------------------------------------------------------------------ plan
     def _plan_residual(self):
@@ -281,6 +341,108 @@ class TriplesEngine:
                                            max(b[4] - b[3] for b in batches), 2.0 * nrows * v2 * ov,
                                            keep=(tdev,)))
         return launches
+
+
+    # ------------------------------------------------- T3 -> r1/r2 and T3 -> W
+    def _setup_lower(self):
+        o, v = self.nocc, self.nvir
+        v3 = v ** 3
+        lo = {}
+        lo['Z'] = DeviceArray((o, o, v, v, v))
+        lo['Ecomb'] = DeviceArray((o, o, o, v))      # [l,r,p,a]
+        lo['Evovv'] = DeviceArray((v, o, v, v))      # [c,p,d,a] = E_vovv[c,p,a,d]
+        lo['Eoovv'] = DeviceArray((o, o, v, v))      # [r,p,d,e]
+        lo['Eoovv_t'] = DeviceArray((o, o, v, v))    # [m,p,e,d] = E_oovv[m,p,d,e]
+        lo['Er1'] = DeviceArray((o, o, v, v, 1))     # [j,k,b,c] = E - 1/2 E^(bc)
+        lo['r1'] = DeviceArray((o, v, 1))
+        lo['r2'] = DeviceArray((o, o, v, v))
+        ext = {x: v for x in 'abcdexyz'}
+        ext.update({'p': o, 'r': o, 'l': o, 'm': o, 'i': o, 'k': o, 'Q': o, 'X': 1})
+        Z = lo['Z']
+        slices = []
+        for q in range(o):
+            jobs = []
+            for p in range(o):
+                for r in range(o):
+                    case, (ci, cj, ck) = canonical_case(p, q, r)
+                    P = TP_T3[case]
+                    coef = np.zeros(6)
+                    for c, Qp in ((2.0, (0, 1, 2)), (-1.0, (2, 1, 0)), (-1.0, (1, 0, 2))):
+                        comp = (P[Qp[0]], P[Qp[1]], P[Qp[2]])
+                        coef[TP_T3.index(comp)] += c
+                    jobs.append(((p * o + r) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
+            fx = [dict(Q=q)]
+            r2_launches = [
+                plan_term('t3->r2 E_vovv q%d' % q, 'cpda,prdba->rbc',
+                          Operand(lo['Evovv'], 'cpda', (v, o, v, v)), Operand(Z, 'prdba', (o, o, v, v, v)),
+                          Operand(lo['r2'], 'Qrbc', (o, o, v, v)), 1.0, 1.0, fx, ext),
+                plan_term('t3->r2 E_ooov q%d' % q, 'lrpa,prabc->lbc',
+                          Operand(lo['Ecomb'], 'lrpa', (o, o, o, v)), Operand(Z, 'prabc', (o, o, v, v, v)),
+                          Operand(lo['r2'], 'Qlbc', (o, o, v, v)), 1.0, 1.0, fx, ext),
+                plan_term('t3->r1 q%d' % q, 'ikabc,kbcX->iaX',
+                          Operand(Z, 'ikabc', (o, o, v, v, v)), Operand(lo['Er1'], 'QkbcX', (o, o, v, v, 1)),
+                          Operand(lo['r1'], 'iaX', (o, v, 1)), 1.0, 1.0, fx, ext),
+            ]
+            w_launches = [
+                plan_term('t3->W_vooo q%d' % q, 'pread,mped->ram',
+                          Operand(Z, 'pread', (o, o, v, v, v)), Operand(lo['Eoovv_t'], 'mped', (o, o, v, v)),
+                          Operand(self.W_oovo, 'Qram', (o, o, v, o)), 1.0, 1.0, fx, ext),
+                plan_term('t3->W_vvvo q%d' % q, 'preba,rpde->abd',
+                          Operand(Z, 'preba', (o, o, v, v, v)), Operand(lo['Eoovv'], 'rpde', (o, o, v, v)),
+                          Operand(self.W_ovvv, 'Qabd', (o, v, v, v)), -1.0, 1.0, fx, ext),
+            ]
+            slices.append((DeviceArray.from_host(s3_jobs_array(jobs)), len(jobs), r2_launches, w_launches))
+        lo['slices'] = slices
+        self._lower = lo
+
+    def set_t1_dressed(self, t1_fock_ov, E_oovv, E_ooov, E_vovv):
+        """Slices of the T1-dressed Fock/ERIs used by T3 -> r1/r2/W:
+        t1_fock[:o, o:], t1_eris[:o,:o,o:,o:], t1_eris[:o,:o,:o,o:], t1_eris[o:,:o,o:,o:]."""
+        if self._lower is None:
+            self._setup_lower()
+        lo, o = self._lower, self.nocc
+        f = np.asarray(t1_fock_ov, dtype=np.float64)
+        Eo = np.asarray(E_oovv, dtype=np.float64)
+        comb = -np.asarray(E_ooov, dtype=np.float64).transpose(2, 0, 1, 3).copy()  # [l,r,p,a]
+        for l in range(o):
+            comb[l, l] += 0.5 * f
+        lo['Ecomb'].upload(np.ascontiguousarray(comb), self.stream)
+        lo['Evovv'].upload(np.ascontiguousarray(np.asarray(E_vovv, dtype=np.float64).transpose(0, 1, 3, 2)),
+                           self.stream)
+        lo['Eoovv'].upload(np.ascontiguousarray(Eo), self.stream)
+        lo['Eoovv_t'].upload(np.ascontiguousarray(Eo.transpose(0, 1, 3, 2)), self.stream)
+        lo['Er1'].upload(np.ascontiguousarray((Eo - 0.5 * Eo.transpose(0, 1, 3, 2))[..., None]), self.stream)
+
+    def _build_z(self, q):
+        jobs, njobs = self._lower['slices'][q][:2]
+        s = self.stream.handle if self.stream is not None else None
+        L.call("dcc_s3_transform_f64", self._lower['Z'].vp, self.t3.vp, self.nvir, 0.0, jobs.vp, njobs, s)
+
+    def t3_to_r1r2(self):
+        """T3 contributions to r1 and the un-symmetrized r2 (rccsdt.py:253-321); returns host arrays."""
+        lo = self._lower
+        lo['r1'].fill(0.0, self.stream)
+        lo['r2'].fill(0.0, self.stream)
+        for q in range(self.nocc):
+            self._build_z(q)
+            for g in lo['slices'][q][2]:
+                g.run(self.stream)
+        return (lo['r1'].download(stream=self.stream).reshape(self.nocc, self.nvir),
+                lo['r2'].download(stream=self.stream))
+
+    def t3_to_w(self):
+        """Add the T3 terms of rccsdt.py:323-366 to the device W_vooo / W_vvvo (after set_intermediates)."""
+        lo = self._lower
+        for q in range(self.nocc):
+            self._build_z(q)
+            for g in lo['slices'][q][3]:
+                g.run(self.stream)
+
+    def get_w_t3_adjusted(self):
+        """Device W_vooo, W_vvvo (incl. T3 terms) back in the reference layouts."""
+        w_oovo = self.W_oovo.download(stream=self.stream)
+        w_ovvv = self.W_ovvv.download(stream=self.stream)
+        return w_oovo.transpose(2, 3, 0, 1), w_ovvv.transpose(1, 2, 3, 0)
 
     # --------------------------------------------------------------- inputs
     def set_t3(self, t3_local):

@@ -161,6 +161,30 @@ int dcc_oooo_prefactors_f64(const double *W_oooo, const double *F_oo, int64_t no
 int dcc_ring_weights_f64(const double *W_ovov, const double *W_ovvo, int64_t nocc, int64_t nvir, double *WR1,
                          double *WR3a, double *WR3b, dcc_stream_t stream);
 
+/* ------------------------------------------------------ (Q)/[Q] energy */
+
+/*
+ * out[a,b,c,i,j,k] (a,b,c < vpad) = T[ijk][abc] for a,b,c < nvir and 0 in the
+ * padding, from the canonical i<=j<=k block array t3_tri.  The one-GPU form of
+ * the IJK -> ABC redistribution (distr_cc/distribute_t3.py:1544-1642) and of
+ * the per-task gather fill_t3_from_ptr_array (src/rccsdt_q.c:665-828).
+ */
+int dcc_t3_to_abc_f64(const double *t3_tri, int64_t nocc, int64_t nvir, int64_t vpad, double *out,
+                      dcc_stream_t stream);
+
+/*
+ * Per (task, a, b, c, d) of a chunk of (Q) tasks (tasks: int32 (ntask, 4) block
+ * starts a0,b0,c0,d0): t4 <- (t4 + z4) / D with D -> 0 when |D| <= 1e-15,
+ * P4_444 on the occupied indices, times the 24/12/6/0 factor; then
+ * partials[wg] = (<t4, z4>, <t4, z4_sq>) with z4_sq built from vvoo (vpad,vpad,o,o)
+ * and t2 (vpad,vpad,o,o).  t4/z4 are (ntask, bs, bs, bs, bs, o, o, o, o); nocc <= 9.
+ * Replaces t4_add_, e_abcdijkl_division_, t4_spin_summation_inplace_,
+ * t4_multiply_factor_ and the dot products of distr_cc/rccsdt_q.py:1131-1153.
+ */
+int dcc_q_finalize_f64(const double *t4, const double *z4, const int32_t *tasks, int64_t ntask, int64_t blksize,
+                       int64_t nocc, int64_t nvir, int64_t vpad, const double *e_occ, const double *e_vir,
+                       const double *vvoo, const double *t2, double *partials, dcc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

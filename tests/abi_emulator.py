@@ -202,6 +202,73 @@ class Emulator:
                 add((0, 1, 2), (i, j, l), -Fv[l, k], q)
         return 0
 
+    def dcc_t3_to_abc_f64(self, t3, nocc, nvir, vpad, out, stream):
+        o, v, vp = int(nocc), int(nvir), int(vpad)
+        n3 = o * (o + 1) * (o + 2) // 6
+        T = self.dev.view(_ptr(t3), np.float64, n3 * v ** 3).reshape(n3, v, v, v)
+        lin = {}
+        n = 0
+        for i in range(o):
+            for j in range(i, o):
+                for k in range(j, o):
+                    lin[(i, j, k)] = n
+                    n += 1
+        full = np.zeros((vp, vp, vp, o, o, o))
+        for i, j, k in itertools.product(range(o), repeat=3):
+            c = _case(i, j, k)
+            blk = T[lin[tuple(sorted((i, j, k)))]].transpose(_PERMS[c])
+            full[:v, :v, :v, i, j, k] = blk
+        self.dev.view(_ptr(out), np.float64, full.size)[:] = full.ravel()
+        return 0
+
+    def dcc_q_finalize_f64(self, t4, z4, tasks, ntask, bs, nocc, nvir, vpad, e_occ, e_vir, vvoo, t2, partials,
+                           stream):
+        nt, bs, o, v, vp = int(ntask), int(bs), int(nocc), int(nvir), int(vpad)
+        n = nt * bs ** 4 * o ** 4
+        T4 = self.dev.view(_ptr(t4), np.float64, n).reshape(nt, bs, bs, bs, bs, o, o, o, o)
+        Z4 = self.dev.view(_ptr(z4), np.float64, n).reshape(nt, bs, bs, bs, bs, o, o, o, o)
+        tk = self.dev.view(_ptr(tasks), np.int32, 4 * nt).reshape(nt, 4)
+        eo = self.dev.view(_ptr(e_occ), np.float64, o)
+        ev = self.dev.view(_ptr(e_vir), np.float64, vp)
+        VV = self.dev.view(_ptr(vvoo), np.float64, vp * vp * o * o).reshape(vp, vp, o, o)
+        T2 = self.dev.view(_ptr(t2), np.float64, vp * vp * o * o).reshape(vp, vp, o, o)
+        P = self.dev.view(_ptr(partials), np.float64, 2 * nt * bs ** 4).reshape(nt, bs, bs, bs, bs, 2)
+        eocc4 = eo[:, None, None, None] + eo[None, :, None, None] + eo[None, None, :, None] + eo[None, None, None, :]
+
+        def fac(a, b, c, d):
+            if a < b < c < d:
+                return 24.0
+            if (a == b < c < d) or (a < b == c < d) or (a < b < c == d):
+                return 12.0
+            if a == b < c == d:
+                return 6.0
+            return 0.0
+
+        def sw(X, x, y):
+            ax = [0, 1, 2, 3]
+            ax[x], ax[y] = ax[y], ax[x]
+            return X.transpose(ax)
+
+        for t in range(nt):
+            for ia, ib, ic, id_ in itertools.product(range(bs), repeat=4):
+                a, b, c, d = (int(tk[t, 0]) + ia, int(tk[t, 1]) + ib, int(tk[t, 2]) + ic, int(tk[t, 3]) + id_)
+                f = fac(a, b, c, d) if max(a, b, c, d) < v else 0.0
+                if f == 0.0:
+                    P[t, ia, ib, ic, id_] = 0.0
+                    continue
+                den = eocc4 - (ev[a] + ev[b] + ev[c] + ev[d])
+                num = T4[t, ia, ib, ic, id_] + Z4[t, ia, ib, ic, id_]
+                x = np.where(np.abs(den) > 1e-15, num / np.where(den == 0, 1, den), 0.0)
+                x1 = 2 * x - sw(x, 0, 1) - sw(x, 0, 2) - sw(x, 0, 3)
+                x2 = 2 * x1 - sw(x1, 1, 2) - sw(x1, 1, 3)
+                x3 = f * (2 * x2 - sw(x2, 2, 3))
+                zsq = (np.einsum('ij,kl->ijkl', VV[a, b], T2[c, d]) + np.einsum('ik,jl->ijkl', VV[a, c], T2[b, d])
+                       + np.einsum('il,jk->ijkl', VV[a, d], T2[b, c]) + np.einsum('jk,il->ijkl', VV[b, c], T2[a, d])
+                       + np.einsum('jl,ik->ijkl', VV[b, d], T2[a, c]) + np.einsum('kl,ij->ijkl', VV[c, d], T2[a, b]))
+                P[t, ia, ib, ic, id_] = ((x3 * Z4[t, ia, ib, ic, id_]).sum(), (x3 * zsq).sum())
+        self._npart = 0
+        return 0
+
     def dcc_ring_weights_f64(self, Wovov, Wovvo, nocc, nvir, WR1, WR3a, WR3b, stream):
         o, v = int(nocc), int(nvir)
         A = self.dev.view(_ptr(Wovov), np.float64, (o * v) ** 2).reshape(o, v, o, v)

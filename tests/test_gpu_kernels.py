@@ -112,3 +112,59 @@ def test_residual_and_update_vs_oracle(dev, o, v):
     t3o, r3o, n2o, nao = tri.t3_update(o, Ro, t3, s.mo_energy, level_shift=0.05, nvir_diis=nd)
     assert abs(t3_new - t3o).max() <= 1e-13 * abs(t3o).max()
     assert abs(n2 - n2o) <= 1e-12 * n2o + 1e-30 and abs(na - nao) <= 1e-12 * nao + 1e-30
+
+
+@pytest.mark.parametrize("o,v", [(1, 3), (3, 7), (4, 12), (5, 19)])
+def test_t3_to_r1r2_and_w_vs_oracle(dev, o, v):
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    from oracle import rccsdt_tri as tri
+    _say("t3 -> r1/r2/W o=%d v=%d" % (o, v))
+    s = SyntheticSystem(o, v, naux=30, seed=300 + o * v)
+    im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
+    nmo = o + v
+    rng = np.random.default_rng(o + v)
+    E = s.pppp() + 0.01 * rng.normal(size=(nmo,) * 4)
+    E = 0.5 * (E + E.transpose(1, 0, 3, 2))
+    F = s.fock + 0.01 * rng.normal(size=(nmo, nmo))
+    eng = TriplesEngine(o, v)
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.set_t1_dressed(F[:o, o:], E[:o, :o, o:, o:], E[:o, :o, :o, o:], E[o:, :o, o:, o:])
+    r1, r2 = eng.t3_to_r1r2()
+    r1o, r2o = np.zeros((o, v)), np.zeros((o, o, v, v))
+    tri.r1r2_add_t3_tri(o, v, F, E, t3, r1o, r2o)
+    floor = 1e-12 * abs(t3).max() * abs(E).max()
+    assert abs(r1 - r1o).max() <= 1e-12 * abs(r1o).max() + floor
+    assert abs(r2 - r2o).max() <= 1e-12 * abs(r2o).max() + floor
+    eng.t3_to_w()
+    Wv, Wvv = eng.get_w_t3_adjusted()
+    Wvo, Wvvo = im['W_vooo'].copy(), im['W_vvvo'].copy()
+    tri.intermediates_t3_add_t3_tri(o, v, E, t3, Wvo, Wvvo)
+    assert abs(Wv - Wvo).max() <= 1e-12 * abs(Wvo).max()
+    assert abs(Wvv - Wvvo).max() <= 1e-12 * abs(Wvvo).max()
+    # the residual sees the T3-adjusted W_vooo / W_vvvo (rccsdt.py:935-941)
+    eng.residual()
+    im2 = dict(im, W_vooo=Wvo, W_vvvo=Wvvo)
+    Ro = tri.compute_r3_tri(o, v, im2, t2, t3)
+    assert abs(eng.R.download() - Ro).max() <= 1e-12 * abs(Ro).max()
+
+
+@pytest.mark.parametrize("o,v,bs", [(2, 5, 2), (3, 7, 4), (5, 9, 4), (4, 11, 3)])
+def test_q_energy_vs_oracle(dev, o, v, bs):
+    from distr_cc.device import DeviceArray
+    from distr_cc.q_engine import QEngine
+    from distr_cc.synthetic import SyntheticSystem
+    from oracle import q_energy as Q
+    from oracle import t3_layout as TL
+    _say("(Q) o=%d v=%d blksize=%d" % (o, v, bs))
+    s = SyntheticSystem(o, v, naux=30, seed=500 + o * v, sigma_t3=1e-3)
+    pppp, t2, t3 = s.pppp(), s.t2(), s.t3_tri()
+    ref = Q.kernel(o, v, pppp, t2, TL.full_t3_from_tri(t3, o), s.mo_energy, blksize=bs)
+    got = QEngine(o, v, blksize=bs).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy)
+    for g, r in zip(got, ref):
+        assert abs(g - r) <= 1e-12 * abs(r)
+    # job splitting is additive (tests/test_rccsdt_q.py:70-94)
+    parts = [QEngine(o, v, blksize=bs).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy, job_idx=j,
+                                              n_jobs=3) for j in range(3)]
+    assert abs(sum(p[1] for p in parts) - got[1]) <= 1e-12 * abs(got[1])
