@@ -24,6 +24,26 @@
 namespace {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Element type -> MFMA instruction and accumulator layout.
+//   double: v_mfma_f64_16x16x4f64, lane holds D[lane/16 + 4r][lane%16]
+//   float : v_mfma_f32_16x16x4f32, lane holds D[4(lane/16) + r][lane%16]
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef dbl4 acc_t;
+    static __device__ __forceinline__ acc_t step(double a, double b, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return lk + 4 * r; }
+};
+template <> struct Mfma<float> {
+    typedef flt4 acc_t;
+    static __device__ __forceinline__ acc_t step(float a, float b, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
+};
 
 constexpr int kThreads = 256;
 constexpr int kBK = 16;
@@ -78,12 +98,13 @@ __device__ __forceinline__ void kadvance(const KMap &m, KIter &it) {
     }
 }
 
+template <typename T>
 struct ContractArgs {
     int M, N, K;
     double alpha, beta;
-    const double *A;
-    const double *B;
-    double *C;
+    const T *A;
+    const T *B;
+    T *C;
     dcc_idxmap am, bn, cm, cn;
     KMap ak, bk;
     const dcc_batch *batches;
@@ -101,15 +122,17 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t L, int64_t total) {
     return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-template <int BM, int BN, bool A_KFAST, bool B_NFAST>
-__global__ __launch_bounds__(kThreads) void contract_f64_kernel(const ContractArgs p) {
+template <typename T, int BM, int BN, bool A_KFAST, bool B_NFAST>
+__global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T> p) {
+    typedef Mfma<T> MF;
+    typedef typename MF::acc_t acc_t;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int LDA = BM + 1, LDB = BN + 1;
     constexpr int A_PER_T = BM * kBK / kThreads;
     constexpr int B_PER_T = BN * kBK / kThreads;
-    __shared__ double As[2][kBK][LDA];
-    __shared__ double Bs[2][kBK][LDB];
+    __shared__ T As[2][kBK][LDA];
+    __shared__ T Bs[2][kBK][LDB];
 
     const int tid = threadIdx.x;
     const int64_t L = xcd_remap((int64_t)blockIdx.x, p.total_tiles);
@@ -158,16 +181,16 @@ __global__ __launch_bounds__(kThreads) void contract_f64_kernel(const ContractAr
     KIter ka = kinit(p.ak, (uint32_t)a_kk);
     KIter kb = kinit(p.bk, (uint32_t)b_kk);
 
-    double a_reg[A_PER_T], b_reg[B_PER_T];
+    T a_reg[A_PER_T], b_reg[B_PER_T];
     auto load_tiles = [&](int k0) {
         const bool ak_ok = k0 + a_kk < p.K;
         const bool bk_ok = k0 + b_kk < p.K;
 #pragma unroll
         for (int r = 0; r < A_PER_T; ++r)
-            a_reg[r] = (ak_ok && ((a_ok >> r) & 1u)) ? p.A[a_row[r] + ka.off] : 0.0;
+            a_reg[r] = (ak_ok && ((a_ok >> r) & 1u)) ? p.A[a_row[r] + ka.off] : T(0);
 #pragma unroll
         for (int r = 0; r < B_PER_T; ++r)
-            b_reg[r] = (bk_ok && ((b_ok >> r) & 1u)) ? p.B[b_col[r] + kb.off] : 0.0;
+            b_reg[r] = (bk_ok && ((b_ok >> r) & 1u)) ? p.B[b_col[r] + kb.off] : T(0);
         kadvance(p.ak, ka);
         kadvance(p.bk, kb);
     };
@@ -181,11 +204,11 @@ __global__ __launch_bounds__(kThreads) void contract_f64_kernel(const ContractAr
     const int wave = tid / 64, lane = tid % 64;
     const int wm = wave / 2, wn = wave % 2;
     const int lrow = lane % 16, lk = lane / 16;
-    dbl4 acc[FM][FN];
+    acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = dbl4{0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     const int nk = (p.K + kBK - 1) / kBK;
     load_tiles(0);
@@ -197,7 +220,7 @@ __global__ __launch_bounds__(kThreads) void contract_f64_kernel(const ContractAr
 #pragma unroll
         for (int ks = 0; ks < kBK / 4; ++ks) {
             const int kr = ks * 4 + lk;
-            double af[FM], bf[FN];
+            T af[FM], bf[FN];
 #pragma unroll
             for (int i = 0; i < FM; ++i) af[i] = As[buf][kr][wm * WM + i * 16 + lrow];
 #pragma unroll
@@ -206,13 +229,13 @@ __global__ __launch_bounds__(kThreads) void contract_f64_kernel(const ContractAr
             for (int i = 0; i < FM; ++i)
 #pragma unroll
                 for (int j = 0; j < FN; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
         }
         if (kt + 1 < nk) store_tiles(buf ^ 1);
         __syncthreads();
     }
 
-    // ---- epilogue: for v_mfma_f64_16x16x4f64 lane holds D[lane/16 + 4*r][lane%16], r = 0..3 ----
+    // ---- epilogue: accumulator row of (lane, r) from Mfma<T>::row, column lane%16 ----
     int64_t ccol[FN];
     uint32_t cok = 0;
 #pragma unroll
@@ -229,27 +252,27 @@ __global__ __launch_bounds__(kThreads) void contract_f64_kernel(const ContractAr
         uint32_t rok = 0;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t m = m0 + wm * WM + i * 16 + lk + 4 * r;
+            const int64_t m = m0 + wm * WM + i * 16 + MF::row(lk, r);
             const bool ok = m < m_end;
             rok |= (ok ? 1u : 0u) << r;
             crow[r] = ok ? bt.c_off + map_off(p.cm, (uint32_t)m, batch) : 0;
         }
-        double old[4][FN];
+        double old[4][FN];  // C read in its own precision, combined in FP64
         if (p.beta != 0.0) {
 #pragma unroll
             for (int r = 0; r < 4; ++r)
 #pragma unroll
                 for (int j = 0; j < FN; ++j)
-                    old[r][j] = (((rok >> r) & (cok >> j)) & 1u) ? p.C[crow[r] + ccol[j]] : 0.0;
+                    old[r][j] = (((rok >> r) & (cok >> j)) & 1u) ? (double)p.C[crow[r] + ccol[j]] : 0.0;
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
                 if (!(((rok >> r) & (cok >> j)) & 1u)) continue;
-                double v = p.alpha * acc[i][j][r];
+                double v = p.alpha * (double)acc[i][j][r];
                 if (p.beta != 0.0) v += p.beta * old[r][j];
-                p.C[crow[r] + ccol[j]] = v;
+                p.C[crow[r] + ccol[j]] = (T)v;
             }
     }
 }
@@ -275,8 +298,8 @@ KMap make_kmap(const dcc_idxmap &m) {
     return k;
 }
 
-template <int BM, int BN>
-int launch(ContractArgs &args, int64_t max_rows, bool a_kfast, bool b_nfast, hipStream_t stream) {
+template <typename T, int BM, int BN>
+int launch(ContractArgs<T> &args, int64_t max_rows, bool a_kfast, bool b_nfast, hipStream_t stream) {
     args.tiles_m = (int)((max_rows + BM - 1) / BM);
     args.tiles_n = (int)((args.N + BN - 1) / BN);
     const int64_t nbatch = args.total_tiles;  // caller passes nbatch here
@@ -287,13 +310,13 @@ int launch(ContractArgs &args, int64_t max_rows, bool a_kfast, bool b_nfast, hip
     }
     const dim3 grid((unsigned)args.total_tiles), block(kThreads);
     if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_f64_kernel<BM, BN, true, true>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, true>), grid, block, 0, stream, args);
     else if (a_kfast)
-        hipLaunchKernelGGL((contract_f64_kernel<BM, BN, true, false>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, false>), grid, block, 0, stream, args);
     else if (b_nfast)
-        hipLaunchKernelGGL((contract_f64_kernel<BM, BN, false, true>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, true>), grid, block, 0, stream, args);
     else
-        hipLaunchKernelGGL((contract_f64_kernel<BM, BN, false, false>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, false>), grid, block, 0, stream, args);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -303,12 +326,10 @@ bool valid_map(const dcc_idxmap &m) {
     return m.e1 >= 1 && m.e2 >= 1 && m.e1 * m.e2 < (int64_t)1 << 32;
 }
 
-}  // namespace
-
-extern "C" int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, double beta, const double *A,
-                                dcc_idxmap am, dcc_idxmap ak, const double *B, dcc_idxmap bk, dcc_idxmap bn,
-                                double *C, dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches,
-                                int64_t nbatch, int64_t max_rows, dcc_stream_t stream) {
+template <typename T>
+int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, const T *A, dcc_idxmap am,
+                   dcc_idxmap ak, const T *B, dcc_idxmap bk, dcc_idxmap bn, T *C, dcc_idxmap cm, dcc_idxmap cn,
+                   const dcc_batch *batches, int64_t nbatch, int64_t max_rows, dcc_stream_t stream) {
     DCC_CHECK(M >= 0 && N >= 0 && K >= 0 && nbatch >= 0 && max_rows >= 0, "negative extent");
     DCC_CHECK(M < ((int64_t)1 << 31) && N < ((int64_t)1 << 31) && K < ((int64_t)1 << 31),
               "extent exceeds 2^31");
@@ -319,7 +340,7 @@ extern "C" int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, d
     if (M == 0 || N == 0 || nbatch == 0 || max_rows == 0) return 0;
     DCC_CHECK(A != nullptr && B != nullptr && C != nullptr && batches != nullptr, "null pointer");
 
-    ContractArgs args;
+    ContractArgs<T> args;
     args.M = (int)M;
     args.N = (int)N;
     args.K = (int)K;
@@ -350,8 +371,26 @@ extern "C" int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, d
         tbm = 64;
         tbn = 64;
     }
-    if (tbm == 128 && tbn == 128) return launch<128, 128>(args, max_rows, a_kfast, b_nfast, s);
-    if (tbm == 128 && tbn == 64) return launch<128, 64>(args, max_rows, a_kfast, b_nfast, s);
-    if (tbm == 64 && tbn == 128) return launch<64, 128>(args, max_rows, a_kfast, b_nfast, s);
-    return launch<64, 64>(args, max_rows, a_kfast, b_nfast, s);
+    if (tbm == 128 && tbn == 128) return launch<T, 128, 128>(args, max_rows, a_kfast, b_nfast, s);
+    if (tbm == 128 && tbn == 64) return launch<T, 128, 64>(args, max_rows, a_kfast, b_nfast, s);
+    if (tbm == 64 && tbn == 128) return launch<T, 64, 128>(args, max_rows, a_kfast, b_nfast, s);
+    return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, s);
+}
+
+}  // namespace
+
+extern "C" int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, double beta, const double *A,
+                                dcc_idxmap am, dcc_idxmap ak, const double *B, dcc_idxmap bk, dcc_idxmap bn,
+                                double *C, dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches,
+                                int64_t nbatch, int64_t max_rows, dcc_stream_t stream) {
+    return contract_entry<double>(M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch,
+                                  max_rows, stream);
+}
+
+extern "C" int dcc_contract_f32(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
+                                dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn,
+                                float *C, dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches,
+                                int64_t nbatch, int64_t max_rows, dcc_stream_t stream) {
+    return contract_entry<float>(M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch,
+                                 max_rows, stream);
 }

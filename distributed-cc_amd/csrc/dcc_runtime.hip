@@ -14,7 +14,8 @@ void clear_error() { g_last_error.clear(); }
 
 namespace {
 
-__global__ void fill_f64_kernel(double *__restrict__ dst, int64_t n, double value) {
+template <typename T>
+__global__ void fill_kernel(T *__restrict__ dst, int64_t n, T value) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) dst[i] = value;
@@ -86,7 +87,18 @@ int dcc_fill_f64(double *dst, int64_t n, double value, dcc_stream_t stream) {
     DCC_CHECK(dst != nullptr, "null dst");
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(fill_f64_kernel, dim3((unsigned)blocks), dim3(256), 0, dcc::as_stream(stream), dst, n,
+    hipLaunchKernelGGL(fill_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, dcc::as_stream(stream), dst, n,
+                       value);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+int dcc_fill_f32(float *dst, int64_t n, float value, dcc_stream_t stream) {
+    if (n <= 0) return 0;
+    DCC_CHECK(dst != nullptr, "null dst");
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, dcc::as_stream(stream), dst, n,
                        value);
     DCC_LAUNCH_CHECK();
     return 0;

@@ -39,8 +39,8 @@ __device__ __forceinline__ void decode_orbit(int o, int &A, int &B, int &C) {
 __host__ __device__ __forceinline__ int64_t n_orbits(int64_t nt) { return nt * (nt + 1) * (nt + 2) / 6; }
 
 // Load the six permuted tiles of orbit (R0,R1,R2) of block `src` into LDS.
-__device__ __forceinline__ void load_orbit(double (*tiles)[TE], const double *__restrict__ src, int nvir,
-                                           const int R[3]) {
+template <typename T>
+__device__ __forceinline__ void load_orbit(T (*tiles)[TE], const T *__restrict__ src, int nvir, const int R[3]) {
     const int64_t v = nvir, v2 = v * v;
     for (int s = 0; s < 6; ++s) {
         const int t0 = R[kP[s][0]], t1 = R[kP[s][1]], t2 = R[kP[s][2]];
@@ -48,7 +48,7 @@ __device__ __forceinline__ void load_orbit(double (*tiles)[TE], const double *__
         for (int h = 0; h < kPer; ++h) {
             const int e = threadIdx.x + h * kThreads;
             const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
-            double val = 0.0;
+            T val = T(0);
             if (g0 < nvir && g1 < nvir && g2 < nvir) val = src[g0 * v2 + g1 * v + g2];
             tiles[s][e] = val;
         }
@@ -65,7 +65,8 @@ __device__ __forceinline__ bool duplicate_tile(int s, const int R[3]) {
 
 // Value of X.transpose(p)[x] for x = local element e of output tile s.
 // y[p[n]] = x[n]; y lives in tile s' with ranges R[s'[m]] = R[s[pinv[m]]].
-__device__ __forceinline__ double permuted(const double (*tiles)[TE], int s, int p, int e) {
+template <typename T>
+__device__ __forceinline__ double permuted(const T (*tiles)[TE], int s, int p, int e) {
     const int x[3] = {e >> 6, (e >> 3) & 7, e & 7};
     int y[3], sy[3];
 #pragma unroll
@@ -74,14 +75,14 @@ __device__ __forceinline__ double permuted(const double (*tiles)[TE], int s, int
         sy[kP[p][n]] = kP[s][n];
     }
     const int sp = perm_index(sy[0], sy[1], sy[2]);
-    return tiles[sp][(y[0] << 6) | (y[1] << 3) | y[2]];
+    return (double)tiles[sp][(y[0] << 6) | (y[1] << 3) | y[2]];
 }
 
-__global__ __launch_bounds__(kThreads) void s3_transform_kernel(double *__restrict__ out,
-                                                                const double *__restrict__ src, int nvir,
-                                                                double beta, const dcc_s3_job *__restrict__ jobs,
-                                                                int norb) {
-    __shared__ double tiles[6][TE];
+template <typename T>
+__global__ __launch_bounds__(kThreads) void s3_transform_kernel(T *__restrict__ out, const T *__restrict__ src,
+                                                                int nvir, double beta,
+                                                                const dcc_s3_job *__restrict__ jobs, int norb) {
+    __shared__ T tiles[6][TE];
     const int64_t job = blockIdx.x / norb;
     const int orbit = blockIdx.x - (int)(job * norb);
     int R[3];
@@ -90,7 +91,7 @@ __global__ __launch_bounds__(kThreads) void s3_transform_kernel(double *__restri
     load_orbit(tiles, src + jb.src, nvir, R);
     __syncthreads();
     const int64_t v = nvir, v2 = v * v;
-    double *dst = out + jb.dst;
+    T *dst = out + jb.dst;
     for (int s = 0; s < 6; ++s) {
         if (duplicate_tile(s, R)) continue;
         const int t0 = R[kP[s][0]], t1 = R[kP[s][1]], t2 = R[kP[s][2]];
@@ -103,27 +104,29 @@ __global__ __launch_bounds__(kThreads) void s3_transform_kernel(double *__restri
 #pragma unroll
             for (int p = 0; p < 6; ++p)
                 if (jb.coef[p] != 0.0) val += jb.coef[p] * permuted(tiles, s, p, e);
-            double *o = dst + g0 * v2 + g1 * v + g2;
-            if (beta != 0.0) val += beta * (*o);
-            *o = val;
+            T *o = dst + g0 * v2 + g1 * v + g2;
+            if (beta != 0.0) val += beta * (double)(*o);
+            *o = (T)val;
         }
     }
 }
 
 // Fused P3_full projection + denominator + t3 update + norms (one tile orbit).
-__global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const double *__restrict__ R, double *__restrict__ t3,
-                                                               double *__restrict__ r3_out, int nvir,
+// Storage in T, arithmetic (projection, division, update, norms) in FP64.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const T *__restrict__ R, T *__restrict__ t3,
+                                                               T *__restrict__ r3_out, int nvir,
                                                                const int *__restrict__ blk_ijk,
                                                                const double *__restrict__ eia, int nvir_diis,
                                                                double *__restrict__ partials, int norb) {
-    __shared__ double tiles[6][TE];
+    __shared__ T tiles[6][TE];
     __shared__ double red[2][kThreads / 64];
     const int64_t blk = blockIdx.x / norb;
     const int orbit = blockIdx.x - (int)(blk * norb);
     int Rg[3];
     decode_orbit(orbit, Rg[0], Rg[1], Rg[2]);
     const int64_t v = nvir, v2 = v * v, v3 = v2 * v;
-    const double *Rb = R + blk * v3;
+    const T *Rb = R + blk * v3;
     load_orbit(tiles, Rb, nvir, Rg);
     const int oi = blk_ijk[3 * blk], oj = blk_ijk[3 * blk + 1], ok = blk_ijk[3 * blk + 2];
     const double *ei = eia + (int64_t)oi * v, *ej = eia + (int64_t)oj * v, *ek = eia + (int64_t)ok * v;
@@ -143,8 +146,8 @@ __global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const double *__r
             const double own = permuted(tiles, s, 0, e);
             const double r = (own - sym / 6.0) / (ei[g0] + ej[g1] + ek[g2]);
             const int64_t off = blk * v3 + g0 * v2 + g1 * v + g2;
-            t3[off] += r;
-            if (r3_out != nullptr) r3_out[off] = r;
+            t3[off] = (T)((double)t3[off] + r);
+            if (r3_out != nullptr) r3_out[off] = (T)r;
             nrm += r * r;
             if (g0 < nvir_diis && g1 < nvir_diis && g2 < nvir_diis) nact += r * r;
         }
@@ -198,20 +201,53 @@ __global__ __launch_bounds__(256) void reduce_pairs_kernel(const double *__restr
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double beta, const dcc_s3_job *jobs,
-                         int64_t njobs, dcc_stream_t stream) {
+template <typename T>
+int s3_transform(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_job *jobs, int64_t njobs,
+                 dcc_stream_t stream) {
     DCC_CHECK(nvir >= 0 && njobs >= 0, "negative extent");
     if (nvir == 0 || njobs == 0) return 0;
     DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
     const int64_t nt = (nvir + E - 1) / E;
     const int64_t norb = n_orbits(nt);
     DCC_CHECK(norb * njobs < ((int64_t)1 << 31), "grid too large");
-    hipLaunchKernelGGL(s3_transform_kernel, dim3((unsigned)(norb * njobs)), dim3(kThreads), 0,
+    hipLaunchKernelGGL(s3_transform_kernel<T>, dim3((unsigned)(norb * njobs)), dim3(kThreads), 0,
                        dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)norb);
     DCC_LAUNCH_CHECK();
     return 0;
+}
+
+template <typename T>
+int r3_finalize(const T *R, T *t3, T *r3_out, int64_t nblk, int64_t nvir, const int32_t *blk_ijk, const double *eia,
+                int64_t nocc, int64_t nvir_diis, double *partials, dcc_stream_t stream) {
+    DCC_CHECK(nblk >= 0 && nvir >= 0 && nocc >= 0, "negative extent");
+    if (nblk == 0 || nvir == 0) return 0;
+    DCC_CHECK(R != nullptr && t3 != nullptr && blk_ijk != nullptr && eia != nullptr && partials != nullptr,
+              "null pointer");
+    const int64_t nt = (nvir + E - 1) / E;
+    const int64_t norb = n_orbits(nt);
+    DCC_CHECK(norb * nblk < ((int64_t)1 << 31), "grid too large");
+    if (nvir_diis < 0 || nvir_diis > nvir) nvir_diis = nvir;
+    hipLaunchKernelGGL(r3_finalize_kernel<T>, dim3((unsigned)(norb * nblk)), dim3(kThreads), 0,
+                       dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis, partials,
+                       (int)norb);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double beta, const dcc_s3_job *jobs,
+                         int64_t njobs, dcc_stream_t stream) {
+    return s3_transform<double>(out, src, nvir, beta, jobs, njobs, stream);
+}
+
+int dcc_s3_transform_f32(float *out, const float *src, int64_t nvir, double beta, const dcc_s3_job *jobs,
+                         int64_t njobs, dcc_stream_t stream) {
+    return s3_transform<float>(out, src, nvir, beta, jobs, njobs, stream);
 }
 
 int64_t dcc_r3_finalize_tiles(int64_t nvir) {
@@ -222,18 +258,13 @@ int64_t dcc_r3_finalize_tiles(int64_t nvir) {
 int dcc_r3_finalize_f64(const double *R, double *t3, double *r3_out, int64_t nblk, int64_t nvir,
                         const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
                         double *partials, dcc_stream_t stream) {
-    DCC_CHECK(nblk >= 0 && nvir >= 0 && nocc >= 0, "negative extent");
-    if (nblk == 0 || nvir == 0) return 0;
-    DCC_CHECK(R != nullptr && t3 != nullptr && blk_ijk != nullptr && eia != nullptr && partials != nullptr,
-              "null pointer");
-    const int64_t norb = dcc_r3_finalize_tiles(nvir);
-    DCC_CHECK(norb * nblk < ((int64_t)1 << 31), "grid too large");
-    if (nvir_diis < 0 || nvir_diis > nvir) nvir_diis = nvir;
-    hipLaunchKernelGGL(r3_finalize_kernel, dim3((unsigned)(norb * nblk)), dim3(kThreads), 0,
-                       dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis, partials,
-                       (int)norb);
-    DCC_LAUNCH_CHECK();
-    return 0;
+    return r3_finalize<double>(R, t3, r3_out, nblk, nvir, blk_ijk, eia, nocc, nvir_diis, partials, stream);
+}
+
+int dcc_r3_finalize_f32(const float *R, float *t3, float *r3_out, int64_t nblk, int64_t nvir,
+                        const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
+                        double *partials, dcc_stream_t stream) {
+    return r3_finalize<float>(R, t3, r3_out, nblk, nvir, blk_ijk, eia, nocc, nvir_diis, partials, stream);
 }
 
 int dcc_reduce_pairs_f64(const double *x, int64_t n, double *out, dcc_stream_t stream) {

@@ -10,16 +10,17 @@ namespace {
 // distr_cc/rccsdt.py:726-731).  The conditions are those of
 // distr_cc/rccsdt.py:630-723: each ordered pair (l,m) of the sum over
 // W_oooo[l,m,.,.] T[l,m,.] (and each l of F_oo) is reached exactly once.
-__global__ void oooo_prefactors_kernel(const double *__restrict__ W, const double *__restrict__ F, int no,
-                                       const int *__restrict__ ijk, int n3, double *__restrict__ pfs) {
+template <typename T>
+__global__ void oooo_prefactors_kernel(const T *__restrict__ W, const T *__restrict__ F, int no,
+                                       const int *__restrict__ ijk, int n3, T *__restrict__ pfs) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n3 * n3) return;
     const int p = (int)(t / n3), q = (int)(t % n3);
     const int a = ijk[3 * p], b = ijk[3 * p + 1], c = ijk[3 * p + 2];  // o00, o10, o20
     const int i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
     const int64_t n = no;
-    auto w = [&](int l, int m, int x, int y) { return W[((l * n + m) * n + x) * n + y]; };
-    auto f = [&](int l, int x) { return F[l * n + x]; };
+    auto w = [&](int l, int m, int x, int y) { return (double)W[((l * n + m) * n + x) * n + y]; };
+    auto f = [&](int l, int x) { return (double)F[l * n + x]; };
     double s[6] = {0, 0, 0, 0, 0, 0};
     const bool ab = a != b, bc = b != c;
     // part 0: W_oooo[l,m,i,j] T[l,m,k]
@@ -57,7 +58,7 @@ __global__ void oooo_prefactors_kernel(const double *__restrict__ W, const doubl
     if (ab && i == b && j == c) s[4] -= f(a, k);
     const int64_t nn = (int64_t)n3 * n3;
 #pragma unroll
-    for (int u = 0; u < 6; ++u) pfs[u * nn + t] = s[u];
+    for (int u = 0; u < 6; ++u) pfs[u * nn + t] = (T)s[u];
 }
 
 // Weight matrices of the ring terms, all (o*v) x (o*v) with rows (r, x) and
@@ -65,9 +66,9 @@ __global__ void oooo_prefactors_kernel(const double *__restrict__ W, const doubl
 //   WR1[r,x,l,d]  = -W_ovov[l,x,r,d]
 //   WR3a[r,x,l,d] =  W_ovvo[l,x,d,r]
 //   WR3b[r,x,l,d] = -1/2 (W_ovov[l,x,r,d] + W_ovvo[l,x,d,r])
-__global__ void ring_weights_kernel(const double *__restrict__ Wovov, const double *__restrict__ Wovvo, int no,
-                                    int nv, double *__restrict__ WR1, double *__restrict__ WR3a,
-                                    double *__restrict__ WR3b) {
+template <typename T>
+__global__ void ring_weights_kernel(const T *__restrict__ Wovov, const T *__restrict__ Wovvo, int no, int nv,
+                                    T *__restrict__ WR1, T *__restrict__ WR3a, T *__restrict__ WR3b) {
     const int64_t o = no, v = nv;
     const int64_t total = o * v * o * v;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -75,9 +76,39 @@ __global__ void ring_weights_kernel(const double *__restrict__ Wovov, const doub
     const int64_t d = t % v, l = (t / v) % o, x = (t / (v * o)) % v, r = t / (v * o * v);
     const double ovov = Wovov[((l * v + x) * o + r) * v + d];
     const double ovvo = Wovvo[((l * v + x) * v + d) * o + r];
-    WR1[t] = -ovov;
-    WR3a[t] = ovvo;
-    WR3b[t] = -0.5 * (ovov + ovvo);
+    WR1[t] = (T)(-ovov);
+    WR3a[t] = (T)ovvo;
+    WR3b[t] = (T)(-0.5 * (ovov + ovvo));
+}
+
+}  // namespace
+
+namespace {
+
+template <typename T>
+int oooo_prefactors(const T *W_oooo, const T *F_oo, int64_t nocc, const int32_t *blk_ijk, int64_t nblk, T *pfs,
+                    dcc_stream_t stream) {
+    DCC_CHECK(nocc >= 0 && nblk >= 0, "negative extent");
+    if (nblk == 0) return 0;
+    DCC_CHECK(W_oooo && F_oo && blk_ijk && pfs, "null pointer");
+    const int64_t total = nblk * nblk;
+    hipLaunchKernelGGL(oooo_prefactors_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       dcc::as_stream(stream), W_oooo, F_oo, (int)nocc, blk_ijk, (int)nblk, pfs);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename T>
+int ring_weights(const T *W_ovov, const T *W_ovvo, int64_t nocc, int64_t nvir, T *WR1, T *WR3a, T *WR3b,
+                 dcc_stream_t stream) {
+    DCC_CHECK(nocc >= 0 && nvir >= 0, "negative extent");
+    const int64_t total = nocc * nvir * nocc * nvir;
+    if (total == 0) return 0;
+    DCC_CHECK(W_ovov && W_ovvo && WR1 && WR3a && WR3b, "null pointer");
+    hipLaunchKernelGGL(ring_weights_kernel<T>, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                       dcc::as_stream(stream), W_ovov, W_ovvo, (int)nocc, (int)nvir, WR1, WR3a, WR3b);
+    DCC_LAUNCH_CHECK();
+    return 0;
 }
 
 }  // namespace
@@ -86,26 +117,22 @@ extern "C" {
 
 int dcc_oooo_prefactors_f64(const double *W_oooo, const double *F_oo, int64_t nocc, const int32_t *blk_ijk,
                             int64_t nblk, double *pfs, dcc_stream_t stream) {
-    DCC_CHECK(nocc >= 0 && nblk >= 0, "negative extent");
-    if (nblk == 0) return 0;
-    DCC_CHECK(W_oooo && F_oo && blk_ijk && pfs, "null pointer");
-    const int64_t total = nblk * nblk;
-    hipLaunchKernelGGL(oooo_prefactors_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       dcc::as_stream(stream), W_oooo, F_oo, (int)nocc, blk_ijk, (int)nblk, pfs);
-    DCC_LAUNCH_CHECK();
-    return 0;
+    return oooo_prefactors<double>(W_oooo, F_oo, nocc, blk_ijk, nblk, pfs, stream);
+}
+
+int dcc_oooo_prefactors_f32(const float *W_oooo, const float *F_oo, int64_t nocc, const int32_t *blk_ijk,
+                            int64_t nblk, float *pfs, dcc_stream_t stream) {
+    return oooo_prefactors<float>(W_oooo, F_oo, nocc, blk_ijk, nblk, pfs, stream);
 }
 
 int dcc_ring_weights_f64(const double *W_ovov, const double *W_ovvo, int64_t nocc, int64_t nvir, double *WR1,
                          double *WR3a, double *WR3b, dcc_stream_t stream) {
-    DCC_CHECK(nocc >= 0 && nvir >= 0, "negative extent");
-    const int64_t total = nocc * nvir * nocc * nvir;
-    if (total == 0) return 0;
-    DCC_CHECK(W_ovov && W_ovvo && WR1 && WR3a && WR3b, "null pointer");
-    hipLaunchKernelGGL(ring_weights_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
-                       dcc::as_stream(stream), W_ovov, W_ovvo, (int)nocc, (int)nvir, WR1, WR3a, WR3b);
-    DCC_LAUNCH_CHECK();
-    return 0;
+    return ring_weights<double>(W_ovov, W_ovvo, nocc, nvir, WR1, WR3a, WR3b, stream);
+}
+
+int dcc_ring_weights_f32(const float *W_ovov, const float *W_ovvo, int64_t nocc, int64_t nvir, float *WR1,
+                         float *WR3a, float *WR3b, dcc_stream_t stream) {
+    return ring_weights<float>(W_ovov, W_ovvo, nocc, nvir, WR1, WR3a, WR3b, stream);
 }
 
 }  // extern "C"

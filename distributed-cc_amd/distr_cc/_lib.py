@@ -54,6 +54,7 @@ _SIGS = {
     "dcc_memcpy_dtoh": ([c_vp, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_memcpy_dtod": ([c_vp, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_fill_f64": ([c_vp, c_i64, c_dbl, c_vp], ctypes.c_int),
+    "dcc_fill_f32": ([c_vp, c_i64, ctypes.c_float, c_vp], ctypes.c_int),
     "dcc_stream_create": ([ctypes.POINTER(c_vp)], ctypes.c_int),
     "dcc_stream_destroy": ([c_vp], ctypes.c_int),
     "dcc_stream_synchronize": ([c_vp], ctypes.c_int),
@@ -64,13 +65,20 @@ _SIGS = {
     "dcc_event_elapsed_ms": ([ctypes.POINTER(ctypes.c_float), c_vp, c_vp], ctypes.c_int),
     "dcc_contract_f64": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap, IdxMap,
                           c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_vp], ctypes.c_int),
+    "dcc_contract_f32": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap, IdxMap,
+                          c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_transform_f64": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
+    "dcc_s3_transform_f32": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_r3_finalize_tiles": ([c_i64], c_i64),
     "dcc_r3_finalize_f64": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp],
+                            ctypes.c_int),
+    "dcc_r3_finalize_f32": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp],
                             ctypes.c_int),
     "dcc_reduce_pairs_f64": ([c_vp, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_oooo_prefactors_f64": ([c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_ring_weights_f64": ([c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp], ctypes.c_int),
+    "dcc_ring_weights_f32": ([c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp], ctypes.c_int),
+    "dcc_oooo_prefactors_f32": ([c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_t3_to_abc_f64": ([c_vp, c_i64, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_q_finalize_f64": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_vp], ctypes.c_int),

@@ -107,8 +107,8 @@ class DeviceArray:
     def zeros(cls, shape, dtype=np.float64, stream=None):
         d = cls(shape, dtype)
         if d.nbytes:
-            if d.dtype == np.float64:
-                L.call("dcc_fill_f64", d.vp, d.size, 0.0, stream.handle if stream else None)
+            if d.dtype in (np.float64, np.float32):
+                d.fill(0.0, stream)
             else:
                 d.upload(np.zeros(d.shape, d.dtype), stream)
         return d
@@ -129,8 +129,8 @@ class DeviceArray:
         return out
 
     def fill(self, value, stream=None):
-        assert self.dtype == np.float64
-        L.call("dcc_fill_f64", self.vp, self.size, float(value), stream.handle if stream else None)
+        fn = {np.dtype(np.float64): "dcc_fill_f64", np.dtype(np.float32): "dcc_fill_f32"}[self.dtype]
+        L.call(fn, self.vp, self.size, float(value), stream.handle if stream else None)
 
     def free(self):
         if self._owner and self.ptr:

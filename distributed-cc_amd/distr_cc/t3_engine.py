@@ -80,11 +80,18 @@ def table_map(table_dev, bstride):
     return L.IdxMap(table_dev.ptr, int(bstride), 1, 1, 0, 0, 0)
 
 
+def _suffix(dtype):
+    return {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32"}[np.dtype(dtype)]
+
+
 class GemmLaunch:
-    """One dcc_contract_f64 call with fixed operands, maps and batch table."""
+    """One dcc_contract_f64 / dcc_contract_f32 call with fixed operands, maps and batch table."""
 
     def __init__(self, name, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, max_rows,
                  flops, keep=()):
+        if not (A.dtype == B.dtype == C.dtype):
+            raise TypeError("%s: operand dtypes differ (%s, %s, %s)" % (name, A.dtype, B.dtype, C.dtype))
+        self.fn = "dcc_contract_" + _suffix(A.dtype)
         self.name = name
         self.args = (int(M), int(N), int(K), float(alpha), float(beta), ctypes.c_void_p(A.ptr), am, ak,
                      ctypes.c_void_p(B.ptr), bk, bn, ctypes.c_void_p(C.ptr), cm, cn)
@@ -98,7 +105,7 @@ class GemmLaunch:
         self.keep = keep  # device tables referenced by maps
 
     def run(self, stream):
-        L.call("dcc_contract_f64", *self.args, self.batches.vp, self.nbatch, self.max_rows,
+        L.call(self.fn, *self.args, self.batches.vp, self.nbatch, self.max_rows,
                stream.handle if stream is not None else None)
 
 
@@ -191,10 +198,15 @@ def s3_jobs_array(jobs):
 class TriplesEngine:
     """Holds the device state of one RCCSDT triples update for (nocc, nvir)."""
 
-    def __init__(self, nocc, nvir, stream=None, device=0):
+    def __init__(self, nocc, nvir, stream=None, device=0, dtype=np.float64):
+        """``dtype`` float64 is the plain FP64 path; float32 stores T3, R, Y and the
+        intermediates in FP32 and contracts on FP32 MFMA, keeping the denominator,
+        the projection arithmetic and the norms in FP64."""
         device_init(device)
         o, v = int(nocc), int(nvir)
         self.nocc, self.nvir = o, v
+        self.dtype = dt = np.dtype(dtype)
+        self.sfx = _suffix(dt)
         self.n3 = n3 = nocc3(o)
         self.stream = stream
         self.triples = canonical_triples(o)
@@ -203,24 +215,24 @@ class TriplesEngine:
         self.npairs = len(self.pairs)
         v3 = v ** 3
         # --- device buffers ---
-        self.t3 = DeviceArray((n3, v, v, v))
-        self.R = DeviceArray((n3, v, v, v))
-        self.Y = DeviceArray((self.npairs, o, v, v, v))
-        self.t2 = DeviceArray((o, o, v, v))
-        self.F_oo = DeviceArray((o, o))
-        self.F_vv = DeviceArray((v, v))
-        self.W_oooo = DeviceArray((o, o, o, o))
-        self.W_ovvo = DeviceArray((o, v, v, o))
-        self.W_ovov = DeviceArray((o, v, o, v))
-        self.W_vooo = DeviceArray((v, o, o, o))
-        self.W_vvvo = DeviceArray((v, v, v, o))
-        self.W_ovvv = DeviceArray((o, v, v, v))   # W_vvvo[a,b,d,j] stored as [j,a,b,d]
-        self.W_oovo = DeviceArray((o, o, v, o))   # W_vooo[a,l,i,j] stored as [i,j,a,l]
-        self.W_vvvv = DeviceArray((v, v, v, v))
-        self.WR1 = DeviceArray((o, v, o, v))
-        self.WR3a = DeviceArray((o, v, o, v))
-        self.WR3b = DeviceArray((o, v, o, v))
-        self.pfs = DeviceArray((6, n3, n3))
+        self.t3 = DeviceArray((n3, v, v, v), dt)
+        self.R = DeviceArray((n3, v, v, v), dt)
+        self.Y = DeviceArray((self.npairs, o, v, v, v), dt)
+        self.t2 = DeviceArray((o, o, v, v), dt)
+        self.F_oo = DeviceArray((o, o), dt)
+        self.F_vv = DeviceArray((v, v), dt)
+        self.W_oooo = DeviceArray((o, o, o, o), dt)
+        self.W_ovvo = DeviceArray((o, v, v, o), dt)
+        self.W_ovov = DeviceArray((o, v, o, v), dt)
+        self.W_vooo = DeviceArray((v, o, o, o), dt)
+        self.W_vvvo = DeviceArray((v, v, v, o), dt)
+        self.W_ovvv = DeviceArray((o, v, v, v), dt)   # W_vvvo[a,b,d,j] stored as [j,a,b,d]
+        self.W_oovo = DeviceArray((o, o, v, o), dt)   # W_vooo[a,l,i,j] stored as [i,j,a,l]
+        self.W_vvvv = DeviceArray((v, v, v, v), dt)
+        self.WR1 = DeviceArray((o, v, o, v), dt)
+        self.WR3a = DeviceArray((o, v, o, v), dt)
+        self.WR3b = DeviceArray((o, v, o, v), dt)
+        self.pfs = DeviceArray((6, n3, n3), dt)
         self.eia = DeviceArray((o, v))
         self.norb = int(L._lib.dcc_r3_finalize_tiles(v))
         self.partials = DeviceArray((n3 * self.norb, 2))
@@ -348,14 +360,14 @@ class TriplesEngine:
         o, v = self.nocc, self.nvir
         v3 = v ** 3
         lo = {}
-        lo['Z'] = DeviceArray((o, o, v, v, v))
-        lo['Ecomb'] = DeviceArray((o, o, o, v))      # [l,r,p,a]
-        lo['Evovv'] = DeviceArray((v, o, v, v))      # [c,p,d,a] = E_vovv[c,p,a,d]
-        lo['Eoovv'] = DeviceArray((o, o, v, v))      # [r,p,d,e]
-        lo['Eoovv_t'] = DeviceArray((o, o, v, v))    # [m,p,e,d] = E_oovv[m,p,d,e]
-        lo['Er1'] = DeviceArray((o, o, v, v, 1))     # [j,k,b,c] = E - 1/2 E^(bc)
-        lo['r1'] = DeviceArray((o, v, 1))
-        lo['r2'] = DeviceArray((o, o, v, v))
+        lo['Z'] = DeviceArray((o, o, v, v, v), self.dtype)
+        lo['Ecomb'] = DeviceArray((o, o, o, v), self.dtype)      # [l,r,p,a]
+        lo['Evovv'] = DeviceArray((v, o, v, v), self.dtype)      # [c,p,d,a] = E_vovv[c,p,a,d]
+        lo['Eoovv'] = DeviceArray((o, o, v, v), self.dtype)      # [r,p,d,e]
+        lo['Eoovv_t'] = DeviceArray((o, o, v, v), self.dtype)    # [m,p,e,d] = E_oovv[m,p,d,e]
+        lo['Er1'] = DeviceArray((o, o, v, v, 1), self.dtype)     # [j,k,b,c] = E - 1/2 E^(bc)
+        lo['r1'] = DeviceArray((o, v, 1), self.dtype)
+        lo['r2'] = DeviceArray((o, o, v, v), self.dtype)
         ext = {x: v for x in 'abcdexyz'}
         ext.update({'p': o, 'r': o, 'l': o, 'm': o, 'i': o, 'k': o, 'Q': o, 'X': 1})
         Z = lo['Z']
@@ -406,17 +418,18 @@ class TriplesEngine:
         comb = -np.asarray(E_ooov, dtype=np.float64).transpose(2, 0, 1, 3).copy()  # [l,r,p,a]
         for l in range(o):
             comb[l, l] += 0.5 * f
-        lo['Ecomb'].upload(np.ascontiguousarray(comb), self.stream)
-        lo['Evovv'].upload(np.ascontiguousarray(np.asarray(E_vovv, dtype=np.float64).transpose(0, 1, 3, 2)),
-                           self.stream)
-        lo['Eoovv'].upload(np.ascontiguousarray(Eo), self.stream)
-        lo['Eoovv_t'].upload(np.ascontiguousarray(Eo.transpose(0, 1, 3, 2)), self.stream)
-        lo['Er1'].upload(np.ascontiguousarray((Eo - 0.5 * Eo.transpose(0, 1, 3, 2))[..., None]), self.stream)
+        dt = self.dtype
+        lo['Ecomb'].upload(np.ascontiguousarray(comb, dtype=dt), self.stream)
+        lo['Evovv'].upload(np.ascontiguousarray(np.asarray(E_vovv).transpose(0, 1, 3, 2), dtype=dt), self.stream)
+        lo['Eoovv'].upload(np.ascontiguousarray(Eo, dtype=dt), self.stream)
+        lo['Eoovv_t'].upload(np.ascontiguousarray(Eo.transpose(0, 1, 3, 2), dtype=dt), self.stream)
+        lo['Er1'].upload(np.ascontiguousarray((Eo - 0.5 * Eo.transpose(0, 1, 3, 2))[..., None], dtype=dt),
+                         self.stream)
 
     def _build_z(self, q):
         jobs, njobs = self._lower['slices'][q][:2]
         s = self.stream.handle if self.stream is not None else None
-        L.call("dcc_s3_transform_f64", self._lower['Z'].vp, self.t3.vp, self.nvir, 0.0, jobs.vp, njobs, s)
+        L.call("dcc_s3_transform_" + self.sfx, self._lower['Z'].vp, self.t3.vp, self.nvir, 0.0, jobs.vp, njobs, s)
 
     def t3_to_r1r2(self):
         """T3 contributions to r1 and the un-symmetrized r2 (rccsdt.py:253-321); returns host arrays."""
@@ -427,8 +440,8 @@ class TriplesEngine:
             self._build_z(q)
             for g in lo['slices'][q][2]:
                 g.run(self.stream)
-        return (lo['r1'].download(stream=self.stream).reshape(self.nocc, self.nvir),
-                lo['r2'].download(stream=self.stream))
+        return (lo['r1'].download(stream=self.stream).reshape(self.nocc, self.nvir).astype(np.float64),
+                lo['r2'].download(stream=self.stream).astype(np.float64))
 
     def t3_to_w(self):
         """Add the T3 terms of rccsdt.py:323-366 to the device W_vooo / W_vvvo (after set_intermediates)."""
@@ -440,34 +453,37 @@ class TriplesEngine:
 
     def get_w_t3_adjusted(self):
         """Device W_vooo, W_vvvo (incl. T3 terms) back in the reference layouts."""
-        w_oovo = self.W_oovo.download(stream=self.stream)
-        w_ovvv = self.W_ovvv.download(stream=self.stream)
+        w_oovo = self.W_oovo.download(stream=self.stream).astype(np.float64)
+        w_ovvv = self.W_ovvv.download(stream=self.stream).astype(np.float64)
         return w_oovo.transpose(2, 3, 0, 1), w_ovvv.transpose(1, 2, 3, 0)
 
     # --------------------------------------------------------------- inputs
     def set_t3(self, t3_local):
-        self.t3.upload(np.ascontiguousarray(t3_local, dtype=np.float64), self.stream)
+        self.t3.upload(np.ascontiguousarray(t3_local, dtype=self.dtype), self.stream)
 
     def get_t3(self):
-        return self.t3.download(stream=self.stream)
+        return self.t3.download(stream=self.stream).astype(np.float64)
 
     def set_intermediates(self, imds, t2):
         o, v = self.nocc, self.nvir
         for name in ('F_oo', 'F_vv', 'W_oooo', 'W_ovvo', 'W_ovov', 'W_vooo', 'W_vvvo', 'W_vvvv'):
-            getattr(self, name).upload(np.ascontiguousarray(imds[name], dtype=np.float64), self.stream)
-        self.t2.upload(np.ascontiguousarray(t2, dtype=np.float64), self.stream)
-        self.W_ovvv.upload(np.ascontiguousarray(np.asarray(imds['W_vvvo']).transpose(3, 0, 1, 2)), self.stream)
-        self.W_oovo.upload(np.ascontiguousarray(np.asarray(imds['W_vooo']).transpose(2, 3, 0, 1)), self.stream)
+            getattr(self, name).upload(np.ascontiguousarray(imds[name], dtype=self.dtype), self.stream)
+        self.t2.upload(np.ascontiguousarray(t2, dtype=self.dtype), self.stream)
+        self.W_ovvv.upload(np.ascontiguousarray(np.asarray(imds['W_vvvo']).transpose(3, 0, 1, 2), dtype=self.dtype),
+                           self.stream)
+        self.W_oovo.upload(np.ascontiguousarray(np.asarray(imds['W_vooo']).transpose(2, 3, 0, 1), dtype=self.dtype),
+                           self.stream)
         s = self.stream.handle if self.stream is not None else None
-        L.call("dcc_ring_weights_f64", self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
+        L.call("dcc_ring_weights_" + self.sfx, self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
                self.WR3b.vp, s)
-        L.call("dcc_oooo_prefactors_f64", self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
+        L.call("dcc_oooo_prefactors_" + self.sfx, self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
                self.pfs.vp, s)
 
     # -------------------------------------------------------------- compute
     def materialize_y(self):
         s = self.stream.handle if self.stream is not None else None
-        L.call("dcc_s3_transform_f64", self.Y.vp, self.t3.vp, self.nvir, 0.0, self.y_jobs.vp, self.n_y_jobs, s)
+        L.call("dcc_s3_transform_" + self.sfx, self.Y.vp, self.t3.vp, self.nvir, 0.0, self.y_jobs.vp,
+               self.n_y_jobs, s)
 
     def residual(self, timer=None):
         """R <- r3 of distr_cc/rccsdt.py:368-433 for the current t3 and intermediates.
@@ -492,7 +508,7 @@ class TriplesEngine:
         self.eia.upload(np.ascontiguousarray(eia), self.stream)
         nd = v if nvir_diis is None else int(nvir_diis)
         s = self.stream.handle if self.stream is not None else None
-        L.call("dcc_r3_finalize_f64", self.R.vp, self.t3.vp, r3_out.vp if r3_out is not None else None,
+        L.call("dcc_r3_finalize_" + self.sfx, self.R.vp, self.t3.vp, r3_out.vp if r3_out is not None else None,
                self.n3, v, self.blk_ijk.vp, self.eia.vp, o, nd, self.partials.vp, s)
         L.call("dcc_reduce_pairs_f64", self.partials.vp, self.n3 * self.norb, self.norms.vp, s)
         n = self.norms.download(stream=self.stream)

@@ -48,6 +48,7 @@ int dcc_memcpy_dtoh(void *dst, const void *src, int64_t bytes, dcc_stream_t stre
 int dcc_memcpy_dtod(void *dst, const void *src, int64_t bytes, dcc_stream_t stream);
 /* Fill with a double value using a kernel (safe inside captured graphs). */
 int dcc_fill_f64(double *dst, int64_t n, double value, dcc_stream_t stream);
+int dcc_fill_f32(float *dst, int64_t n, float value, dcc_stream_t stream);
 int dcc_stream_create(dcc_stream_t *stream);
 int dcc_stream_destroy(dcc_stream_t stream);
 int dcc_stream_synchronize(dcc_stream_t stream);
@@ -95,6 +96,18 @@ int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, double beta,
                      const dcc_batch *batches, int64_t nbatch, int64_t max_rows,
                      dcc_stream_t stream);
 
+/*
+ * Same contraction with FP32 operands on FP32 MFMA (v_mfma_f32_16x16x4f32,
+ * FP32 accumulation); alpha/beta and the C update are applied in FP64 and
+ * rounded to FP32 once.  The mixed-precision form of dcc_contract_f64.
+ */
+int dcc_contract_f32(int64_t M, int64_t N, int64_t K, double alpha, double beta,
+                     const float *A, dcc_idxmap am, dcc_idxmap ak,
+                     const float *B, dcc_idxmap bk, dcc_idxmap bn,
+                     float *C, dcc_idxmap cm, dcc_idxmap cn,
+                     const dcc_batch *batches, int64_t nbatch, int64_t max_rows,
+                     dcc_stream_t stream);
+
 /* ---------------------------------------- S3 block transforms (HBM-bound) */
 
 /*
@@ -120,6 +133,9 @@ typedef struct dcc_s3_job {
  */
 int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double beta,
                          const dcc_s3_job *jobs, int64_t njobs, dcc_stream_t stream);
+/* FP32 storage, FP64 arithmetic. */
+int dcc_s3_transform_f32(float *out, const float *src, int64_t nvir, double beta,
+                         const dcc_s3_job *jobs, int64_t njobs, dcc_stream_t stream);
 
 /*
  * Fused end of the triples update (distr_cc/rccsdt.py:947-971):
@@ -134,6 +150,10 @@ int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double be
  */
 int64_t dcc_r3_finalize_tiles(int64_t nvir);
 int dcc_r3_finalize_f64(const double *R, double *t3, double *r3_out, int64_t nblk, int64_t nvir,
+                        const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
+                        double *partials, dcc_stream_t stream);
+/* FP32 storage of R and t3; projection, division, update and norms in FP64. */
+int dcc_r3_finalize_f32(const float *R, float *t3, float *r3_out, int64_t nblk, int64_t nvir,
                         const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
                         double *partials, dcc_stream_t stream);
 
@@ -151,6 +171,8 @@ int dcc_reduce_pairs_f64(const double *x, int64_t n, double *out, dcc_stream_t s
  */
 int dcc_oooo_prefactors_f64(const double *W_oooo, const double *F_oo, int64_t nocc, const int32_t *blk_ijk,
                             int64_t nblk, double *pfs, dcc_stream_t stream);
+int dcc_oooo_prefactors_f32(const float *W_oooo, const float *F_oo, int64_t nocc, const int32_t *blk_ijk,
+                            int64_t nblk, float *pfs, dcc_stream_t stream);
 
 /*
  * Ring-term weight matrices ((nocc*nvir) x (nocc*nvir), rows (r,x), cols (l,d)):
@@ -160,6 +182,8 @@ int dcc_oooo_prefactors_f64(const double *W_oooo, const double *F_oo, int64_t no
  */
 int dcc_ring_weights_f64(const double *W_ovov, const double *W_ovvo, int64_t nocc, int64_t nvir, double *WR1,
                          double *WR3a, double *WR3b, dcc_stream_t stream);
+int dcc_ring_weights_f32(const float *W_ovov, const float *W_ovvo, int64_t nocc, int64_t nvir, float *WR1,
+                         float *WR3a, float *WR3b, dcc_stream_t stream);
 
 /* ------------------------------------------------------ (Q)/[Q] energy */
 

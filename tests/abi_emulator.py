@@ -102,16 +102,26 @@ class Emulator:
         self.dev.view(_ptr(dst), np.float64, int(n))[:] = value
         return 0
 
+    def dcc_fill_f32(self, dst, n, value, stream):
+        self.dev.view(_ptr(dst), np.float32, int(n))[:] = value
+        return 0
+
     def dcc_stream_synchronize(self, s):
         return 0
 
     def dcc_device_synchronize(self):
         return 0
 
-    def dcc_contract_f64(self, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
-                         stream):
+    def dcc_contract_f64(self, *a):
+        return self._contract(np.float64, *a)
+
+    def dcc_contract_f32(self, *a):
+        return self._contract(np.float32, *a)
+
+    def _contract(self, dt, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
+                  stream):
         dev = self.dev
-        Av, Bv, Cv = dev.view(_ptr(A), np.float64), dev.view(_ptr(B), np.float64), dev.view(_ptr(C), np.float64)
+        Av, Bv, Cv = dev.view(_ptr(A), dt), dev.view(_ptr(B), dt), dev.view(_ptr(C), dt)
         bt = dev.view(_ptr(batches), np.int64, 5 * int(nbatch)).reshape(-1, 5)
         ks = np.arange(K)
         ns = np.arange(N)
@@ -123,37 +133,49 @@ class Emulator:
             ai = a_off + _map(dev, am, ms, b)[:, None] + _map(dev, ak, ks, b)[None, :]
             bi = b_off + _map(dev, bk, ks, b)[:, None] + _map(dev, bn, ns, b)[None, :]
             ci = c_off + _map(dev, cm, ms, b)[:, None] + _map(dev, cn, ns, b)[None, :]
-            prod = Av[ai] @ Bv[bi] if K > 0 else np.zeros(ci.shape)
-            old = Cv[ci] if beta != 0.0 else 0.0
-            Cv[ci] = alpha * prod + beta * old
+            prod = (Av[ai].astype(np.float64) @ Bv[bi].astype(np.float64)) if K > 0 else np.zeros(ci.shape)
+            old = Cv[ci].astype(np.float64) if beta != 0.0 else 0.0
+            Cv[ci] = (alpha * prod + beta * old).astype(dt)
         return 0
 
-    def dcc_s3_transform_f64(self, out, src, nvir, beta, jobs, njobs, stream):
+    def dcc_s3_transform_f64(self, *a):
+        return self._s3(np.float64, *a)
+
+    def dcc_s3_transform_f32(self, *a):
+        return self._s3(np.float32, *a)
+
+    def _s3(self, dt, out, src, nvir, beta, jobs, njobs, stream):
         v = int(nvir)
-        O = self.dev.view(_ptr(out), np.float64)
-        S = self.dev.view(_ptr(src), np.float64)
+        O = self.dev.view(_ptr(out), dt)
+        S = self.dev.view(_ptr(src), dt)
         raw = self.dev.view(_ptr(jobs), np.uint8, int(njobs) * 64)
         jb = raw.view(np.dtype([("dst", np.int64), ("src", np.int64), ("coef", np.float64, (6,))]))
         for j in jb:
-            x = S[int(j["src"]): int(j["src"]) + v ** 3].reshape(v, v, v)
+            x = S[int(j["src"]): int(j["src"]) + v ** 3].reshape(v, v, v).astype(np.float64)
             val = sum(c * x.transpose(p) for c, p in zip(j["coef"], _PERMS) if c != 0.0)
             o = O[int(j["dst"]): int(j["dst"]) + v ** 3].reshape(v, v, v)
-            o[...] = (beta * o if beta != 0.0 else 0.0) + val
+            o[...] = ((beta * o.astype(np.float64) if beta != 0.0 else 0.0) + val).astype(dt)
         return 0
 
-    def dcc_r3_finalize_f64(self, R, t3, r3_out, nblk, nvir, blk_ijk, eia, nocc, nvir_diis, partials, stream):
+    def dcc_r3_finalize_f64(self, *a):
+        return self._finalize(np.float64, *a)
+
+    def dcc_r3_finalize_f32(self, *a):
+        return self._finalize(np.float32, *a)
+
+    def _finalize(self, dt, R, t3, r3_out, nblk, nvir, blk_ijk, eia, nocc, nvir_diis, partials, stream):
         v, n3, o = int(nvir), int(nblk), int(nocc)
-        Rv = self.dev.view(_ptr(R), np.float64, n3 * v ** 3).reshape(n3, v, v, v)
-        Tv = self.dev.view(_ptr(t3), np.float64, n3 * v ** 3).reshape(n3, v, v, v)
+        Rv = self.dev.view(_ptr(R), dt, n3 * v ** 3).reshape(n3, v, v, v).astype(np.float64)
+        Tv = self.dev.view(_ptr(t3), dt, n3 * v ** 3).reshape(n3, v, v, v)
         ijk = self.dev.view(_ptr(blk_ijk), np.int32, 3 * n3).reshape(n3, 3)
         e = self.dev.view(_ptr(eia), np.float64, o * v).reshape(o, v)
         sym = sum(Rv.transpose(0, *(1 + np.array(p))) for p in _PERMS)
         r = Rv - sym / 6.0
         d = e[ijk[:, 0], :, None, None] + e[ijk[:, 1], None, :, None] + e[ijk[:, 2], None, None, :]
         r = r / d
-        Tv += r
+        Tv[...] = (Tv.astype(np.float64) + r).astype(dt)
         if _ptr(r3_out):
-            self.dev.view(_ptr(r3_out), np.float64, n3 * v ** 3)[:] = r.ravel()
+            self.dev.view(_ptr(r3_out), dt, n3 * v ** 3)[:] = r.ravel().astype(dt)
         nd = int(nvir_diis)
         self._norms = (float((r * r).sum()), float((r[:, :nd, :nd, :nd] ** 2).sum()))
         P = self.dev.view(_ptr(partials), np.float64, 2)
@@ -172,13 +194,19 @@ class Emulator:
         nt = (int(nvir) + 7) // 8
         return nt * (nt + 1) * (nt + 2) // 6
 
-    def dcc_oooo_prefactors_f64(self, W, F, nocc, blk_ijk, nblk, pfs, stream):
+    def dcc_oooo_prefactors_f64(self, *a):
+        return self._pfs(np.float64, *a)
+
+    def dcc_oooo_prefactors_f32(self, *a):
+        return self._pfs(np.float32, *a)
+
+    def _pfs(self, dt, W, F, nocc, blk_ijk, nblk, pfs, stream):
         o, n3 = int(nocc), int(nblk)
-        Wv = self.dev.view(_ptr(W), np.float64, o ** 4).reshape(o, o, o, o)
-        Fv = self.dev.view(_ptr(F), np.float64, o * o).reshape(o, o)
+        Wv = self.dev.view(_ptr(W), dt, o ** 4).reshape(o, o, o, o).astype(np.float64)
+        Fv = self.dev.view(_ptr(F), dt, o * o).reshape(o, o).astype(np.float64)
         ijk = self.dev.view(_ptr(blk_ijk), np.int32, 3 * n3).reshape(n3, 3)
-        P = self.dev.view(_ptr(pfs), np.float64, 6 * n3 * n3).reshape(6, n3, n3)
-        P[...] = 0.0
+        Pd = self.dev.view(_ptr(pfs), dt, 6 * n3 * n3).reshape(6, n3, n3)
+        P = np.zeros((6, n3, n3))
         # dense definition: coefficient of T[p]^s in R[q] collected from the
         # full-tensor W_oooo / F_oo terms by enumerating ordered (l,m) pairs
         lin = {tuple(int(x) for x in t): n for n, t in enumerate(ijk)}
@@ -200,6 +228,7 @@ class Emulator:
                 add((0, 1, 2), (l, j, k), -Fv[l, i], q)
                 add((0, 1, 2), (i, l, k), -Fv[l, j], q)
                 add((0, 1, 2), (i, j, l), -Fv[l, k], q)
+        Pd[...] = P.astype(dt)
         return 0
 
     def dcc_t3_to_abc_f64(self, t3, nocc, nvir, vpad, out, stream):
@@ -269,15 +298,21 @@ class Emulator:
         self._npart = 0
         return 0
 
-    def dcc_ring_weights_f64(self, Wovov, Wovvo, nocc, nvir, WR1, WR3a, WR3b, stream):
+    def dcc_ring_weights_f64(self, *a):
+        return self._ring(np.float64, *a)
+
+    def dcc_ring_weights_f32(self, *a):
+        return self._ring(np.float32, *a)
+
+    def _ring(self, dt, Wovov, Wovvo, nocc, nvir, WR1, WR3a, WR3b, stream):
         o, v = int(nocc), int(nvir)
-        A = self.dev.view(_ptr(Wovov), np.float64, (o * v) ** 2).reshape(o, v, o, v)
-        B = self.dev.view(_ptr(Wovvo), np.float64, (o * v) ** 2).reshape(o, v, v, o)
+        A = self.dev.view(_ptr(Wovov), dt, (o * v) ** 2).reshape(o, v, o, v).astype(np.float64)
+        B = self.dev.view(_ptr(Wovvo), dt, (o * v) ** 2).reshape(o, v, v, o).astype(np.float64)
         ovov = A.transpose(2, 1, 0, 3)     # [r,x,l,d] = W_ovov[l,x,r,d]
         ovvo = B.transpose(3, 1, 0, 2)     # [r,x,l,d] = W_ovvo[l,x,d,r]
-        self.dev.view(_ptr(WR1), np.float64, (o * v) ** 2)[:] = (-ovov).ravel()
-        self.dev.view(_ptr(WR3a), np.float64, (o * v) ** 2)[:] = ovvo.ravel()
-        self.dev.view(_ptr(WR3b), np.float64, (o * v) ** 2)[:] = (-0.5 * (ovov + ovvo)).ravel()
+        self.dev.view(_ptr(WR1), dt, (o * v) ** 2)[:] = (-ovov).ravel().astype(dt)
+        self.dev.view(_ptr(WR3a), dt, (o * v) ** 2)[:] = ovvo.ravel().astype(dt)
+        self.dev.view(_ptr(WR3b), dt, (o * v) ** 2)[:] = (-0.5 * (ovov + ovvo)).ravel().astype(dt)
         return 0
 
 

@@ -72,3 +72,22 @@ def test_q_engine_plan(monkeypatch, o, v, bs):
     got = QEngine(o, v, blksize=bs).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy)
     for g, r in zip(got, ref):
         assert abs(g - r) <= 1e-12 * abs(r) + 1e-25
+
+
+@pytest.mark.parametrize("o,v", [(3, 4), (4, 5)])
+def test_engine_fp32_plan(monkeypatch, o, v):
+    """FP32 storage path: same plan, FP32 rounding only (relative 1e-5 of the residual scale)."""
+    abi_emulator.install(monkeypatch)
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    s = SyntheticSystem(o, v, naux=20, seed=40 + o)
+    im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
+    eng = TriplesEngine(o, v, dtype=np.float32)
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.residual()
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3)
+    assert abs(eng.R.download().astype(np.float64) - Ro).max() <= 1e-5 * abs(Ro).max()
+    eng.finalize(s.mo_energy)
+    t3o = tri.t3_update(o, Ro, t3, s.mo_energy)[0]
+    assert abs(eng.get_t3() - t3o).max() <= 1e-5 * abs(t3o).max()

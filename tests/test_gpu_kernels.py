@@ -168,3 +168,45 @@ def test_q_energy_vs_oracle(dev, o, v, bs):
     parts = [QEngine(o, v, blksize=bs).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy, job_idx=j,
                                               n_jobs=3) for j in range(3)]
     assert abs(sum(p[1] for p in parts) - got[1]) <= 1e-12 * abs(got[1])
+
+
+@pytest.mark.parametrize("M,N,K", [(16, 16, 4), (37, 53, 29), (130, 70, 300)])
+def test_contract_f32_plain_gemm(dev, M, N, K):
+    """FP32 MFMA path; pins the v_mfma_f32_16x16x4f32 accumulator layout and masking."""
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    _say("contract f32 %dx%dx%d" % (M, N, K))
+    rng = np.random.default_rng(M * N + K)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    C0 = rng.normal(size=(M, N)).astype(np.float32)
+    dA, dB, dC = DeviceArray.from_host(A), DeviceArray.from_host(B), DeviceArray.from_host(C0)
+    bt = np.zeros(1, dtype=L.BATCH_DTYPE)
+    bt[0] = (0, 0, 0, 0, M)
+    dbt = DeviceArray.from_host(bt)
+    lin = lambda s: L.IdxMap(None, 0, 1, 1 << 20, 0, 0, s)
+    L.call("dcc_contract_f32", M, N, K, 1.5, 0.5, dA.vp, lin(K), lin(1), dB.vp, lin(N), lin(1),
+           dC.vp, lin(N), lin(1), dbt.vp, 1, M, None)
+    ref = 0.5 * C0.astype(np.float64) + 1.5 * (A.astype(np.float64) @ B.astype(np.float64))
+    err = abs(dC.download().astype(np.float64) - ref).max()
+    assert err < 3e-6 * max(1.0, abs(ref).max()) * np.sqrt(K), err
+
+
+@pytest.mark.parametrize("o,v", [(3, 9), (5, 17)])
+def test_residual_fp32_vs_oracle(dev, o, v):
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    from oracle import rccsdt_tri as tri
+    _say("residual fp32 o=%d v=%d" % (o, v))
+    s = SyntheticSystem(o, v, naux=30, seed=700 + o * v)
+    im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
+    eng = TriplesEngine(o, v, dtype=np.float32)
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.residual()
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3)
+    assert abs(eng.R.download().astype(np.float64) - Ro).max() <= 2e-5 * abs(Ro).max()
+    n2, _ = eng.finalize(s.mo_energy)
+    t3o, _, n2o, _ = tri.t3_update(o, Ro, t3, s.mo_energy)
+    assert abs(eng.get_t3() - t3o).max() <= 2e-5 * abs(t3o).max()
+    assert abs(n2 - n2o) <= 1e-4 * n2o
