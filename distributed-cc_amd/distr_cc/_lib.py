@@ -37,6 +37,11 @@ class S3Job(ctypes.Structure):
     _fields_ = [("dst", c_i64), ("src", c_i64), ("coef", c_dbl * 6)]
 
 
+class Strided(ctypes.Structure):
+    """dcc_strided: up to 6 axes with per-side element strides."""
+    _fields_ = [("ndim", c_i64), ("shape", c_i64 * 6), ("dst_stride", c_i64 * 6), ("src_stride", c_i64 * 6)]
+
+
 BATCH_DTYPE = np.dtype([("a_off", np.int64), ("b_off", np.int64), ("c_off", np.int64),
                         ("m_begin", np.int64), ("m_end", np.int64)])
 S3JOB_DTYPE = np.dtype([("dst", np.int64), ("src", np.int64), ("coef", np.float64, (6,))])
@@ -82,6 +87,12 @@ _SIGS = {
     "dcc_t3_to_abc_f64": ([c_vp, c_i64, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_q_finalize_f64": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
                             c_vp], ctypes.c_int),
+    "dcc_tensor_axpby_f64": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
+    "dcc_tensor_axpby_f32": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
+    "dcc_dot_partials": ([], c_i64),
+    "dcc_dot_f64": ([c_vp, c_vp, c_i64, c_vp, c_vp, c_vp], ctypes.c_int),
+    "dcc_dot_f32": ([c_vp, c_vp, c_i64, c_vp, c_vp, c_vp], ctypes.c_int),
+    "dcc_denom_divide_f64": ([c_vp, c_i64, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
 }
 
 for _name, (_args, _res) in _SIGS.items():

@@ -53,14 +53,24 @@ def c_strides(shape):
 class Operand:
     """A device tensor with one letter per axis (letters not in a term are fixed per batch)."""
 
-    def __init__(self, buf, letters, shape):
+    def __init__(self, buf, letters, shape, base=0):
         assert len(letters) == len(shape)
         self.buf, self.letters, self.shape = buf, letters, tuple(int(s) for s in shape)
         self.strides = dict(zip(letters, c_strides(self.shape)))
         self.ext = dict(zip(letters, self.shape))
+        self.base = int(base)
+
+    @classmethod
+    def strided(cls, buf, strides, base=0):
+        """A view with explicit per-letter element strides and a base offset."""
+        op = cls.__new__(cls)
+        op.buf, op.letters, op.shape, op.ext = buf, ''.join(strides), None, {}
+        op.strides = {k: int(v) for k, v in strides.items()}
+        op.base = int(base)
+        return op
 
     def offset(self, fixed):
-        return sum(self.strides[k] * int(v) for k, v in fixed.items() if k in self.strides)
+        return self.base + sum(self.strides[k] * int(v) for k, v in fixed.items() if k in self.strides)
 
 
 def idxmap(letters, ext, strides):
@@ -179,6 +189,23 @@ def plan_einsum(name, sub, A, B, C, alpha, beta, ext, batches, max_levels=3):
     bt = [(A.offset(fa), B.offset(fb), C.offset(fc), 0, M) for fa, fb, fc in entries]
     return GemmLaunch(name, M, N, K, alpha, beta, A.buf, am, ak, B.buf, bk, bn, C.buf, cm, cn, bt, M,
                       2.0 * M * N * K * len(bt))
+
+
+def tensor_axpby(dst, src, letters, shape, alpha, beta, stream):
+    """dst (+)= alpha * src over the index space ``letters`` x ``shape`` (Operand views;
+    a letter missing from ``src`` broadcasts with stride 0)."""
+    if dst.buf.dtype != src.buf.dtype:
+        raise TypeError("tensor_axpby: dtypes differ")
+    lay = L.Strided()
+    lay.ndim = len(letters)
+    for d, (x, e) in enumerate(zip(letters, shape)):
+        lay.shape[d] = int(e)
+        lay.dst_stride[d] = dst.strides[x]
+        lay.src_stride[d] = src.strides.get(x, 0)
+    item = dst.buf.dtype.itemsize
+    L.call("dcc_tensor_axpby_" + _suffix(dst.buf.dtype), ctypes.c_void_p(dst.buf.ptr + dst.base * item),
+           ctypes.c_void_p(src.buf.ptr + src.base * item), ctypes.byref(lay), float(alpha), float(beta),
+           stream.handle if stream is not None else None)
 
 
 def plan_term(name, sub, A, B, C, alpha, beta, batches_fixed, ext):
@@ -426,13 +453,57 @@ class TriplesEngine:
         lo['Er1'].upload(np.ascontiguousarray((Eo - 0.5 * Eo.transpose(0, 1, 3, 2))[..., None], dtype=dt),
                          self.stream)
 
+    def set_t1_dressed_device(self, G, Fd, nmo):
+        """Build the T3 -> r1/r2/W operands from device copies of the T1-dressed
+        integrals G = t1_eris (nmo^4) and Fock Fd = t1_fock (nmo^2), both FP64."""
+        if self._lower is None:
+            self._setup_lower()
+        if self.dtype != np.float64:
+            raise TypeError("device dressing needs the FP64 triples engine")
+        lo, o, v, n = self._lower, self.nocc, self.nvir, int(nmo)
+        g_str = {'P': n ** 3, 'Q': n ** 2, 'R': n, 'S': 1}
+        fv = o  # first virtual orbital
+
+        def gv(letters, starts):
+            st = dict(zip(letters, (g_str[x] for x in 'PQRS')))
+            return Operand.strided(G, st, sum(st[x] * starts.get(x, 0) for x in letters))
+
+        def run(dst, src, letters, shape, alpha, beta):
+            tensor_axpby(dst, src, letters, shape, alpha, beta, self.stream)
+
+        Ecomb = Operand(lo['Ecomb'], 'lrpa', (o, o, o, v))
+        run(Ecomb, gv('rpla', {'a': fv}), 'lrpa', (o, o, o, v), -1.0, 0.0)
+        diag = Operand.strided(lo['Ecomb'], {'l': o * o * v + o * v, 'p': v, 'a': 1})
+        run(diag, Operand.strided(Fd, {'l': 0, 'p': n, 'a': 1}, fv), 'lpa', (o, o, v), 0.5, 1.0)
+        run(Operand(lo['Evovv'], 'cpda', (v, o, v, v)), gv('cpad', {'c': fv, 'a': fv, 'd': fv}), 'cpda',
+            (v, o, v, v), 1.0, 0.0)
+        oovv = gv('rpde', {'d': fv, 'e': fv})
+        run(Operand(lo['Eoovv'], 'rpde', (o, o, v, v)), oovv, 'rpde', (o, o, v, v), 1.0, 0.0)
+        run(Operand(lo['Eoovv_t'], 'rped', (o, o, v, v)), oovv, 'rpde', (o, o, v, v), 1.0, 0.0)
+        Er1 = Operand(lo['Er1'], 'rpdeX', (o, o, v, v, 1))
+        run(Er1, oovv, 'rpde', (o, o, v, v), 1.0, 0.0)
+        run(Er1, Operand.strided(G, {'r': n ** 3, 'p': n ** 2, 'e': n, 'd': 1}, (fv * n + fv)), 'rpde',
+            (o, o, v, v), -0.5, 1.0)
+
+    def prepare_residual(self):
+        """Ring weights and W_oooo/F_oo prefactors from the device intermediates."""
+        o, v = self.nocc, self.nvir
+        s = self.stream.handle if self.stream is not None else None
+        L.call("dcc_ring_weights_" + self.sfx, self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
+               self.WR3b.vp, s)
+        L.call("dcc_oooo_prefactors_" + self.sfx, self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
+               self.pfs.vp, s)
+
     def _build_z(self, q):
         jobs, njobs = self._lower['slices'][q][:2]
         s = self.stream.handle if self.stream is not None else None
         L.call("dcc_s3_transform_" + self.sfx, self._lower['Z'].vp, self.t3.vp, self.nvir, 0.0, jobs.vp, njobs, s)
 
-    def t3_to_r1r2(self):
-        """T3 contributions to r1 and the un-symmetrized r2 (rccsdt.py:253-321); returns host arrays."""
+    def t3_to_r1r2(self, download=True):
+        """T3 contributions to r1 and the un-symmetrized r2 (rccsdt.py:253-321).
+
+        Returns host arrays, or with ``download=False`` the device buffers
+        r1 (o, v, 1) and r2 (o, o, v, v)."""
         lo = self._lower
         lo['r1'].fill(0.0, self.stream)
         lo['r2'].fill(0.0, self.stream)
@@ -440,6 +511,8 @@ class TriplesEngine:
             self._build_z(q)
             for g in lo['slices'][q][2]:
                 g.run(self.stream)
+        if not download:
+            return lo['r1'], lo['r2']
         return (lo['r1'].download(stream=self.stream).reshape(self.nocc, self.nvir).astype(np.float64),
                 lo['r2'].download(stream=self.stream).astype(np.float64))
 
@@ -473,11 +546,7 @@ class TriplesEngine:
                            self.stream)
         self.W_oovo.upload(np.ascontiguousarray(np.asarray(imds['W_vooo']).transpose(2, 3, 0, 1), dtype=self.dtype),
                            self.stream)
-        s = self.stream.handle if self.stream is not None else None
-        L.call("dcc_ring_weights_" + self.sfx, self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
-               self.WR3b.vp, s)
-        L.call("dcc_oooo_prefactors_" + self.sfx, self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
-               self.pfs.vp, s)
+        self.prepare_residual()
 
     # -------------------------------------------------------------- compute
     def materialize_y(self):

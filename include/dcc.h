@@ -209,6 +209,46 @@ int dcc_q_finalize_f64(const double *t4, const double *z4, const int32_t *tasks,
                        int64_t nocc, int64_t nvir, int64_t vpad, const double *e_occ, const double *e_vir,
                        const double *vvoo, const double *t2, double *partials, dcc_stream_t stream);
 
+/* ------------------------------------- lower-order iteration (around T3) */
+
+/*
+ * Strided index space of up to 6 axes; strides are in elements.  Used for the
+ * block views, transposes and packing of the singles/doubles update, which
+ * the reference writes as NumPy slicing and .transpose (distr_cc/rccsdt.py:
+ * 909-933, the intermediates of :935 and the DIIS packing of
+ * distr_cc/_runtime.py:151-186).
+ */
+typedef struct dcc_strided {
+    int64_t ndim;
+    int64_t shape[6];
+    int64_t dst_stride[6];
+    int64_t src_stride[6];
+} dcc_strided;
+
+/* dst[i] = beta * dst[i] + alpha * src[i] over the index space (dst is not read when beta == 0). */
+int dcc_tensor_axpby_f64(double *dst, const double *src, const dcc_strided *layout, double alpha, double beta,
+                         dcc_stream_t stream);
+int dcc_tensor_axpby_f32(float *dst, const float *src, const dcc_strided *layout, double alpha, double beta,
+                         dcc_stream_t stream);
+
+/* Number of doubles the `partials` scratch of dcc_dot_* must hold. */
+int64_t dcc_dot_partials(void);
+
+/*
+ * out[0] = sum_i x[i] y[i] in FP64, summed in a fixed order (deterministic).
+ * Replaces the np.linalg.norm / np.dot calls of distr_cc/rccsdt.py:929,957-963
+ * and distr_cc/diis.py:122-128.
+ */
+int dcc_dot_f64(const double *x, const double *y, int64_t n, double *partials, double *out, dcc_stream_t stream);
+int dcc_dot_f32(const float *x, const float *y, int64_t n, double *partials, double *out, dcc_stream_t stream);
+
+/*
+ * rank 1: x[i,a] /= eia[i,a]; rank 2: x[i,j,a,b] /= eia[i,a] + eia[j,b], with
+ * eia = e_i - e_a - level_shift (the r1r2_divide_e_ call of distr_cc/rccsdt.py:924).
+ */
+int dcc_denom_divide_f64(double *x, int64_t nocc, int64_t nvir, int64_t rank, const double *eia,
+                         dcc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

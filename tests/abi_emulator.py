@@ -138,6 +138,59 @@ class Emulator:
             Cv[ci] = (alpha * prod + beta * old).astype(dt)
         return 0
 
+    def dcc_tensor_axpby_f64(self, *a):
+        return self._axpby(np.float64, *a)
+
+    def dcc_tensor_axpby_f32(self, *a):
+        return self._axpby(np.float32, *a)
+
+    def _axpby(self, dt, dst, src, layout, alpha, beta, stream):
+        lay = layout._obj if hasattr(layout, "_obj") else layout.contents
+        nd = int(lay.ndim)
+        shape = [int(lay.shape[d]) for d in range(nd)]
+        if int(np.prod(shape)) == 0:
+            return 0
+        idx = np.indices(shape).reshape(nd, -1)
+        od = sum(idx[d] * int(lay.dst_stride[d]) for d in range(nd))
+        os_ = sum(idx[d] * int(lay.src_stride[d]) for d in range(nd))
+        D, S = self.dev.view(_ptr(dst), dt), self.dev.view(_ptr(src), dt)
+        val = alpha * S[os_].astype(np.float64)
+        if beta != 0.0:
+            val = val + beta * D[od].astype(np.float64)
+        D[od] = val.astype(dt)
+        return 0
+
+    def dcc_dot_partials(self):
+        return 1024
+
+    def dcc_dot_f64(self, *a):
+        return self._dot(np.float64, *a)
+
+    def dcc_dot_f32(self, *a):
+        return self._dot(np.float32, *a)
+
+    def _dot(self, dt, x, y, n, partials, out, stream):
+        n = int(n)
+        o = self.dev.view(_ptr(out), np.float64, 1)
+        if n == 0:
+            o[0] = 0.0
+            return 0
+        X = self.dev.view(_ptr(x), dt, n).astype(np.float64)
+        Y = self.dev.view(_ptr(y), dt, n).astype(np.float64)
+        o[0] = float(X @ Y)
+        return 0
+
+    def dcc_denom_divide_f64(self, x, nocc, nvir, rank, eia, stream):
+        o, v = int(nocc), int(nvir)
+        E = self.dev.view(_ptr(eia), np.float64, o * v).reshape(o, v)
+        if rank == 1:
+            X = self.dev.view(_ptr(x), np.float64, o * v).reshape(o, v)
+            X /= E
+        else:
+            X = self.dev.view(_ptr(x), np.float64, o * o * v * v).reshape(o, o, v, v)
+            X /= E[:, None, :, None] + E[None, :, None, :]
+        return 0
+
     def dcc_s3_transform_f64(self, *a):
         return self._s3(np.float64, *a)
 

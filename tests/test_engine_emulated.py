@@ -91,3 +91,79 @@ def test_engine_fp32_plan(monkeypatch, o, v):
     eng.finalize(s.mo_energy)
     t3o = tri.t3_update(o, Ro, t3, s.mo_energy)[0]
     assert abs(eng.get_t3() - t3o).max() <= 1e-5 * abs(t3o).max()
+
+
+@pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (3, 4, 2), (1, 4, 3)])
+def test_full_iteration_plan(monkeypatch, o, v, seed):
+    """RCCSDTEngine.update_amps (dressing, lower order, T3, update) == oracle update_amps."""
+    abi_emulator.install(monkeypatch)
+    from ccsdt_cases import molecule_like, random_amplitudes
+    from distr_cc.rccsdt_engine import RCCSDTEngine
+    from oracle import rccsdt_lower as lo
+    F, g, e, _ = molecule_like(o, v, seed)
+    t1, t2, t3 = random_amplitudes(o, v, seed + 100)
+    eng = RCCSDTEngine(o, v, F, g, e, level_shift=0.05, nvir_diis=max(1, v - 1))
+    eng.set_amplitudes(t1, t2, t3)
+    assert abs(eng.energy() - tri.energy_rhf(o, F, g, t1, t2)) < 1e-14
+    res = eng.update_amps()
+    a1, a2, a3 = eng.get_amplitudes()
+    b1, b2, b3, rres = lo.update_amps(o, F, g, e, t1, t2, t3, level_shift=0.05, nvir_diis=max(1, v - 1))
+    assert np.allclose(res, rres, rtol=1e-12, atol=1e-15)
+    for a, b in ((a1, b1), (a2, b2), (a3, b3)):
+        assert abs(a - b).max() <= 1e-12 * max(abs(b).max(), 1e-30)
+
+
+def test_diis_packing_round_trip_and_steps(monkeypatch):
+    abi_emulator.install(monkeypatch)
+    from ccsdt_cases import molecule_like, random_amplitudes
+    from distr_cc.device import DeviceArray
+    from distr_cc.diis import DeviceDIIS
+    from distr_cc.rccsdt_engine import RCCSDTEngine
+    from oracle.diis import DIIS
+    o, v = 3, 4
+    F, g, e, _ = molecule_like(o, v, 7)
+    eng = RCCSDTEngine(o, v, F, g, e, nvir_diis=3)
+    t1, t2, t3 = random_amplitudes(o, v, 8)
+    eng.set_amplitudes(t1, t2, t3)
+    x = DeviceArray((eng.diis_size(),))
+    eng.pack(x)
+    host = x.download()
+    pairs = [(i, j) for i in range(o) for j in range(i, o)]
+    ref = np.concatenate([t1.ravel(), np.concatenate([t2[i, j].ravel() for i, j in pairs]),
+                          t3[:, :3, :3, :3].ravel()])
+    assert np.array_equal(host, ref)
+    eng.set_amplitudes(0 * t1, 0 * t2, t3)
+    eng.unpack(x)
+    a1, a2, a3 = eng.get_amplitudes()
+    assert np.array_equal(a1, t1) and np.array_equal(a2, t2) and np.array_equal(a3, t3)
+    rng = np.random.default_rng(9)
+    dd, od = DeviceDIIS(ref.size, space=3), DIIS(space=3)
+    vec = ref.copy()
+    for step in range(6):
+        vec = vec + 0.5 ** step * rng.standard_normal(vec.size)
+        x.upload(vec)
+        got = dd.update(x).download()
+        want = od.update(vec)
+        assert abs(got - want).max() <= 1e-12 * abs(want).max()
+
+
+def test_kernel_converges_to_oracle_fixed_point(monkeypatch):
+    abi_emulator.install(monkeypatch)
+    from ccsdt_cases import DuckMF, molecule_like
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    from oracle import rccsdt_lower as lo
+    o, v = 2, 3
+    F, g, e, _ = molecule_like(o, v, 22)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.conv_tol, cc.conv_tol_normt, cc.verbose = 1e-12, 1e-10, 0
+    e_cc, tamps = cc.kernel(eris=_PhysicistsERIs(F, e, g))
+    assert cc.converged
+    _, t1, t2 = tri.init_amps_rhf(o, F, e, g)
+    t3 = np.zeros_like(tamps[2][1])
+    for _ in range(200):
+        t1, t2, t3, res = lo.update_amps(o, F, g, e, t1, t2, t3)
+        if max(res) < 1e-12:
+            break
+    assert abs(e_cc - tri.energy_rhf(o, F, g, t1, t2)) < 1e-10
+    assert abs(tamps[1] - t2).max() < 1e-8

@@ -1,0 +1,103 @@
+"""GPU parity of the whole RCCSDT iteration (T1 dressing, lower order, T3, DIIS, loop)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import rccsdt_lower as lo
+from oracle import rccsdt_tri as tri
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    from distr_cc import device
+    arch = device.init(0)
+    _say("device arch %s" % arch)
+    return device
+
+
+@pytest.mark.parametrize("o,v,seed", [(1, 4, 3), (2, 3, 1), (3, 4, 2), (4, 7, 5), (5, 9, 6)])
+def test_full_iteration_vs_oracle(dev, o, v, seed):
+    from ccsdt_cases import molecule_like, random_amplitudes
+    from distr_cc.rccsdt_engine import RCCSDTEngine
+    _say("full iteration o=%d v=%d" % (o, v))
+    F, g, e, _ = molecule_like(o, v, seed)
+    t1, t2, t3 = random_amplitudes(o, v, seed + 100)
+    eng = RCCSDTEngine(o, v, F, g, e, level_shift=0.05, nvir_diis=max(1, v - 1))
+    eng.set_amplitudes(t1, t2, t3)
+    assert abs(eng.energy() - tri.energy_rhf(o, F, g, t1, t2)) < 1e-13
+    res = eng.update_amps()
+    a1, a2, a3 = eng.get_amplitudes()
+    b1, b2, b3, rres = lo.update_amps(o, F, g, e, t1, t2, t3, level_shift=0.05, nvir_diis=max(1, v - 1))
+    assert np.allclose(res, rres, rtol=1e-11, atol=1e-14)
+    for a, b in ((a1, b1), (a2, b2), (a3, b3)):
+        assert abs(a - b).max() <= 1e-11 * max(abs(b).max(), 1e-30)
+
+
+def test_kernel_two_electrons_is_fci(dev):
+    """Two electrons: RCCSDT == FCI (tests/test_rccsdt.py:221-230 uses He)."""
+    from ccsdt_cases import DuckMF, molecule_like
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    from oracle.fci_cc import DetSpace
+    o, v = 1, 5
+    _say("kernel two-electron o=%d v=%d" % (o, v))
+    F, g, e, h = molecule_like(o, v, 31)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.conv_tol, cc.conv_tol_normt, cc.verbose = 1e-12, 1e-10, 0
+    e_cc, _ = cc.kernel(eris=_PhysicistsERIs(F, e, g))
+    assert cc.converged
+    ds = DetSpace(o + v, o)
+    w, U = np.linalg.eigh(ds.hamiltonian(h, g).toarray())
+    root = np.argmax(np.abs(U[ds.ref]))
+    e_hf = 2 * h[0, 0] + g[0, 0, 0, 0]
+    assert abs((w[root] - e_hf) - e_cc) < 1e-9
+
+
+def test_kernel_matches_oracle_fixed_point(dev):
+    from ccsdt_cases import DuckMF, molecule_like
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 3, 6
+    _say("kernel o=%d v=%d" % (o, v))
+    F, g, e, _ = molecule_like(o, v, 23)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.conv_tol, cc.conv_tol_normt, cc.verbose = 1e-12, 1e-10, 0
+    e_cc, tamps = cc.kernel(eris=_PhysicistsERIs(F, e, g))
+    assert cc.converged
+    _, t1, t2 = tri.init_amps_rhf(o, F, e, g)
+    t3 = np.zeros_like(tamps[2][1])
+    for _ in range(300):
+        t1, t2, t3, res = lo.update_amps(o, F, g, e, t1, t2, t3)
+        if max(res) < 1e-12:
+            break
+    assert abs(e_cc - tri.energy_rhf(o, F, g, t1, t2)) < 1e-10
+
+
+def test_device_ao2mo(dev):
+    from distr_cc.rccsdt import _device_ao2mo
+    _say("ao2mo")
+    rng = np.random.default_rng(12)
+    nao, nocc = 9, 3
+    h = rng.standard_normal((nao, nao))
+    h = h + h.T
+    x = rng.standard_normal((nao,) * 4)
+    c = x + x.transpose(1, 0, 2, 3)
+    c = c + c.transpose(0, 1, 3, 2)
+    c = c + c.transpose(2, 3, 0, 1)
+    C = np.linalg.qr(rng.standard_normal((nao, nao)))[0]
+    Fmo, pppp, e_elec = _device_ao2mo(h, c, C[:, 1:], C[:, :nocc])
+    D = 2 * C[:, :nocc] @ C[:, :nocc].T
+    Fao = h + np.einsum("mnls,ls->mn", c, D) - 0.5 * np.einsum("mlns,ls->mn", c, D)
+    ref_F = C[:, 1:].T @ Fao @ C[:, 1:]
+    chem = np.einsum("mnls,mp,nq,lr,st->pqrt", c, C[:, 1:], C[:, 1:], C[:, 1:], C[:, 1:], optimize=True)
+    assert abs(Fmo - ref_F).max() < 1e-11 * abs(ref_F).max()
+    assert abs(pppp - chem.transpose(0, 2, 1, 3)).max() < 1e-11 * abs(chem).max()
+    assert abs(e_elec - 0.5 * np.sum(D * (h + Fao))) < 1e-10
