@@ -101,3 +101,26 @@ def test_device_ao2mo(dev):
     assert abs(Fmo - ref_F).max() < 1e-11 * abs(ref_F).max()
     assert abs(pppp - chem.transpose(0, 2, 1, 3)).max() < 1e-11 * abs(chem).max()
     assert abs(e_elec - 0.5 * np.sum(D * (h + Fao))) < 1e-10
+
+
+def test_rccsdt_then_q(dev):
+    """RCCSDT(mf, comm).kernel() then rccsdt_q.kernel(): (Q) of the converged amplitudes
+    equals the oracle's (Q) on the same amplitudes."""
+    from ccsdt_cases import DuckMF, molecule_like
+    from distr_cc import rccsdt_q
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    from oracle import q_energy
+    from oracle.t3_layout import full_t3_from_tri
+    o, v = 3, 6
+    _say("rccsdt + (Q) o=%d v=%d" % (o, v))
+    F, g, e, _ = molecule_like(o, v, 41)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.verbose = 0
+    eris = _PhysicistsERIs(F, e, g)
+    cc.kernel(eris=eris)
+    assert cc.converged
+    for bs in (2, 4):
+        eb, ep = rccsdt_q.kernel(cc, eris=eris, blksize=bs, comm=SINGLE_RANK)
+        rb, rp = q_energy.kernel(o, v, g, cc.t2, full_t3_from_tri(cc.t3[1], o), e, blksize=bs)
+        assert abs(eb - rb) < 1e-12 and abs(ep - rp) < 1e-12
