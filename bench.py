@@ -71,6 +71,36 @@ def cpu_baseline(o, v, sysm, im, t2, t3, budget_s=25.0):
                       "(%.1f s, %.1f s), extrapolated linearly to all blocks" % (n1, n2, n3, t1, t2s)}
 
 
+def measured_traffic(o, v):
+    """HBM bytes per W_vvvv launch from the committed PMC summary of this workload, if any."""
+    path = os.path.join(ROOT, "profiles", "contract_f64_w_vvvv_o%d_v%d.md" % (o, v))
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch"):
+                return float(line.split("=")[1]), os.path.relpath(path, ROOT)
+    except OSError:
+        pass
+    return None, None
+
+
+def calibrate_copy(mib):
+    """dst = src over ``mib`` MiB of FP64 with dcc_tensor_axpby_f64 (reads and writes mib MiB each)."""
+    import ctypes
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray, Stream
+    n = mib * (1 << 20) // 8
+    src, dst = DeviceArray((n,)), DeviceArray((n,))
+    s = Stream()
+    src.fill(1.0, s)
+    lay = L.Strided()
+    lay.ndim = 1
+    lay.shape[0], lay.dst_stride[0], lay.src_stride[0] = n, 1, 1
+    for _ in range(3):
+        L.call("dcc_tensor_axpby_f64", dst.vp, src.vp, ctypes.byref(lay), 1.0, 0.0, s.handle)
+    s.synchronize()
+    print(json.dumps({"copy_mib": mib, "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +110,8 @@ def main():
     ap.add_argument("--nvir", type=int, default=90)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="print per-launch timings to stderr")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the triples path on one GPU; --gpus %d is not supported "
@@ -92,6 +124,9 @@ def main():
 
     o, v = args.nocc, args.nvir
     init(0)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     print("bench: building synthetic system o=%d v=%d" % (o, v), file=sys.stderr, flush=True)
     sysm = SyntheticSystem(o, v, naux=300, seed=20261202)
     im = sysm.bare_intermediates()
@@ -142,6 +177,7 @@ def main():
     dom_flops = float(np.mean([g.flops for g in dom]))
     achieved = dom_flops / (dom_avg_ms * 1e-3) / 1e12
     if args.breakdown:
+        print("bench: launch order per step: " + " | ".join(g.name for g in eng.launches), file=sys.stderr)
         tot = 0.0
         for g in eng.launches:
             d = np.mean(durs[g.name]) if durs[g.name] else float("nan")
@@ -170,7 +206,8 @@ def main():
                    "residual_frac_fp64_peak": F / (ms * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,
                    "r3_norm_sq": norms[0], "wall_s": wall},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_MFMA_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_MFMA_TFLOPS, "traffic": measured_traffic(o, v)[0],
+                     "traffic_source": measured_traffic(o, v)[1],
                      "kernel": "contract_f64_kernel (W_vvvv x T3 launches)",
                      "flops_per_launch": dom_flops, "avg_launch_ms": dom_avg_ms},
     }
