@@ -83,6 +83,83 @@ def measured_traffic(o, v):
     return None, None
 
 
+def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01):
+    """Full device RCCSDT iterations (T1 dressing, lower order, T3 -> r1/r2/W, triples residual,
+    update; no DIIS) from the synthetic amplitudes, once with FP64 triples and once with FP32 triples
+    (FP32 MFMA contractions, FP64 elsewhere).  Returns per-iteration |E_mixed - E_fp64| and the mean
+    time of one iteration in each precision (HIP events on the engine stream)."""
+    from distr_cc.device import Event, Stream
+    from distr_cc.rccsdt_engine import RCCSDTEngine
+    from distr_cc.synthetic import SyntheticSystem
+    sysm = SyntheticSystem(o, v, naux=300, seed=seed, sigma_b=sigma_b)
+    pppp = sysm.pppp()
+    t1, t2, t3 = np.zeros((o, v)), sysm.t2(), sysm.t3_tri()
+    out = {}
+    for name, dt in (("fp64", np.float64), ("fp32", np.float32)):
+        st = Stream()
+        eng = RCCSDTEngine(o, v, sysm.fock, pppp, sysm.mo_energy, stream=st, t3_dtype=dt)
+        eng.set_amplitudes(t1, t2, t3)
+        st.synchronize()
+        eng.update_amps()       # warm-up iteration (first-launch costs), then restart
+        eng.set_amplitudes(t1, t2, t3)
+        energies, times = [eng.energy()], []
+        phase_ev = []
+
+        def timer(ph, what):
+            e = Event()
+            e.record(st)
+            phase_ev.append((ph, what, e))
+
+        for _ in range(iters):
+            a, b = Event(), Event()
+            a.record(st)
+            eng.update_amps(timer=timer)
+            b.record(st)
+            times.append(a.elapsed_ms(b))
+            energies.append(eng.energy())
+        phases = {}
+        starts = {}
+        for ph, what, e in phase_ev:
+            if what == "start":
+                starts[ph] = e
+            else:
+                phases.setdefault(ph, []).append(starts[ph].elapsed_ms(e))
+        out[name] = (energies, times, {k: float(np.mean(x)) for k, x in phases.items()})
+        del eng
+    e64, e32 = out["fp64"][0], out["fp32"][0]
+    de = [abs(x - y) for x, y in zip(e32, e64)]
+    t64, t32 = float(np.mean(out["fp64"][1])), float(np.mean(out["fp32"][1]))
+    return {"workload": "synthetic RCCSDT iterations nocc=%d nvir=%d, B-factor sigma=%g (weakly correlated, "
+                        "so the Jacobi iteration converges), no DIIS, %d iterations" % (o, v, sigma_b, iters),
+            "fp64_ms_per_iteration": t64, "fp32_ms_per_iteration": t32, "speedup_fp32": t64 / t32,
+            "e_corr_fp64": e64, "abs_dE_per_iteration": de, "max_abs_dE": max(de),
+            "fp64_phase_ms": out["fp64"][2], "fp32_phase_ms": out["fp32"][2]}
+
+
+def q_time(o, v, blksize=4, seed=20261202):
+    """(Q)/[Q] of a synthetic canonical-block T3 already on the device (rccsdt_q.kernel's work:
+    integral slices to HBM, IJK -> ABC gather, W_vvvvoo/W_vvoooo, all tasks, energies)."""
+    from distr_cc.device import DeviceArray, Event, Stream
+    from distr_cc.q_engine import QEngine
+    from distr_cc.synthetic import SyntheticSystem
+    sysm = SyntheticSystem(o, v, naux=300, seed=seed)
+    pppp, t2, t3 = sysm.pppp(), sysm.t2(), sysm.t3_tri()
+    st = Stream()
+    t3d = DeviceArray.from_host(t3, st)
+    q = QEngine(o, v, blksize=blksize, stream=st)
+    q.kernel(pppp, t2, t3d, sysm.mo_energy)     # warm-up
+    a, b = Event(), Event()
+    w0 = time.perf_counter()
+    a.record(st)
+    eb, ep = q.kernel(pppp, t2, t3d, sysm.mo_energy)
+    b.record(st)
+    st.synchronize()
+    return {"workload": "synthetic (Q) nocc=%d nvir=%d blksize=%d" % (o, v, blksize),
+            "q_time_s": a.elapsed_ms(b) / 1e3, "wall_s": time.perf_counter() - w0,
+            "e_bracket": eb, "e_paren": ep,
+            "note": "includes host->device copies of the integral slices (PCIe) done inside the call"}
+
+
 def calibrate_copy(mib):
     """dst = src over ``mib`` MiB of FP64 with dcc_tensor_axpby_f64 (reads and writes mib MiB each)."""
     import ctypes
@@ -110,6 +187,10 @@ def main():
     ap.add_argument("--nvir", type=int, default=90)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="print per-launch timings to stderr")
+    ap.add_argument("--mixed-iters", type=int, default=3,
+                    help="full mixed-vs-FP64 iterations to time and compare (0 = skip)")
+    ap.add_argument("--q-nocc", type=int, default=5)
+    ap.add_argument("--q-nvir", type=int, default=19, help="(Q) timing size (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -211,6 +292,13 @@ def main():
                      "kernel": "contract_f64_kernel (W_vvvv x T3 launches)",
                      "flops_per_launch": dom_flops, "avg_launch_ms": dom_avg_ms},
     }
+    if args.mixed_iters > 0:
+        print("bench: mixed vs FP64 full iterations", file=sys.stderr, flush=True)
+        del eng
+        out["config"]["mixed_vs_fp64"] = mixed_vs_fp64(o, v, args.mixed_iters)
+    if args.q_nvir > 0:
+        print("bench: (Q) timing", file=sys.stderr, flush=True)
+        out["config"]["q"] = q_time(args.q_nocc, args.q_nvir)
     if not args.no_cpu_baseline:
         print("bench: CPU baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(o, v, sysm, im, t2, t3)

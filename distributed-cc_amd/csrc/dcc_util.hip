@@ -10,9 +10,9 @@ constexpr int kDotBlocks = 1024;  // partial sums per dot product (fixed: determ
 // dst[idx] = beta * dst[idx] + alpha * src[idx] over an up-to-6-d index space.
 // The innermost axis is mapped to consecutive lanes so both sides coalesce
 // when it is unit-stride.
-template <typename T>
-__global__ __launch_bounds__(256) void axpby_kernel(T *__restrict__ dst, const T *__restrict__ src,
-                                                    dcc_strided lay, int64_t total, T alpha, T beta) {
+template <typename TD, typename TS>
+__global__ __launch_bounds__(256) void axpby_kernel(TD *__restrict__ dst, const TS *__restrict__ src,
+                                                    dcc_strided lay, int64_t total, double alpha, double beta) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
         int64_t rem = t, od = 0, os = 0;
@@ -23,8 +23,8 @@ __global__ __launch_bounds__(256) void axpby_kernel(T *__restrict__ dst, const T
             od += x * lay.dst_stride[d];
             os += x * lay.src_stride[d];
         }
-        const T s = alpha * src[os];
-        dst[od] = (beta == T(0)) ? s : beta * dst[od] + s;
+        const double s = alpha * (double)src[os];
+        dst[od] = (TD)((beta == 0.0) ? s : beta * (double)dst[od] + s);
     }
 }
 
@@ -89,8 +89,8 @@ inline unsigned grid_for(int64_t total) {
     return (unsigned)blocks;
 }
 
-template <typename T>
-int tensor_axpby(T *dst, const T *src, const dcc_strided *layout, double alpha, double beta, dcc_stream_t stream) {
+template <typename TD, typename TS>
+int tensor_axpby(TD *dst, const TS *src, const dcc_strided *layout, double alpha, double beta, dcc_stream_t stream) {
     DCC_CHECK(layout != nullptr, "null layout");
     DCC_CHECK(layout->ndim >= 1 && layout->ndim <= 6, "ndim must be 1..6");
     int64_t total = 1;
@@ -100,8 +100,8 @@ int tensor_axpby(T *dst, const T *src, const dcc_strided *layout, double alpha, 
     }
     if (total == 0) return 0;
     DCC_CHECK(dst != nullptr && src != nullptr, "null pointer");
-    hipLaunchKernelGGL(axpby_kernel<T>, dim3(grid_for(total)), dim3(256), 0, dcc::as_stream(stream), dst, src,
-                       *layout, total, (T)alpha, (T)beta);
+    hipLaunchKernelGGL((axpby_kernel<TD, TS>), dim3(grid_for(total)), dim3(256), 0, dcc::as_stream(stream), dst,
+                       src, *layout, total, alpha, beta);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -145,12 +145,22 @@ int64_t dcc_dot_partials(void) { return kDotBlocks; }
 
 int dcc_tensor_axpby_f64(double *dst, const double *src, const dcc_strided *layout, double alpha, double beta,
                          dcc_stream_t stream) {
-    return tensor_axpby<double>(dst, src, layout, alpha, beta, stream);
+    return tensor_axpby<double, double>(dst, src, layout, alpha, beta, stream);
 }
 
 int dcc_tensor_axpby_f32(float *dst, const float *src, const dcc_strided *layout, double alpha, double beta,
                          dcc_stream_t stream) {
-    return tensor_axpby<float>(dst, src, layout, alpha, beta, stream);
+    return tensor_axpby<float, float>(dst, src, layout, alpha, beta, stream);
+}
+
+int dcc_tensor_axpby_f32_f64(float *dst, const double *src, const dcc_strided *layout, double alpha, double beta,
+                             dcc_stream_t stream) {
+    return tensor_axpby<float, double>(dst, src, layout, alpha, beta, stream);
+}
+
+int dcc_tensor_axpby_f64_f32(double *dst, const float *src, const dcc_strided *layout, double alpha, double beta,
+                             dcc_stream_t stream) {
+    return tensor_axpby<double, float>(dst, src, layout, alpha, beta, stream);
 }
 
 int dcc_dot_f64(const double *x, const double *y, int64_t n, double *partials, double *out, dcc_stream_t stream) {

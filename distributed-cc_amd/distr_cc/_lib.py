@@ -89,6 +89,8 @@ _SIGS = {
                             c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f64": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f32": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
+    "dcc_tensor_axpby_f32_f64": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
+    "dcc_tensor_axpby_f64_f32": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_dot_partials": ([], c_i64),
     "dcc_dot_f64": ([c_vp, c_vp, c_i64, c_vp, c_vp, c_vp], ctypes.c_int),
     "dcc_dot_f32": ([c_vp, c_vp, c_i64, c_vp, c_vp, c_vp], ctypes.c_int),

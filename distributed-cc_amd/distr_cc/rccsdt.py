@@ -127,6 +127,9 @@ class RCCSDT:
     level_shift = 0.0
     verbose = 4
     callback = None
+    # "fp64" (the reference's arithmetic) or "fp32": T3 and the triples contractions in FP32 on MFMA,
+    # everything else in FP64
+    triples_precision = "fp64"
 
     def __init__(self, mf, comm=None, frozen=None, mo_coeff=None, mo_occ=None):
         if comm is None:
@@ -224,8 +227,9 @@ class RCCSDT:
     def _engine(self, eris):
         if self.engine is None or self.engine._eris is not eris:
             nocc, nmo = self.nocc, self.nmo
+            t3_dtype = {"fp64": np.float64, "fp32": np.float32}[self.triples_precision]
             self.engine = RCCSDTEngine(nocc, nmo - nocc, eris.fock, eris.pppp, eris.mo_energy,
-                                       level_shift=self.level_shift, nvir_diis=self.nvir_diis)
+                                       level_shift=self.level_shift, nvir_diis=self.nvir_diis, t3_dtype=t3_dtype)
             self.engine._eris = eris
         return self.engine
 

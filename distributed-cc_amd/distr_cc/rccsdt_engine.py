@@ -60,7 +60,10 @@ class RCCSDTEngine:
     """Device state of the RCCSDT amplitude iteration for one closed-shell system."""
 
     def __init__(self, nocc, nvir, fock, pppp, mo_energy, level_shift=0.0, nvir_diis=None, stream=None,
-                 device=0):
+                 device=0, t3_dtype=np.float64):
+        """``t3_dtype`` float32 selects the mixed-precision iteration: T3, the triples residual and its
+        operands in FP32 with FP32-MFMA contractions, everything else (integrals, T1/T2, lower-order
+        residual, denominators, projection arithmetic, norms, energy, DIIS) in FP64."""
         o, v = int(nocc), int(nvir)
         n = o + v
         fock = np.asarray(fock, dtype=np.float64)
@@ -71,7 +74,8 @@ class RCCSDTEngine:
         self.stream = stream
         self.level_shift = float(level_shift)
         self.nvir_diis = v if nvir_diis is None else int(nvir_diis)
-        self.tr = tr = TriplesEngine(o, v, stream=stream, device=device)
+        self.tr = tr = TriplesEngine(o, v, stream=stream, device=device, dtype=t3_dtype)
+        self.mixed = tr.dtype != np.float64
         self.ext = {x: o for x in OCC}
         self.ext.update({x: v for x in VIR})
         self.ext.update({x: n for x in GEN})
@@ -89,7 +93,10 @@ class RCCSDTEngine:
         self.Fd = DeviceArray((n, n), f64)
         self.ones = DeviceArray.from_host(np.ones((o, 1)), stream)
         self.t1 = DeviceArray.zeros((o, v), f64, stream)
-        self.t2 = tr.t2
+        self.t2 = DeviceArray((o, o, v, v), f64) if self.mixed else tr.t2
+        # FP64 targets of the triples intermediates (the triples engine's own buffers unless mixed)
+        names = ("F_oo", "F_vv", "W_oooo", "W_vvvv", "W_ovov", "W_ovvo", "W_ovvv", "W_oovo")
+        self.imd = {k: (DeviceArray(getattr(tr, k).shape, f64) if self.mixed else getattr(tr, k)) for k in names}
         self.r1 = DeviceArray((o, v), f64)
         self.r2 = DeviceArray((o, o, v, v), f64)
         self.r2s = DeviceArray((o, o, v, v), f64)
@@ -240,17 +247,17 @@ class RCCSDTEngine:
 
         # -- T2-dressed triples intermediates into the triples engine (:935)
         p = _Program(self)
-        F_oo, F_vv = nv(tr.F_oo, "li"), nv(tr.F_vv, "ad")
+        F_oo, F_vv = nv(self.imd["F_oo"], "li"), nv(self.imd["F_vv"], "ad")
         p.axpby(F_oo, gv(Fd, "li"), "li", 1.0, 0.0)
         p.ein("F_oo a", "lmde,imde->li", gv(G, "lmde"), t2v("imde"), F_oo, 2.0, 1.0)
         p.ein("F_oo b", "lmed,imde->li", gv(G, "lmed"), t2v("imde"), F_oo, -1.0, 1.0)
         p.axpby(F_vv, gv(Fd, "ad"), "ad", 1.0, 0.0)
         p.ein("F_vv a", "lmde,lmae->ad", gv(G, "lmde"), t2v("lmae"), F_vv, -2.0, 1.0)
         p.ein("F_vv b", "lmed,lmae->ad", gv(G, "lmed"), t2v("lmae"), F_vv, 1.0, 1.0)
-        W_oooo = nv(tr.W_oooo, "lmij")
+        W_oooo = nv(self.imd["W_oooo"], "lmij")
         p.axpby(W_oooo, gv(G, "lmij"), "lmij", 1.0, 0.0)
         p.ein("W_oooo", "lmde,ijde->lmij", gv(G, "lmde"), t2v("ijde"), W_oooo, 1.0, 1.0)
-        W_vvvv = nv(tr.W_vvvv, "abde")
+        W_vvvv = nv(self.imd["W_vvvv"], "abde")
         p.axpby(W_vvvv, gv(G, "abde"), "abde", 1.0, 0.0)
         p.ein("W_vvvv", "lmde,lmab->abde", gv(G, "lmde"), t2v("lmab"), W_vvvv, 1.0, 1.0)
         h = nv(self.h_ovvo, "ladi")
@@ -258,13 +265,13 @@ class RCCSDTEngine:
         p.ein("h_ovvo a", "mled,imea->ladi", gv(G, "mled"), t2v("imea"), h, -1.0, 1.0)
         p.ein("h_ovvo b", "mlde,imae->ladi", gv(G, "mlde"), t2v("imae"), h, -1.0, 1.0)
         p.ein("h_ovvo c", "mled,imae->ladi", gv(G, "mled"), t2v("imae"), h, 2.0, 1.0)
-        W_ovov = nv(tr.W_ovov, "lbid")
+        W_ovov = nv(self.imd["W_ovov"], "lbid")
         p.axpby(W_ovov, gv(G, "lbid"), "lbid", 1.0, 0.0)
         p.ein("W_ovov", "mlde,imeb->lbid", gv(G, "mlde"), t2v("imeb"), W_ovov, -1.0, 1.0)
-        W_ovvo = nv(tr.W_ovvo, "ladi")
+        W_ovvo = nv(self.imd["W_ovvo"], "ladi")
         p.axpby(W_ovvo, h, "ladi", 2.0, 0.0)
-        p.axpby(W_ovvo, nv(tr.W_ovov, "laid"), "ladi", -1.0, 1.0)
-        Wvvvo = nv(tr.W_ovvv, "jabd")        # W_vvvo[a,b,d,j] stored as [j,a,b,d]
+        p.axpby(W_ovvo, nv(self.imd["W_ovov"], "laid"), "ladi", -1.0, 1.0)
+        Wvvvo = nv(self.imd["W_ovvv"], "jabd")        # W_vvvo[a,b,d,j] stored as [j,a,b,d]
         p.axpby(Wvvvo, gv(G, "abdj"), "abdj", 1.0, 0.0)
         p.ein("W_vvvo a", "amde,jmbe->abdj", gv(G, "amde"), t2v("jmbe"), Wvvvo, 2.0, 1.0)
         p.ein("W_vvvo b", "amde,jmeb->abdj", gv(G, "amde"), t2v("jmeb"), Wvvvo, -1.0, 1.0)
@@ -272,13 +279,21 @@ class RCCSDTEngine:
         p.ein("W_vvvo d", "bmed,jmea->abdj", gv(G, "bmed"), t2v("jmea"), Wvvvo, -1.0, 1.0)
         p.ein("W_vvvo e", "mndj,mnab->abdj", gv(G, "mndj"), t2v("mnab"), Wvvvo, 1.0, 1.0)
         p.ein("W_vvvo f", "md,mjab->abdj", gv(Fd, "md"), t2v("mjab"), Wvvvo, -1.0, 1.0)
-        Wvooo = nv(tr.W_oovo, "ijal")         # W_vooo[a,l,i,j] stored as [i,j,a,l]
+        Wvooo = nv(self.imd["W_oovo"], "ijal")         # W_vooo[a,l,i,j] stored as [i,j,a,l]
         p.axpby(Wvooo, gv(G, "alij"), "alij", 1.0, 0.0)
         p.ein("W_vooo a", "alef,ijef->alij", gv(G, "alef"), t2v("ijef"), Wvooo, 1.0, 1.0)
         p.ein("W_vooo b", "lmje,imae->alij", gv(G, "lmje"), t2v("imae"), Wvooo, 2.0, 1.0)
         p.ein("W_vooo c", "lmje,imea->alij", gv(G, "lmje"), t2v("imea"), Wvooo, -1.0, 1.0)
         p.ein("W_vooo d", "lmej,imae->alij", gv(G, "lmej"), t2v("imae"), Wvooo, -1.0, 1.0)
         p.ein("W_vooo e", "lmei,jmea->alij", gv(G, "lmei"), t2v("jmea"), Wvooo, -1.0, 1.0)
+        if self.mixed:   # round the FP64 intermediates and T2 into the FP32 triples engine
+            for k, buf in self.imd.items():
+                dst = getattr(tr, k)
+                letters = "ABCD"[:len(buf.shape)]
+                p.steps.append(lambda _s, d=dst, b=buf, l=letters: tensor_axpby(
+                    Operand(d, l, d.shape), Operand(b, l, b.shape), l, d.shape, 1.0, 0.0, self.stream))
+            p.steps.append(lambda _s: tensor_axpby(Operand(tr.t2, "ijab", (o, o, v, v)), t2v("ijab"), "ijab",
+                                                   (o, o, v, v), 1.0, 0.0, self.stream))
         steps["imds3"] = p
         return steps
 
@@ -293,19 +308,27 @@ class RCCSDTEngine:
         s = self.stream
         return self.t1.download(stream=s), self.t2.download(stream=s), self.tr.get_t3()
 
-    def update_amps(self):
+    def update_amps(self, timer=None):
         """One iteration of update_amps_rccsdt_tri_ (rccsdt.py:890-977) on the device.
 
-        Returns [|r1|, |r2|, |r3|] (the res_norm of :929 and :966)."""
+        Returns [|r1|, |r2|, |r3|] (the res_norm of :929 and :966).  ``timer`` (optional) is
+        called as timer(phase, 'start'|'end') around each phase, e.g. to record HIP events."""
         st = self.stream
-        self.prog["dress"].run(st)
-        self.prog["r1r2"].run(st)
-        self.prog["update12"].run(st)
-        self.prog["imds3"].run(st)
-        self.tr.t3_to_w()
-        self.tr.prepare_residual()
-        self.tr.residual()
+        tick = timer if timer is not None else (lambda name, what: None)
+        phases = [("dress", lambda: self.prog["dress"].run(st)),
+                  ("lower r1/r2", lambda: self.prog["r1r2"].run(st)),
+                  ("T3->r1/r2 + update t1/t2", lambda: self.prog["update12"].run(st)),
+                  ("triples intermediates", lambda: self.prog["imds3"].run(st)),
+                  ("T3->W", self.tr.t3_to_w),
+                  ("ring/oooo operands", self.tr.prepare_residual),
+                  ("triples residual", self.tr.residual)]
+        for name, fn in phases:
+            tick(name, "start")
+            fn()
+            tick(name, "end")
+        tick("finalize", "start")
         n3, act = self.tr.finalize(self.mo_energy, self.level_shift, self.nvir_diis)
+        tick("finalize", "end")
         self.last_t3_norm_sq, self.last_t3_active_norm_sq = n3, act
         sc = self.scal.download(stream=st)
         return [float(np.sqrt(max(sc[0], 0.0))), float(np.sqrt(max(sc[1], 0.0))), float(np.sqrt(max(n3, 0.0)))]
@@ -341,8 +364,11 @@ class RCCSDTEngine:
         lay.ndim = len(shape)
         for d, (e, sd, ss) in enumerate(zip(shape, dst_strides, src_strides)):
             lay.shape[d], lay.dst_stride[d], lay.src_stride[d] = int(e), int(sd), int(ss)
-        L.call("dcc_tensor_axpby_f64", dst.offset_ptr(dst_off), src.offset_ptr(src_off), ctypes.byref(lay),
-               float(alpha), float(beta), self.stream.handle if self.stream is not None else None)
+        sd = "f64" if dst.dtype == np.float64 else "f32"
+        ss = "f64" if src.dtype == np.float64 else "f32"
+        L.call("dcc_tensor_axpby_" + (sd if sd == ss else sd + "_" + ss), dst.offset_ptr(dst_off),
+               src.offset_ptr(src_off), ctypes.byref(lay), float(alpha), float(beta),
+               self.stream.handle if self.stream is not None else None)
 
     def pack(self, x):
         o, v, nd, n3 = self.nocc, self.nvir, self.nvir_diis, self.tr.n3

@@ -194,18 +194,17 @@ def plan_einsum(name, sub, A, B, C, alpha, beta, ext, batches, max_levels=3):
 def tensor_axpby(dst, src, letters, shape, alpha, beta, stream):
     """dst (+)= alpha * src over the index space ``letters`` x ``shape`` (Operand views;
     a letter missing from ``src`` broadcasts with stride 0)."""
-    if dst.buf.dtype != src.buf.dtype:
-        raise TypeError("tensor_axpby: dtypes differ")
+    sd, ss = _suffix(dst.buf.dtype), _suffix(src.buf.dtype)
+    fn = "dcc_tensor_axpby_" + (sd if sd == ss else sd + "_" + ss)
     lay = L.Strided()
     lay.ndim = len(letters)
     for d, (x, e) in enumerate(zip(letters, shape)):
         lay.shape[d] = int(e)
         lay.dst_stride[d] = dst.strides[x]
         lay.src_stride[d] = src.strides.get(x, 0)
-    item = dst.buf.dtype.itemsize
-    L.call("dcc_tensor_axpby_" + _suffix(dst.buf.dtype), ctypes.c_void_p(dst.buf.ptr + dst.base * item),
-           ctypes.c_void_p(src.buf.ptr + src.base * item), ctypes.byref(lay), float(alpha), float(beta),
-           stream.handle if stream is not None else None)
+    L.call(fn, ctypes.c_void_p(dst.buf.ptr + dst.base * dst.buf.dtype.itemsize),
+           ctypes.c_void_p(src.buf.ptr + src.base * src.buf.dtype.itemsize), ctypes.byref(lay), float(alpha),
+           float(beta), stream.handle if stream is not None else None)
 
 
 def plan_term(name, sub, A, B, C, alpha, beta, batches_fixed, ext):
@@ -458,8 +457,6 @@ class TriplesEngine:
         integrals G = t1_eris (nmo^4) and Fock Fd = t1_fock (nmo^2), both FP64."""
         if self._lower is None:
             self._setup_lower()
-        if self.dtype != np.float64:
-            raise TypeError("device dressing needs the FP64 triples engine")
         lo, o, v, n = self._lower, self.nocc, self.nvir, int(nmo)
         g_str = {'P': n ** 3, 'Q': n ** 2, 'R': n, 'S': 1}
         fv = o  # first virtual orbital

@@ -230,6 +230,12 @@ int dcc_tensor_axpby_f64(double *dst, const double *src, const dcc_strided *layo
                          dcc_stream_t stream);
 int dcc_tensor_axpby_f32(float *dst, const float *src, const dcc_strided *layout, double alpha, double beta,
                          dcc_stream_t stream);
+/* Converting forms (FP64 arithmetic, rounded to dst): the FP32-storage triples path exchanges T2, the
+ * intermediates and the T3 partial sums with the FP64 lower-order iteration through these. */
+int dcc_tensor_axpby_f32_f64(float *dst, const double *src, const dcc_strided *layout, double alpha, double beta,
+                             dcc_stream_t stream);
+int dcc_tensor_axpby_f64_f32(double *dst, const float *src, const dcc_strided *layout, double alpha, double beta,
+                             dcc_stream_t stream);
 
 /* Number of doubles the `partials` scratch of dcc_dot_* must hold. */
 int64_t dcc_dot_partials(void);

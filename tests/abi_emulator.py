@@ -144,7 +144,14 @@ class Emulator:
     def dcc_tensor_axpby_f32(self, *a):
         return self._axpby(np.float32, *a)
 
+    def dcc_tensor_axpby_f32_f64(self, *a):
+        return self._axpby((np.float32, np.float64), *a)
+
+    def dcc_tensor_axpby_f64_f32(self, *a):
+        return self._axpby((np.float64, np.float32), *a)
+
     def _axpby(self, dt, dst, src, layout, alpha, beta, stream):
+        dtd, dts = dt if isinstance(dt, tuple) else (dt, dt)
         lay = layout._obj if hasattr(layout, "_obj") else layout.contents
         nd = int(lay.ndim)
         shape = [int(lay.shape[d]) for d in range(nd)]
@@ -153,11 +160,11 @@ class Emulator:
         idx = np.indices(shape).reshape(nd, -1)
         od = sum(idx[d] * int(lay.dst_stride[d]) for d in range(nd))
         os_ = sum(idx[d] * int(lay.src_stride[d]) for d in range(nd))
-        D, S = self.dev.view(_ptr(dst), dt), self.dev.view(_ptr(src), dt)
+        D, S = self.dev.view(_ptr(dst), dtd), self.dev.view(_ptr(src), dts)
         val = alpha * S[os_].astype(np.float64)
         if beta != 0.0:
             val = val + beta * D[od].astype(np.float64)
-        D[od] = val.astype(dt)
+        D[od] = val.astype(dtd)
         return 0
 
     def dcc_dot_partials(self):

@@ -167,3 +167,22 @@ def test_kernel_converges_to_oracle_fixed_point(monkeypatch):
             break
     assert abs(e_cc - tri.energy_rhf(o, F, g, t1, t2)) < 1e-10
     assert abs(tamps[1] - t2).max() < 1e-8
+
+
+def test_mixed_iteration_plan(monkeypatch):
+    """FP32 triples inside the FP64 iteration: one step stays within FP32 rounding of the oracle."""
+    abi_emulator.install(monkeypatch)
+    from ccsdt_cases import molecule_like, random_amplitudes
+    from distr_cc.rccsdt_engine import RCCSDTEngine
+    from oracle import rccsdt_lower as lo
+    o, v = 3, 4
+    F, g, e, _ = molecule_like(o, v, 2)
+    t1, t2, t3 = random_amplitudes(o, v, 102)
+    eng = RCCSDTEngine(o, v, F, g, e, t3_dtype=np.float32)
+    eng.set_amplitudes(t1, t2, t3)
+    eng.update_amps()
+    a1, a2, a3 = eng.get_amplitudes()
+    b1, b2, b3, _ = lo.update_amps(o, F, g, e, t1, t2, t3)
+    assert abs(a1 - b1).max() <= 1e-6 * abs(b1).max()
+    assert abs(a2 - b2).max() <= 1e-6 * abs(b2).max()
+    assert abs(a3 - b3).max() <= 1e-5 * abs(b3).max()

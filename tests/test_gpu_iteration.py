@@ -124,3 +124,45 @@ def test_rccsdt_then_q(dev):
         eb, ep = rccsdt_q.kernel(cc, eris=eris, blksize=bs, comm=SINGLE_RANK)
         rb, rp = q_energy.kernel(o, v, g, cc.t2, full_t3_from_tri(cc.t3[1], o), e, blksize=bs)
         assert abs(eb - rb) < 1e-12 and abs(ep - rp) < 1e-12
+
+
+def test_mixed_iteration_vs_oracle(dev):
+    from ccsdt_cases import molecule_like, random_amplitudes
+    from distr_cc.rccsdt_engine import RCCSDTEngine
+    o, v = 4, 9
+    _say("mixed iteration o=%d v=%d" % (o, v))
+    F, g, e, _ = molecule_like(o, v, 5)
+    t1, t2, t3 = random_amplitudes(o, v, 105)
+    eng = RCCSDTEngine(o, v, F, g, e, t3_dtype=np.float32)
+    eng.set_amplitudes(t1, t2, t3)
+    eng.update_amps()
+    a1, a2, a3 = eng.get_amplitudes()
+    b1, b2, b3, _ = lo.update_amps(o, F, g, e, t1, t2, t3)
+    assert abs(a1 - b1).max() <= 1e-6 * abs(b1).max()
+    assert abs(a2 - b2).max() <= 1e-6 * abs(b2).max()
+    assert abs(a3 - b3).max() <= 1e-5 * abs(b3).max()
+
+
+def test_mixed_kernel_energy_error(dev):
+    """Converged E_corr with FP32 triples within 1 microhartree of the FP64 path.
+
+    FP64 is converged tightly; the FP32 run uses the reference's default thresholds
+    (conv_tol 1e-7, conv_tol_normt 1e-6, rccsdt.py:1164-1165), since FP32 accumulation puts a
+    floor under the T3 residual norm."""
+    from ccsdt_cases import DuckMF, molecule_like
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 4, 10
+    _say("mixed kernel o=%d v=%d" % (o, v))
+    F, g, e, _ = molecule_like(o, v, 51)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+        cc.verbose, cc.triples_precision = 0, prec
+        if prec == "fp64":
+            cc.conv_tol, cc.conv_tol_normt = 1e-10, 1e-8
+        res[prec] = cc.kernel(eris=_PhysicistsERIs(F, e, g))[0]
+        _say("%s converged=%s cycles=%d" % (prec, cc.converged, cc.cycles))
+        assert cc.converged
+    _say("E fp64 %.12f fp32 %.12f" % (res["fp64"], res["fp32"]))
+    assert abs(res["fp64"] - res["fp32"]) < 1e-6
