@@ -252,8 +252,8 @@ def main():
         lst = ev.get(id(g), [])
         starts = [e for w, e in lst if w == "start"]
         ends = [e for w, e in lst if w == "end"]
-        durs[g.name] = [a.elapsed_ms(b) for a, b in zip(starts, ends)]
-    dom_ms = [d for g in dom for d in durs[g.name]]
+        durs[id(g)] = [a.elapsed_ms(b) for a, b in zip(starts, ends)]
+    dom_ms = [d for g in dom for d in durs[id(g)]]
     dom_avg_ms = float(np.mean(dom_ms))
     dom_flops = float(np.mean([g.flops for g in dom]))
     achieved = dom_flops / (dom_avg_ms * 1e-3) / 1e12
@@ -261,7 +261,7 @@ def main():
         print("bench: launch order per step: " + " | ".join(g.name for g in eng.launches), file=sys.stderr)
         tot = 0.0
         for g in eng.launches:
-            d = np.mean(durs[g.name]) if durs[g.name] else float("nan")
+            d = np.mean(durs[id(g)]) if durs[id(g)] else float("nan")
             tot += d
             print("  %-36s %9.3f ms  %7.2f TF/s" % (g.name, d, g.flops / (d * 1e-3) / 1e12), file=sys.stderr)
         print("  sum of GEMM launches %.3f ms of %.3f ms per step" % (tot, ms), file=sys.stderr)

@@ -122,7 +122,12 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t L, int64_t total) {
     return (xcd < r) ? xcd * (q + 1) + local : r * (q + 1) + (xcd - r) * q + local;
 }
 
-template <typename T, int BM, int BN, bool A_KFAST, bool B_NFAST>
+template <int a, int b> struct CMax { static constexpr int v = a > b ? a : b; };
+
+// C_MFAST selects the epilogue: false writes C straight from the accumulators (16 consecutive lanes
+// walk n, coalesced when C is contiguous along n); true stages the tile in LDS and writes it with
+// consecutive lanes walking m, for outputs whose contiguous index is carried by M.
+template <typename T, int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST>
 __global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T> p) {
     typedef Mfma<T> MF;
     typedef typename MF::acc_t acc_t;
@@ -131,8 +136,11 @@ __global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T
     constexpr int LDA = BM + 1, LDB = BN + 1;
     constexpr int A_PER_T = BM * kBK / kThreads;
     constexpr int B_PER_T = BN * kBK / kThreads;
-    __shared__ T As[2][kBK][LDA];
-    __shared__ T Bs[2][kBK][LDB];
+    constexpr int MAIN = 2 * kBK * LDA + 2 * kBK * LDB;
+    constexpr int SMEM = C_MFAST ? CMax<MAIN, BN * (BM + 1)>::v : MAIN;
+    __shared__ T smem[SMEM];
+    T (*As)[kBK][LDA] = reinterpret_cast<T (*)[kBK][LDA]>(smem);
+    T (*Bs)[kBK][LDB] = reinterpret_cast<T (*)[kBK][LDB]>(smem + 2 * kBK * LDA);
 
     const int tid = threadIdx.x;
     const int64_t L = xcd_remap((int64_t)blockIdx.x, p.total_tiles);
@@ -235,6 +243,33 @@ __global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T
         __syncthreads();
     }
 
+    if constexpr (C_MFAST) {
+        // ---- LDS-staged epilogue: tile stored n-major, then written with lanes along m ----
+        T *Cs = smem;  // the main loop ended with a barrier, so the staging buffers are free
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    Cs[(wn * WN + j * 16 + lrow) * (BM + 1) + wm * WM + i * 16 + MF::row(lk, r)] = acc[i][j][r];
+        __syncthreads();
+        constexpr int NSTEP = kThreads / BM;
+        const int ml = tid % BM;
+        const int64_t m = m0 + ml;
+        if (m >= m_end) return;
+        const int64_t crow = bt.c_off + map_off(p.cm, (uint32_t)m, batch);
+        for (int nl = tid / BM; nl < BN; nl += NSTEP) {
+            const int n = n0 + nl;
+            if (n >= p.N) break;
+            T *c = p.C + crow + map_off(p.cn, (uint32_t)n, batch);
+            double v = p.alpha * (double)Cs[nl * (BM + 1) + ml];
+            if (p.beta != 0.0) v += p.beta * (double)*c;
+            *c = (T)v;
+        }
+        return;
+    }
+
     // ---- epilogue: accumulator row of (lane, r) from Mfma<T>::row, column lane%16 ----
     int64_t ccol[FN];
     uint32_t cok = 0;
@@ -298,8 +333,20 @@ KMap make_kmap(const dcc_idxmap &m) {
     return k;
 }
 
+template <typename T, int BM, int BN, bool C_MFAST>
+void launch_variant(ContractArgs<T> &args, bool a_kfast, bool b_nfast, dim3 grid, dim3 block, hipStream_t stream) {
+    if (a_kfast && b_nfast)
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, true, C_MFAST>), grid, block, 0, stream, args);
+    else if (a_kfast)
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, false, C_MFAST>), grid, block, 0, stream, args);
+    else if (b_nfast)
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, true, C_MFAST>), grid, block, 0, stream, args);
+    else
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, false, C_MFAST>), grid, block, 0, stream, args);
+}
+
 template <typename T, int BM, int BN>
-int launch(ContractArgs<T> &args, int64_t max_rows, bool a_kfast, bool b_nfast, hipStream_t stream) {
+int launch(ContractArgs<T> &args, int64_t max_rows, bool a_kfast, bool b_nfast, bool c_mfast, hipStream_t stream) {
     args.tiles_m = (int)((max_rows + BM - 1) / BM);
     args.tiles_n = (int)((args.N + BN - 1) / BN);
     const int64_t nbatch = args.total_tiles;  // caller passes nbatch here
@@ -309,14 +356,10 @@ int launch(ContractArgs<T> &args, int64_t max_rows, bool a_kfast, bool b_nfast, 
         return 2;
     }
     const dim3 grid((unsigned)args.total_tiles), block(kThreads);
-    if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, true>), grid, block, 0, stream, args);
-    else if (a_kfast)
-        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, false>), grid, block, 0, stream, args);
-    else if (b_nfast)
-        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, true>), grid, block, 0, stream, args);
+    if (c_mfast)
+        launch_variant<T, BM, BN, true>(args, a_kfast, b_nfast, grid, block, stream);
     else
-        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, false>), grid, block, 0, stream, args);
+        launch_variant<T, BM, BN, false>(args, a_kfast, b_nfast, grid, block, stream);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -361,6 +404,8 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     // choose the contiguous index of each operand as the lane-fastest one
     const bool a_kfast = inner_stride(ak) == 1 || inner_stride(am) != 1;
     const bool b_nfast = inner_stride(bn) == 1 || inner_stride(bk) != 1;
+    // C written along m (through LDS) unless it is contiguous along n
+    const bool c_mfast = inner_stride(cn) != 1;
     hipStream_t s = dcc::as_stream(stream);
     // Tile shape: DCC_CONTRACT_TILE=128x128|128x64|64x128|64x64 overrides the default.
     static const char *tile_env = getenv("DCC_CONTRACT_TILE");
@@ -371,10 +416,10 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
         tbm = 64;
         tbn = 64;
     }
-    if (tbm == 128 && tbn == 128) return launch<T, 128, 128>(args, max_rows, a_kfast, b_nfast, s);
-    if (tbm == 128 && tbn == 64) return launch<T, 128, 64>(args, max_rows, a_kfast, b_nfast, s);
-    if (tbm == 64 && tbn == 128) return launch<T, 64, 128>(args, max_rows, a_kfast, b_nfast, s);
-    return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, s);
+    if (tbm == 128 && tbn == 128) return launch<T, 128, 128>(args, max_rows, a_kfast, b_nfast, c_mfast, s);
+    if (tbm == 128 && tbn == 64) return launch<T, 128, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, s);
+    if (tbm == 64 && tbn == 128) return launch<T, 64, 128>(args, max_rows, a_kfast, b_nfast, c_mfast, s);
+    return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, s);
 }
 
 }  // namespace

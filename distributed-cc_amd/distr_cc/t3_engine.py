@@ -212,6 +212,21 @@ def plan_term(name, sub, A, B, C, alpha, beta, batches_fixed, ext):
     return plan_einsum(name, sub, A, B, C, alpha, beta, ext, [(f, f, f) for f in batches_fixed])
 
 
+class S3Launch:
+    """One dcc_s3_transform call (out = beta*out + sum_s coef[s] src^{TP_T3[s]} per job) in a launch list."""
+
+    def __init__(self, name, out, src, nvir, beta, jobs):
+        self.name, self.flops = name, 0.0
+        self.fn = "dcc_s3_transform_" + _suffix(out.dtype)
+        self.out, self.src, self.nvir, self.beta = out, src, int(nvir), float(beta)
+        self.jobs = DeviceArray.from_host(s3_jobs_array(jobs))
+        self.njobs = len(jobs)
+
+    def run(self, stream):
+        L.call(self.fn, self.out.vp, self.src.vp, self.nvir, self.beta, self.jobs.vp, self.njobs,
+               stream.handle if stream is not None else None)
+
+
 def s3_jobs_array(jobs):
     arr = np.zeros(len(jobs), dtype=L.S3JOB_DTYPE)
     for n, (dst, src, coef) in enumerate(jobs):
@@ -259,6 +274,7 @@ class TriplesEngine:
         self.WR3a = DeviceArray((o, v, o, v), dt)
         self.WR3b = DeviceArray((o, v, o, v), dt)
         self.pfs = DeviceArray((6, n3, n3), dt)
+        self.Xp = DeviceArray((n3, v, v, v), dt)     # one W_oooo/F_oo term before its permutation
         self.eia = DeviceArray((o, v))
         self.norb = int(L._lib.dcc_r3_finalize_tiles(v))
         self.partials = DeviceArray((n3 * self.norb, 2))
@@ -316,16 +332,23 @@ class TriplesEngine:
             B = Operand(self.t3, tl, (n3, v, v, v))
             launches.append(plan_term('W_vvvv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
         launches += self._plan_ring()
-        # W_oooo / F_oo (rccsdt.py:726-731)
-        pf_terms = ['pq,pabc->qabc', 'pq,pacb->qabc', 'pq,pbac->qabc',
-                    'pq,pabc->qcab', 'pq,pabc->qbca', 'pq,pabc->qcba']
-        for s, sub in enumerate(pf_terms):
-            sa, rest = sub.split(',')
-            sb, sc = rest.split('->')
-            A = Operand(self.pfs, 'Spq', (6, n3, n3))
-            B = Operand(self.t3, sb, (n3, v, v, v))
-            C = Operand(self.R, sc, (n3, v, v, v))
-            launches.append(plan_term('W_oooo/F_oo ' + sub, sub, A, B, C, 1.0, 1.0, [dict(S=s)], ext))
+        # W_oooo / F_oo (rccsdt.py:726-731).  The abc permutation of each of the six terms commutes
+        # with the sum over source blocks p, so every term is the same natural GEMM
+        # X_s[q] = sum_p pfs[s][p][q] T[p] (coalesced in and out) followed, for s > 0, by R[q] +=
+        # X_s[q] transposed by the term's permutation (an S3 orbit-tile pass).
+        # s: 0 'pq,pabc->qabc', 1 'pq,pacb->qabc', 2 'pq,pbac->qabc', 3 'pq,pabc->qcab',
+        #    4 'pq,pabc->qbca', 5 'pq,pabc->qcba'; R[q] += X_s[q].transpose(perm[s]).
+        A = Operand(self.pfs, 'Spq', (6, n3, n3))
+        B = Operand(self.t3, 'pabc', (n3, v, v, v))
+        Rq = Operand(self.R, 'qabc', (n3, v, v, v))
+        launches.append(plan_term('W_oooo/F_oo s0', 'pq,pabc->qabc', A, B, Rq, 1.0, 1.0, [dict(S=0)], ext))
+        X = Operand(self.Xp, 'qabc', (n3, v, v, v))
+        for s in range(1, 6):
+            launches.append(plan_term('W_oooo/F_oo s%d' % s, 'pq,pabc->qabc', A, B, X, 1.0, 0.0, [dict(S=s)], ext))
+            coef = np.zeros(6)
+            coef[TP_T3.index({1: (0, 2, 1), 2: (1, 0, 2), 3: (2, 0, 1), 4: (1, 2, 0), 5: (2, 1, 0)}[s])] = 1.0
+            jobs = [(q * v ** 3, q * v ** 3, coef) for q in range(n3)]
+            launches.append(S3Launch('W_oooo/F_oo add s%d' % s, self.R, self.Xp, v, 1.0, jobs))
         return launches
 
     def _plan_ring(self):
@@ -356,6 +379,8 @@ class TriplesEngine:
         for part in (3, 4, 5):
             S = place[part]
             for wname, mx, (n0, n1), sd, sn0, sn1 in terms:
+                if S[n0] < S[n1]:    # innermost N index = the one contiguous in R (coalesced C)
+                    n0, n1, sn0, sn1 = n1, n0, sn1, sn0
                 # per-pair row table: cm[pair][(r,x)] = target(r)*v^3 + x*S[mx]
                 tbl = np.zeros((self.npairs, ov), dtype=np.int64)
                 batches = []
