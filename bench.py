@@ -95,9 +95,11 @@ def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01):
     pppp = sysm.pppp()
     t1, t2, t3 = np.zeros((o, v)), sysm.t2(), sysm.t3_tri()
     out = {}
-    for name, dt in (("fp64", np.float64), ("fp32", np.float32)):
+    for name, dt, contract in (("fp64", np.float64, "native"), ("fp32", np.float32, "native"),
+                               ("bf16x3", np.float32, "bf16x3")):
         st = Stream()
-        eng = RCCSDTEngine(o, v, sysm.fock, pppp, sysm.mo_energy, stream=st, t3_dtype=dt)
+        eng = RCCSDTEngine(o, v, sysm.fock, pppp, sysm.mo_energy, stream=st, t3_dtype=dt,
+                           triples_contract=contract)
         eng.set_amplitudes(t1, t2, t3)
         st.synchronize()
         eng.update_amps()       # warm-up iteration (first-launch costs), then restart
@@ -126,14 +128,18 @@ def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01):
                 phases.setdefault(ph, []).append(starts[ph].elapsed_ms(e))
         out[name] = (energies, times, {k: float(np.mean(x)) for k, x in phases.items()})
         del eng
-    e64, e32 = out["fp64"][0], out["fp32"][0]
-    de = [abs(x - y) for x, y in zip(e32, e64)]
-    t64, t32 = float(np.mean(out["fp64"][1])), float(np.mean(out["fp32"][1]))
-    return {"workload": "synthetic RCCSDT iterations nocc=%d nvir=%d, B-factor sigma=%g (weakly correlated, "
-                        "so the Jacobi iteration converges), no DIIS, %d iterations" % (o, v, sigma_b, iters),
-            "fp64_ms_per_iteration": t64, "fp32_ms_per_iteration": t32, "speedup_fp32": t64 / t32,
-            "e_corr_fp64": e64, "abs_dE_per_iteration": de, "max_abs_dE": max(de),
-            "fp64_phase_ms": out["fp64"][2], "fp32_phase_ms": out["fp32"][2]}
+    e64 = out["fp64"][0]
+    t64 = float(np.mean(out["fp64"][1]))
+    res = {"workload": "synthetic RCCSDT iterations nocc=%d nvir=%d, B-factor sigma=%g (weakly correlated, "
+                       "so the Jacobi iteration converges), no DIIS, %d iterations" % (o, v, sigma_b, iters),
+           "fp64_ms_per_iteration": t64, "e_corr_fp64": e64, "fp64_phase_ms": out["fp64"][2]}
+    for name in ("fp32", "bf16x3"):
+        de = [abs(x - y) for x, y in zip(out[name][0], e64)]
+        tm = float(np.mean(out[name][1]))
+        res.update({name + "_ms_per_iteration": tm, "speedup_" + name: t64 / tm,
+                    name + "_abs_dE_per_iteration": de, name + "_max_abs_dE": max(de),
+                    name + "_phase_ms": out[name][2]})
+    return res
 
 
 def q_time(o, v, blksize=4, seed=20261202):

@@ -127,8 +127,8 @@ class RCCSDT:
     level_shift = 0.0
     verbose = 4
     callback = None
-    # "fp64" (the reference's arithmetic) or "fp32": T3 and the triples contractions in FP32 on MFMA,
-    # everything else in FP64
+    # "fp64" (the reference's arithmetic), "fp32" (T3 and the triples contractions in FP32 on MFMA)
+    # or "bf16x3" (FP32 T3, contractions as split-BF16x3 on BF16 MFMA); everything else in FP64
     triples_precision = "fp64"
 
     def __init__(self, mf, comm=None, frozen=None, mo_coeff=None, mo_occ=None):
@@ -227,9 +227,11 @@ class RCCSDT:
     def _engine(self, eris):
         if self.engine is None or self.engine._eris is not eris:
             nocc, nmo = self.nocc, self.nmo
-            t3_dtype = {"fp64": np.float64, "fp32": np.float32}[self.triples_precision]
+            t3_dtype, contract = {"fp64": (np.float64, "native"), "fp32": (np.float32, "native"),
+                                  "bf16x3": (np.float32, "bf16x3")}[self.triples_precision]
             self.engine = RCCSDTEngine(nocc, nmo - nocc, eris.fock, eris.pppp, eris.mo_energy,
-                                       level_shift=self.level_shift, nvir_diis=self.nvir_diis, t3_dtype=t3_dtype)
+                                       level_shift=self.level_shift, nvir_diis=self.nvir_diis, t3_dtype=t3_dtype,
+                                       triples_contract=contract)
             self.engine._eris = eris
         return self.engine
 

@@ -60,10 +60,11 @@ class RCCSDTEngine:
     """Device state of the RCCSDT amplitude iteration for one closed-shell system."""
 
     def __init__(self, nocc, nvir, fock, pppp, mo_energy, level_shift=0.0, nvir_diis=None, stream=None,
-                 device=0, t3_dtype=np.float64):
+                 device=0, t3_dtype=np.float64, triples_contract="native"):
         """``t3_dtype`` float32 selects the mixed-precision iteration: T3, the triples residual and its
         operands in FP32 with FP32-MFMA contractions, everything else (integrals, T1/T2, lower-order
-        residual, denominators, projection arithmetic, norms, energy, DIIS) in FP64."""
+        residual, denominators, projection arithmetic, norms, energy, DIIS) in FP64;
+        ``triples_contract="bf16x3"`` (with float32) runs those contractions as split-BF16x3."""
         o, v = int(nocc), int(nvir)
         n = o + v
         fock = np.asarray(fock, dtype=np.float64)
@@ -74,7 +75,8 @@ class RCCSDTEngine:
         self.stream = stream
         self.level_shift = float(level_shift)
         self.nvir_diis = v if nvir_diis is None else int(nvir_diis)
-        self.tr = tr = TriplesEngine(o, v, stream=stream, device=device, dtype=t3_dtype)
+        self.tr = tr = TriplesEngine(o, v, stream=stream, device=device, dtype=t3_dtype,
+                                     contract=triples_contract)
         self.mixed = tr.dtype != np.float64
         self.ext = {x: o for x in OCC}
         self.ext.update({x: v for x in VIR})

@@ -239,10 +239,11 @@ def s3_jobs_array(jobs):
 class TriplesEngine:
     """Holds the device state of one RCCSDT triples update for (nocc, nvir)."""
 
-    def __init__(self, nocc, nvir, stream=None, device=0, dtype=np.float64):
+    def __init__(self, nocc, nvir, stream=None, device=0, dtype=np.float64, contract="native"):
         """``dtype`` float64 is the plain FP64 path; float32 stores T3, R, Y and the
         intermediates in FP32 and contracts on FP32 MFMA, keeping the denominator,
-        the projection arithmetic and the norms in FP64."""
+        the projection arithmetic and the norms in FP64.  ``contract="bf16x3"`` (FP32 storage
+        only) runs every contraction as split-BF16x3 products on BF16 MFMA instead."""
         device_init(device)
         o, v = int(nocc), int(nvir)
         self.nocc, self.nvir = o, v
@@ -289,9 +290,19 @@ class TriplesEngine:
                 jobs.append(((pq * o + l) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
         self.y_jobs = DeviceArray.from_host(s3_jobs_array(jobs))
         self.n_y_jobs = len(jobs)
+        if contract not in ("native", "bf16x3") or (contract == "bf16x3" and dt != np.float32):
+            raise ValueError("contract must be 'native', or 'bf16x3' with float32 storage")
+        self.contract = contract
         self._tables = []
         self.launches = self._plan_residual()
+        self._retarget(self.launches)
         self._lower = None
+
+    def _retarget(self, launches):
+        if self.contract == "bf16x3":
+            for g in launches:
+                if isinstance(g, GemmLaunch):
+                    g.fn = "dcc_contract_f32_bf16x3"
 
     # ------------------------------------------------------------------ plan
     def _plan_residual(self):
@@ -454,6 +465,7 @@ class TriplesEngine:
                           Operand(Z, 'preba', (o, o, v, v, v)), Operand(lo['Eoovv'], 'rpde', (o, o, v, v)),
                           Operand(self.W_ovvv, 'Qabd', (o, v, v, v)), -1.0, 1.0, fx, ext),
             ]
+            self._retarget(r2_launches + w_launches)
             slices.append((DeviceArray.from_host(s3_jobs_array(jobs)), len(jobs), r2_launches, w_launches))
         lo['slices'] = slices
         self._lower = lo

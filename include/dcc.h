@@ -108,6 +108,19 @@ int dcc_contract_f32(int64_t M, int64_t N, int64_t K, double alpha, double beta,
                      const dcc_batch *batches, int64_t nbatch, int64_t max_rows,
                      dcc_stream_t stream);
 
+/*
+ * dcc_contract_f32 with split-BF16x3 products: each FP32 operand x is split into hi = bf16(x) and
+ * lo = bf16(x - hi) (round to nearest even) and every product is accumulated as
+ * hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 in FP32 (relative product error ~2^-16).
+ * Same arguments and semantics as dcc_contract_f32; the reference has no counterpart (its
+ * contractions are FP64 NumPy einsums, distr_cc/rccsdt.py:368-433) -- this is the compensated
+ * low-precision variant of the triples contractions.
+ */
+int dcc_contract_f32_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
+                            dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn, float *C,
+                            dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
+                            int64_t max_rows, dcc_stream_t stream);
+
 /* ---------------------------------------- S3 block transforms (HBM-bound) */
 
 /*

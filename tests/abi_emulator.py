@@ -13,6 +13,21 @@ import numpy as np
 _PERMS = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
 
 
+def bf16_rne(x):
+    """float32 -> nearest-even bfloat16, returned as float64."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+def bf16_split(x):
+    """x = hi + lo with hi = bf16(x), lo = bf16(x - hi) (the split of dcc_contract_f32_bf16x3)."""
+    x = np.asarray(x, dtype=np.float32)
+    hi = bf16_rne(x)
+    lo = bf16_rne((x.astype(np.float64) - hi).astype(np.float32))
+    return hi, lo
+
+
 class FakeDevice:
     def __init__(self):
         self.allocs = {}  # base -> bytearray-backed numpy uint8
@@ -118,8 +133,14 @@ class Emulator:
     def dcc_contract_f32(self, *a):
         return self._contract(np.float32, *a)
 
+    def dcc_contract_f32_bf16x3(self, *a):
+        return self._contract("bf16x3", *a)
+
     def _contract(self, dt, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
                   stream):
+        split = dt == "bf16x3"
+        if split:
+            dt = np.float32
         dev = self.dev
         Av, Bv, Cv = dev.view(_ptr(A), dt), dev.view(_ptr(B), dt), dev.view(_ptr(C), dt)
         bt = dev.view(_ptr(batches), np.int64, 5 * int(nbatch)).reshape(-1, 5)
@@ -133,7 +154,12 @@ class Emulator:
             ai = a_off + _map(dev, am, ms, b)[:, None] + _map(dev, ak, ks, b)[None, :]
             bi = b_off + _map(dev, bk, ks, b)[:, None] + _map(dev, bn, ns, b)[None, :]
             ci = c_off + _map(dev, cm, ms, b)[:, None] + _map(dev, cn, ns, b)[None, :]
-            prod = (Av[ai].astype(np.float64) @ Bv[bi].astype(np.float64)) if K > 0 else np.zeros(ci.shape)
+            if split and K > 0:
+                ah, al = bf16_split(Av[ai])
+                bh, bl = bf16_split(Bv[bi])
+                prod = ah @ bh + ah @ bl + al @ bh
+            else:
+                prod = (Av[ai].astype(np.float64) @ Bv[bi].astype(np.float64)) if K > 0 else np.zeros(ci.shape)
             old = Cv[ci].astype(np.float64) if beta != 0.0 else 0.0
             Cv[ci] = (alpha * prod + beta * old).astype(dt)
         return 0

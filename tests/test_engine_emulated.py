@@ -186,3 +186,19 @@ def test_mixed_iteration_plan(monkeypatch):
     assert abs(a1 - b1).max() <= 1e-6 * abs(b1).max()
     assert abs(a2 - b2).max() <= 1e-6 * abs(b2).max()
     assert abs(a3 - b3).max() <= 1e-5 * abs(b3).max()
+
+
+def test_bf16x3_residual_plan(monkeypatch):
+    abi_emulator.install(monkeypatch)
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    o, v = 3, 5
+    s = SyntheticSystem(o, v, naux=20, seed=31)
+    im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
+    eng = TriplesEngine(o, v, dtype=np.float32, contract="bf16x3")
+    assert all(g.fn == "dcc_contract_f32_bf16x3" for g in eng.launches if hasattr(g, "args"))
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.residual()
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3)
+    assert abs(eng.R.download().astype(np.float64) - Ro).max() <= 3e-4 * abs(Ro).max()

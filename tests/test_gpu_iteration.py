@@ -166,3 +166,23 @@ def test_mixed_kernel_energy_error(dev):
         assert cc.converged
     _say("E fp64 %.12f fp32 %.12f" % (res["fp64"], res["fp32"]))
     assert abs(res["fp64"] - res["fp32"]) < 1e-6
+
+
+def test_bf16x3_kernel_energy_error(dev):
+    """Converged E_corr with split-BF16x3 triples contractions against the FP64 path."""
+    from ccsdt_cases import DuckMF, molecule_like
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 4, 10
+    _say("bf16x3 kernel o=%d v=%d" % (o, v))
+    F, g, e, _ = molecule_like(o, v, 51)
+    res = {}
+    for prec in ("fp64", "bf16x3"):
+        cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+        cc.verbose, cc.triples_precision = 0, prec
+        if prec == "fp64":
+            cc.conv_tol, cc.conv_tol_normt = 1e-10, 1e-8
+        res[prec] = cc.kernel(eris=_PhysicistsERIs(F, e, g))[0]
+        _say("%s converged=%s cycles=%d E=%.12f" % (prec, cc.converged, cc.cycles, res[prec]))
+        assert cc.converged
+    assert abs(res["fp64"] - res["bf16x3"]) < 1e-6

@@ -210,3 +210,48 @@ def test_residual_fp32_vs_oracle(dev, o, v):
     t3o, _, n2o, _ = tri.t3_update(o, Ro, t3, s.mo_energy)
     assert abs(eng.get_t3() - t3o).max() <= 2e-5 * abs(t3o).max()
     assert abs(n2 - n2o) <= 1e-4 * n2o
+
+
+@pytest.mark.parametrize("M,N,K", [(16, 16, 32), (37, 53, 29), (130, 70, 300), (64, 64, 64)])
+def test_contract_bf16x3_plain_gemm(dev, M, N, K):
+    """Split-BF16x3 path: pins the v_mfma_f32_16x16x32_bf16 operand/accumulator layouts and the split
+    (against the same split evaluated exactly on the host), and its error against FP64."""
+    from abi_emulator import bf16_split
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    _say("contract bf16x3 %dx%dx%d" % (M, N, K))
+    rng = np.random.default_rng(M * N + K + 1)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    C0 = rng.normal(size=(M, N)).astype(np.float32)
+    dA, dB, dC = DeviceArray.from_host(A), DeviceArray.from_host(B), DeviceArray.from_host(C0)
+    bt = np.zeros(1, dtype=L.BATCH_DTYPE)
+    bt[0] = (0, 0, 0, 0, M)
+    dbt = DeviceArray.from_host(bt)
+    lin = lambda s: L.IdxMap(None, 0, 1, 1 << 20, 0, 0, s)
+    L.call("dcc_contract_f32_bf16x3", M, N, K, 1.5, 0.5, dA.vp, lin(K), lin(1), dB.vp, lin(N), lin(1),
+           dC.vp, lin(N), lin(1), dbt.vp, 1, M, None)
+    got = dC.download().astype(np.float64)
+    ah, al = bf16_split(A)
+    bh, bl = bf16_split(B)
+    split_ref = 0.5 * C0.astype(np.float64) + 1.5 * (ah @ bh + ah @ bl + al @ bh)
+    exact = 0.5 * C0.astype(np.float64) + 1.5 * (A.astype(np.float64) @ B.astype(np.float64))
+    scale = max(1.0, abs(exact).max())
+    assert abs(got - split_ref).max() < 3e-6 * scale * np.sqrt(K)     # FP32 accumulation only
+    assert abs(got - exact).max() < 1e-4 * scale * np.sqrt(K)         # split error ~2^-16 per product
+
+
+@pytest.mark.parametrize("o,v", [(3, 9), (5, 17)])
+def test_residual_bf16x3_vs_oracle(dev, o, v):
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    from oracle import rccsdt_tri as tri
+    _say("residual bf16x3 o=%d v=%d" % (o, v))
+    s = SyntheticSystem(o, v, naux=30, seed=700 + o * v)
+    im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
+    eng = TriplesEngine(o, v, dtype=np.float32, contract="bf16x3")
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.residual()
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3)
+    assert abs(eng.R.download().astype(np.float64) - Ro).max() <= 3e-4 * abs(Ro).max()
