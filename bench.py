@@ -166,6 +166,40 @@ def q_time(o, v, blksize=4, seed=20261202):
             "note": "includes host->device copies of the integral slices (PCIe) done inside the call"}
 
 
+def adjoint_cost(o, v, reps=3, seed=20261202):
+    """Forward T2->T3 residual vs its hand-written VJP (distr_cc.adjoint) on the synthetic system:
+    mean device time of each, their ratio, and the device memory the adjoint engine holds."""
+    from distr_cc.adjoint import T2T3Adjoint
+    from distr_cc.device import Event, Stream, mem_get_info
+    from distr_cc.synthetic import SyntheticSystem
+    sysm = SyntheticSystem(o, v, naux=300, seed=seed)
+    im = sysm.bare_intermediates()
+    st = Stream()
+    free0, _ = mem_get_info()
+    adj = T2T3Adjoint(o, v, stream=st)
+    adj.set_inputs(sysm.t2(), im["W_vvvo"], im["W_vooo"])
+    adj.lam.upload(np.ascontiguousarray(sysm.t3_tri()), st)
+    free1, _ = mem_get_info()
+
+    def timed(fn):
+        fn()
+        st.synchronize()
+        out = []
+        for _ in range(reps):
+            a, b = Event(), Event()
+            a.record(st)
+            fn()
+            b.record(st)
+            out.append(a.elapsed_ms(b))
+        return float(np.mean(out))
+
+    fwd, bwd = timed(adj.forward), timed(adj.backward)
+    return {"workload": "T2->T3 residual and its VJP (dt2, dW_vvvo, dW_vooo), nocc=%d nvir=%d" % (o, v),
+            "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd,
+            "adjoint_engine_bytes": int(free0 - free1),
+            "dense_adjoint_bytes": int(o ** 3 * v ** 3 * 8)}
+
+
 def calibrate_copy(mib):
     """dst = src over ``mib`` MiB of FP64 with dcc_tensor_axpby_f64 (reads and writes mib MiB each)."""
     import ctypes
@@ -195,6 +229,7 @@ def main():
     ap.add_argument("--breakdown", action="store_true", help="print per-launch timings to stderr")
     ap.add_argument("--mixed-iters", type=int, default=3,
                     help="full mixed-vs-FP64 iterations to time and compare (0 = skip)")
+    ap.add_argument("--adjoint", type=int, default=1, help="time the T2->T3 VJP (0 = skip)")
     ap.add_argument("--q-nocc", type=int, default=5)
     ap.add_argument("--q-nvir", type=int, default=19, help="(Q) timing size (0 = skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -302,6 +337,9 @@ def main():
         print("bench: mixed vs FP64 full iterations", file=sys.stderr, flush=True)
         del eng
         out["config"]["mixed_vs_fp64"] = mixed_vs_fp64(o, v, args.mixed_iters)
+    if args.adjoint:
+        print("bench: adjoint cost", file=sys.stderr, flush=True)
+        out["config"]["adjoint"] = adjoint_cost(o, v)
     if args.q_nvir > 0:
         print("bench: (Q) timing", file=sys.stderr, flush=True)
         out["config"]["q"] = q_time(args.q_nocc, args.q_nvir)

@@ -104,6 +104,15 @@ int dcc_fill_f32(float *dst, int64_t n, float value, dcc_stream_t stream) {
     return 0;
 }
 
+int dcc_mem_get_info(int64_t *free_bytes, int64_t *total_bytes) {
+    DCC_CHECK(free_bytes != nullptr && total_bytes != nullptr, "null pointer");
+    size_t f = 0, t = 0;
+    DCC_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+    return 0;
+}
+
 int dcc_stream_create(dcc_stream_t *stream) {
     DCC_CHECK(stream != nullptr, "null stream");
     hipStream_t s;

@@ -53,6 +53,7 @@ _SIGS = {
     "dcc_last_error": ([], ctypes.c_char_p),
     "dcc_device_count": ([ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
     "dcc_set_device": ([ctypes.c_int, ctypes.c_char_p, c_i64], ctypes.c_int),
+    "dcc_mem_get_info": ([ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)], ctypes.c_int),
     "dcc_malloc": ([ctypes.POINTER(c_vp), c_i64], ctypes.c_int),
     "dcc_free": ([c_vp], ctypes.c_int),
     "dcc_memcpy_htod": ([c_vp, c_vp, c_i64, c_vp], ctypes.c_int),

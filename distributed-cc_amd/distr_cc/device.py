@@ -14,6 +14,13 @@ def device_count():
     return n.value
 
 
+def mem_get_info():
+    """(free, total) device memory in bytes."""
+    f, t = ctypes.c_int64(), ctypes.c_int64()
+    L.call("dcc_mem_get_info", ctypes.byref(f), ctypes.byref(t))
+    return int(f.value), int(t.value)
+
+
 def init(device=0):
     """Select ``device`` for this thread; returns its gfx arch name."""
     if device in _DEVICE_READY:

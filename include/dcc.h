@@ -41,6 +41,9 @@ int dcc_device_count(int *count);
 /* Select the device for this thread and report its gfx arch name (>= 32 bytes). */
 int dcc_set_device(int device, char *arch_name, int64_t arch_len);
 
+/* Free and total device memory of the current device (bytes). */
+int dcc_mem_get_info(int64_t *free_bytes, int64_t *total_bytes);
+
 int dcc_malloc(void **ptr, int64_t bytes);
 int dcc_free(void *ptr);
 int dcc_memcpy_htod(void *dst, const void *src, int64_t bytes, dcc_stream_t stream);
