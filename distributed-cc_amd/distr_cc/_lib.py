@@ -42,6 +42,14 @@ class Strided(ctypes.Structure):
     _fields_ = [("ndim", c_i64), ("shape", c_i64 * 6), ("dst_stride", c_i64 * 6), ("src_stride", c_i64 * 6)]
 
 
+class GemmProblem(ctypes.Structure):
+    """dcc_gemm_problem: one contraction of a grouped launch."""
+    _fields_ = [("N", c_i64), ("K", c_i64), ("alpha", c_dbl), ("beta", c_dbl), ("A", c_vp), ("B", c_vp),
+                ("C", c_vp), ("am", IdxMap), ("ak", IdxMap), ("bk", IdxMap), ("bn", IdxMap), ("cm", IdxMap),
+                ("cn", IdxMap), ("batches", c_vp), ("nbatch", c_i64), ("max_rows", c_i64), ("tiles_m", c_i64),
+                ("tiles_n", c_i64), ("tile_begin", c_i64), ("flags", c_i64)]
+
+
 BATCH_DTYPE = np.dtype([("a_off", np.int64), ("b_off", np.int64), ("c_off", np.int64),
                         ("m_begin", np.int64), ("m_end", np.int64)])
 S3JOB_DTYPE = np.dtype([("dst", np.int64), ("src", np.int64), ("coef", np.float64, (6,))])
@@ -94,6 +102,12 @@ _SIGS = {
     "dcc_tensor_axpby_f32": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f32_f64": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f64_f32": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
+    "dcc_grouped_prepare": ([c_vp, c_i64, c_i64, ctypes.POINTER(c_i64)], ctypes.c_int),
+    "dcc_contract_grouped_f64": ([c_vp, c_i64, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
+    "dcc_graph_capture_begin": ([c_vp], ctypes.c_int),
+    "dcc_graph_capture_end": ([c_vp, ctypes.POINTER(c_vp)], ctypes.c_int),
+    "dcc_graph_launch": ([c_vp, c_vp], ctypes.c_int),
+    "dcc_graph_destroy": ([c_vp], ctypes.c_int),
     "dcc_dot_partials": ([], c_i64),
     "dcc_dot_f64": ([c_vp, c_vp, c_i64, c_vp, c_vp, c_vp], ctypes.c_int),
     "dcc_dot_f32": ([c_vp, c_vp, c_i64, c_vp, c_vp, c_vp], ctypes.c_int),

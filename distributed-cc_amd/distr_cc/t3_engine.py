@@ -603,19 +603,30 @@ class TriplesEngine:
             if timer is not None:
                 timer(g, 'end')
 
-    def finalize(self, mo_energy, level_shift=0.0, nvir_diis=None, r3_out=None):
-        """rccsdt.py:947-971 fused: t3 += P3_full-projected R / D; returns (norm^2, active norm^2)."""
+    def set_denominators(self, mo_energy, level_shift=0.0, nvir_diis=None):
+        """Upload eia = e_i - e_a - shift (rccsdt.py:737) and fix the active virtual count."""
         o, v = self.nocc, self.nvir
         e = np.asarray(mo_energy, dtype=np.float64)
         eia = e[:o, None] - e[None, o:] - level_shift
         self.eia.upload(np.ascontiguousarray(eia), self.stream)
-        nd = v if nvir_diis is None else int(nvir_diis)
+        self.nvir_diis = v if nvir_diis is None else int(nvir_diis)
+
+    def finalize_launch(self, r3_out=None):
+        """Issue the fused projection/denominator/update and the norm reduction (no host sync)."""
         s = self.stream.handle if self.stream is not None else None
         L.call("dcc_r3_finalize_" + self.sfx, self.R.vp, self.t3.vp, r3_out.vp if r3_out is not None else None,
-               self.n3, v, self.blk_ijk.vp, self.eia.vp, o, nd, self.partials.vp, s)
+               self.n3, self.nvir, self.blk_ijk.vp, self.eia.vp, self.nocc, self.nvir_diis, self.partials.vp, s)
         L.call("dcc_reduce_pairs_f64", self.partials.vp, self.n3 * self.norb, self.norms.vp, s)
+
+    def read_norms(self):
         n = self.norms.download(stream=self.stream)
         return float(n[0]), float(n[1])
+
+    def finalize(self, mo_energy, level_shift=0.0, nvir_diis=None, r3_out=None):
+        """rccsdt.py:947-971 fused: t3 += P3_full-projected R / D; returns (norm^2, active norm^2)."""
+        self.set_denominators(mo_energy, level_shift, nvir_diis)
+        self.finalize_launch(r3_out)
+        return self.read_norms()
 
     def residual_flops(self):
         return sum(g.flops for g in self.launches)

@@ -271,6 +271,43 @@ int dcc_dot_f32(const float *x, const float *y, int64_t n, double *partials, dou
 int dcc_denom_divide_f64(double *x, int64_t nocc, int64_t nvir, int64_t rank, const double *eia,
                          dcc_stream_t stream);
 
+/* ------------------------------------------ batched ensembles of small systems */
+
+/*
+ * One contraction of a grouped launch: the arguments of dcc_contract_f64 in descriptor form.
+ * tiles_m, tiles_n, tile_begin and flags are filled by dcc_grouped_prepare.  The reference has no
+ * counterpart (it runs one system per MPI job); these entry points pack many small RCCSDT systems
+ * (uneven nocc/nvir) into one persistent launch per residual term.
+ */
+typedef struct dcc_gemm_problem {
+    int64_t N, K;
+    double alpha, beta;
+    const void *A;
+    const void *B;
+    void *C;
+    dcc_idxmap am, ak, bk, bn, cm, cn;
+    const dcc_batch *batches;
+    int64_t nbatch, max_rows;
+    int64_t tiles_m, tiles_n, tile_begin, flags;
+} dcc_gemm_problem;
+
+typedef void *dcc_graph_t;
+
+/* Host side: tile every problem for tile shape 0 (64x64), 1 (128x64), 2 (64x128) or 3 (128x128),
+ * fill the tiling fields and return the total tile count. */
+int dcc_grouped_prepare(dcc_gemm_problem *probs, int64_t nprob, int64_t tile_shape, int64_t *total_tiles);
+
+/* Persistent grouped FP64 gather-GEMM over a device array of prepared problems whose C regions are
+ * disjoint; nworkgroups = 0 picks 4 per CU.  Every workgroup exits when the tile list is exhausted. */
+int dcc_contract_grouped_f64(const dcc_gemm_problem *probs, int64_t nprob, int64_t total_tiles,
+                             int64_t tile_shape, int64_t nworkgroups, dcc_stream_t stream);
+
+/* HIP graph capture of everything issued on a (non-default) stream, and its replay. */
+int dcc_graph_capture_begin(dcc_stream_t stream);
+int dcc_graph_capture_end(dcc_stream_t stream, dcc_graph_t *graph);
+int dcc_graph_launch(dcc_graph_t graph, dcc_stream_t stream);
+int dcc_graph_destroy(dcc_graph_t graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -82,7 +82,57 @@ class Emulator:
     # each handler returns a status
     def __call__(self, name, *args):
         self.calls.append(name)
+        rec = getattr(self, "_recording", None)
+        if rec is not None and name not in ("dcc_graph_capture_end", "dcc_graph_capture_begin"):
+            rec.append((name, args))
+            return 0
         return getattr(self, name)(*args)
+
+    def dcc_graph_capture_begin(self, stream):
+        self._recording = []
+        return 0
+
+    def dcc_graph_capture_end(self, stream, pgraph):
+        self._graphs = getattr(self, "_graphs", {})
+        gid = len(self._graphs) + 1
+        self._graphs[gid] = self._recording
+        self._recording = None
+        pgraph._obj.value = gid
+        return 0
+
+    def dcc_graph_launch(self, graph, stream):
+        for name, args in self._graphs[_ptr(graph)]:
+            st = getattr(self, name)(*args)
+            if st:
+                return st
+        return 0
+
+    def dcc_graph_destroy(self, graph):
+        return 0
+
+    def dcc_grouped_prepare(self, probs, nprob, tile, ptotal):
+        from distr_cc import _lib as L
+        bm, bn = {0: (64, 64), 1: (128, 64), 2: (64, 128), 3: (128, 128)}[int(tile)]
+        arr = (L.GemmProblem * int(nprob)).from_address(_ptr(probs))
+        acc = 0
+        for p in arr:
+            p.tiles_m = (p.max_rows + bm - 1) // bm
+            p.tiles_n = (p.N + bn - 1) // bn
+            p.tile_begin = acc
+            acc += p.nbatch * p.tiles_m * p.tiles_n
+        ptotal._obj.value = acc
+        return 0
+
+    def dcc_contract_grouped_f64(self, probs, nprob, total, tile, nwg, stream):
+        from distr_cc import _lib as L
+        raw = self.dev.view(_ptr(probs), np.uint8, int(nprob) * ctypes.sizeof(L.GemmProblem))
+        arr = (L.GemmProblem * int(nprob)).from_buffer_copy(raw.tobytes())
+        for p in arr:
+            st = self._contract(np.float64, 0, p.N, p.K, p.alpha, p.beta, p.A, p.am, p.ak, p.B, p.bk, p.bn,
+                                p.C, p.cm, p.cn, p.batches, p.nbatch, p.max_rows, stream)
+            if st:
+                return st
+        return 0
 
     def dcc_device_count(self, count):
         count._obj.value = 1

@@ -1,5 +1,7 @@
 """Host-side planning of the GPU engine (index maps, batch tables, S3 jobs) checked
 against the oracle through the NumPy emulator of the C ABI (no GPU needed)."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -202,3 +204,31 @@ def test_bf16x3_residual_plan(monkeypatch):
     eng.residual()
     Ro = tri.compute_r3_tri(o, v, im, t2, t3)
     assert abs(eng.R.download().astype(np.float64) - Ro).max() <= 3e-4 * abs(Ro).max()
+
+
+def test_batched_ensemble_plan(monkeypatch):
+    """Grouped launches + graph replay == per-system engines, for an uneven ensemble."""
+    abi_emulator.install(monkeypatch)
+    from distr_cc.batched import BatchedTriples
+    from distr_cc.device import Stream
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    systems = [(2, 4), (3, 5), (1, 3), (3, 3)]
+    st = Stream.__new__(Stream)
+    st.handle, st._own = ctypes.c_void_p(7), False
+    bt = BatchedTriples(systems, st)
+    refs = []
+    for n, (o, v) in enumerate(systems):
+        s = SyntheticSystem(o, v, naux=15, seed=50 + n)
+        im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
+        bt.set_system(n, t3, im, t2, s.mo_energy)
+        e = TriplesEngine(o, v)
+        e.set_t3(t3)
+        e.set_intermediates(im, t2)
+        e.residual()
+        refs.append((e.finalize(s.mo_energy), e.get_t3()))
+    bt.capture()
+    bt.step()
+    for e, (nrm, t3ref) in zip(bt.engines, refs):
+        assert abs(e.get_t3() - t3ref).max() <= 1e-13 * abs(t3ref).max()
+        assert abs(e.read_norms()[0] - nrm[0]) <= 1e-12 * nrm[0]
