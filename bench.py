@@ -286,7 +286,8 @@ def main():
               "(sharding over canonical blocks is described in DESIGN.md)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
-    from distr_cc.device import Event, Stream, init
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray, Event, Stream, init
     from distr_cc.synthetic import SyntheticSystem
     from distr_cc.t3_engine import TriplesEngine
 
@@ -303,7 +304,37 @@ def main():
     eng = TriplesEngine(o, v, stream=stream)
     eng.set_t3(t3)
     eng.set_intermediates(im, t2)
+    nmo = o + v
+    # T1 = 0 for the synthetic system, so the T1-dressed integrals are the bare ones
+    O, V = slice(0, o), slice(o, nmo)
+    eng.set_t1_dressed(sysm.fock[:o, o:], sysm.eri_phys(O, O, V, V), sysm.eri_phys(O, O, O, V),
+                       sysm.eri_phys(V, O, V, V))
+    w_saved = [DeviceArray(w.shape, w.dtype) for w in (eng.W_oovo, eng.W_ovvv)]
+    for dst, src in zip(w_saved, (eng.W_oovo, eng.W_ovvv)):
+        L.call("dcc_memcpy_dtod", dst.vp, src.vp, src.nbytes, stream.handle)
     stream.synchronize()
+    phase_ev = []
+
+    def step(timer=None):
+        """A2 + A3 + A4-A6 + A7-A9: T3 -> r1/r2, T3 -> W (on W rebuilt from T2), residual, update."""
+        for src, dst in zip(w_saved, (eng.W_oovo, eng.W_ovvv)):
+            L.call("dcc_memcpy_dtod", dst.vp, src.vp, src.nbytes, stream.handle)
+        if timer is not None:
+            e = Event()
+            e.record(stream)
+            phase_ev.append(("start", e))
+        eng.t3_to_r1r2(download=False)
+        if timer is not None:
+            e = Event()
+            e.record(stream)
+            phase_ev.append(("A2", e))
+        eng.t3_to_w(reuse_z=True)
+        if timer is not None:
+            e = Event()
+            e.record(stream)
+            phase_ev.append(("A3", e))
+        eng.residual(timer=timer)
+        return eng.finalize(sysm.mo_energy)
 
     # dominant kernel: the three W_vvvv x T3 GEMMs (65% of the algorithmic flops)
     dom = [g for g in eng.launches if g.name.startswith("W_vvvv")]
@@ -317,8 +348,7 @@ def main():
 
     print("bench: warmup %d" % args.warmup, file=sys.stderr, flush=True)
     for _ in range(args.warmup):
-        eng.residual()
-        eng.finalize(sysm.mo_energy)
+        step()
     stream.synchronize()
     ev.clear()
     e0, e1 = Event(), Event()
@@ -327,8 +357,7 @@ def main():
     e0.record(stream)
     norms = None
     for _ in range(args.steps):
-        eng.residual(timer=timer)
-        norms = eng.finalize(sysm.mo_energy)
+        norms = step(timer=timer)
     e1.record(stream)
     stream.synchronize()
     wall = time.perf_counter() - w0
@@ -352,6 +381,9 @@ def main():
             tot += d
             print("  %-36s %9.3f ms  %7.2f TF/s" % (g.name, d, g.flops / (d * 1e-3) / 1e12), file=sys.stderr)
         print("  sum of GEMM launches %.3f ms of %.3f ms per step" % (tot, ms), file=sys.stderr)
+        a2 = [b.elapsed_ms(c) for (w1, b), (w2, c) in zip(phase_ev[0::3], phase_ev[1::3])]
+        a3 = [b.elapsed_ms(c) for (w1, b), (w2, c) in zip(phase_ev[1::3], phase_ev[2::3])]
+        print("  T3->r1/r2 (A2) %.3f ms, T3->W (A3) %.3f ms" % (np.mean(a2), np.mean(a3)), file=sys.stderr)
 
     F = algorithmic_flops(o, v)
     out = {
@@ -367,8 +399,9 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "plain-FP64 RCCSDT T3 residual + projection/denominator/update, "
-                               "nocc=%d nvir=%d, %d canonical blocks" % (o, v, eng.n3),
+        "config": {"workload": "plain-FP64 RCCSDT triples update: T3->r1/r2, T3->W, T3 residual, "
+                               "projection/denominator/update, nocc=%d nvir=%d, %d canonical blocks"
+                               % (o, v, eng.n3),
                    "nocc": o, "nvir": v, "naux": 300, "seed": 20261202, "precision": "fp64",
                    "residual_tflops": F / (ms * 1e-3) / 1e12,
                    "residual_frac_fp64_peak": F / (ms * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,

@@ -77,6 +77,18 @@ __global__ __launch_bounds__(256) void denom_kernel(T *__restrict__ x, int64_t n
     }
 }
 
+// dst[i] = beta * dst[i] + alpha * sum_s src[s * n + i], s in fixed order
+template <typename T>
+__global__ __launch_bounds__(256) void sum_slices_kernel(T *__restrict__ dst, const T *__restrict__ src,
+                                                         int64_t nslices, int64_t n, double alpha, double beta) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        double acc = 0.0;
+        for (int64_t s = 0; s < nslices; ++s) acc += (double)src[s * n + i];
+        dst[i] = (T)((beta == 0.0 ? 0.0 : beta * (double)dst[i]) + alpha * acc);
+    }
+}
+
 __global__ __launch_bounds__(256) void fill_zero_kernel(double *__restrict__ p) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < kDotBlocks) p[i] = 0.0;
@@ -137,11 +149,32 @@ int denom_divide(T *x, int64_t nocc, int64_t nvir, int64_t rank, const double *e
     return 0;
 }
 
+template <typename T>
+int sum_slices(T *dst, const T *src, int64_t nslices, int64_t n, double alpha, double beta, dcc_stream_t stream) {
+    DCC_CHECK(nslices >= 0 && n >= 0, "negative extent");
+    if (n == 0) return 0;
+    DCC_CHECK(dst != nullptr && (nslices == 0 || src != nullptr), "null pointer");
+    hipLaunchKernelGGL(sum_slices_kernel<T>, dim3(grid_for(n)), dim3(256), 0, dcc::as_stream(stream), dst, src,
+                       nslices, n, alpha, beta);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
 
 int64_t dcc_dot_partials(void) { return kDotBlocks; }
+
+int dcc_sum_slices_f64(double *dst, const double *src, int64_t nslices, int64_t n, double alpha, double beta,
+                       dcc_stream_t stream) {
+    return sum_slices<double>(dst, src, nslices, n, alpha, beta, stream);
+}
+
+int dcc_sum_slices_f32(float *dst, const float *src, int64_t nslices, int64_t n, double alpha, double beta,
+                       dcc_stream_t stream) {
+    return sum_slices<float>(dst, src, nslices, n, alpha, beta, stream);
+}
 
 int dcc_tensor_axpby_f64(double *dst, const double *src, const dcc_strided *layout, double alpha, double beta,
                          dcc_stream_t stream) {

@@ -108,6 +108,8 @@ _SIGS = {
     "dcc_graph_capture_end": ([c_vp, ctypes.POINTER(c_vp)], ctypes.c_int),
     "dcc_graph_launch": ([c_vp, c_vp], ctypes.c_int),
     "dcc_graph_destroy": ([c_vp], ctypes.c_int),
+    "dcc_sum_slices_f64": ([c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp], ctypes.c_int),
+    "dcc_sum_slices_f32": ([c_vp, c_vp, c_i64, c_i64, c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_dot_partials": ([], c_i64),
     "dcc_dot_f64": ([c_vp, c_vp, c_i64, c_vp, c_vp, c_vp], ctypes.c_int),
     "dcc_dot_f32": ([c_vp, c_vp, c_i64, c_vp, c_vp, c_vp], ctypes.c_int),

@@ -321,7 +321,7 @@ class RCCSDTEngine:
                   ("lower r1/r2", lambda: self.prog["r1r2"].run(st)),
                   ("T3->r1/r2 + update t1/t2", lambda: self.prog["update12"].run(st)),
                   ("triples intermediates", lambda: self.prog["imds3"].run(st)),
-                  ("T3->W", self.tr.t3_to_w),
+                  ("T3->W", lambda: self.tr.t3_to_w(reuse_z=True)),
                   ("ring/oooo operands", self.tr.prepare_residual),
                   ("triples residual", self.tr.residual)]
         for name, fn in phases:

@@ -390,7 +390,9 @@ class TriplesEngine:
         for part in (3, 4, 5):
             S = place[part]
             for wname, mx, (n0, n1), sd, sn0, sn1 in terms:
-                if S[n0] < S[n1]:    # innermost N index = the one contiguous in R (coalesced C)
+                # innermost N index = the one contiguous in R (coalesced C), unless that breaks the
+                # coalescing of the B (Y) loads, which are re-read once per M tile and cost more
+                if S[n0] < S[n1] and (sd == 1 or sn0 == 1):
                     n0, n1, sn0, sn1 = n1, n0, sn1, sn0
                 # per-pair row table: cm[pair][(r,x)] = target(r)*v^3 + x*S[mx]
                 tbl = np.zeros((self.npairs, ov), dtype=np.int64)
@@ -418,56 +420,80 @@ class TriplesEngine:
 
 
     # ------------------------------------------------- T3 -> r1/r2 and T3 -> W
-    def _setup_lower(self):
+    def _setup_lower(self, z_budget_bytes=16 << 30):
+        """Plans of T3 -> r1/r2 (rccsdt.py:253-321) and T3 -> W (rccsdt.py:323-366).
+
+        Z[Q][p][r] = P3_201(T[p, q0+Q, r]) is built for a chunk of middle indices q at a time (as many
+        as ``z_budget_bytes`` allows; all of them at nocc=10, nvir=90), so every contraction folds the
+        chunk into its M dimension.  Contractions whose M x N would leave the chip idle (M or N of one
+        occupied or virtual index against K = o v^2) are split along K: the peeled K letters become
+        batch entries writing partial slices, summed in a fixed order by dcc_sum_slices."""
         o, v = self.nocc, self.nvir
         v3 = v ** 3
-        lo = {}
-        lo['Z'] = DeviceArray((o, o, v, v, v), self.dtype)
-        lo['Ecomb'] = DeviceArray((o, o, o, v), self.dtype)      # [l,r,p,a]
-        lo['Evovv'] = DeviceArray((v, o, v, v), self.dtype)      # [c,p,d,a] = E_vovv[c,p,a,d]
-        lo['Eoovv'] = DeviceArray((o, o, v, v), self.dtype)      # [r,p,d,e]
-        lo['Eoovv_t'] = DeviceArray((o, o, v, v), self.dtype)    # [m,p,e,d] = E_oovv[m,p,d,e]
-        lo['Er1'] = DeviceArray((o, o, v, v, 1), self.dtype)     # [j,k,b,c] = E - 1/2 E^(bc)
-        lo['r1'] = DeviceArray((o, v, 1), self.dtype)
-        lo['r2'] = DeviceArray((o, o, v, v), self.dtype)
+        dt = self.dtype
+        qc = int(max(1, min(o, z_budget_bytes // max(1, o * o * v3 * dt.itemsize))))
+        lo = {'qc': qc}
+        lo['Z'] = DeviceArray((qc, o, o, v, v, v), dt)
+        lo['Ecomb'] = DeviceArray((o, o, o, v), dt)      # [l,r,p,a]
+        lo['Evovv'] = DeviceArray((v, o, v, v), dt)      # [c,p,d,a] = E_vovv[c,p,a,d]
+        lo['Eoovv'] = DeviceArray((o, o, v, v), dt)      # [r,p,d,e]
+        lo['Eoovv_t'] = DeviceArray((o, o, v, v), dt)    # [m,p,e,d] = E_oovv[m,p,d,e]
+        lo['Er1'] = DeviceArray((o, o, v, v, 1), dt)     # [j,k,b,c] = E - 1/2 E^(bc)
+        lo['r1'] = DeviceArray((o, v, 1), dt)
+        lo['r2'] = DeviceArray((o, o, v, v), dt)
+        lo['P2'] = DeviceArray((o, qc, o, v, v), dt)     # split-K partials of r2 (E_vovv term)
+        lo['P1'] = DeviceArray((qc * o, o, v, 1), dt)    # split-K partials of r1
+        lo['P3'] = DeviceArray((o, qc, o, v, o), dt)     # split-K partials of W_vooo
         ext = {x: v for x in 'abcdexyz'}
-        ext.update({'p': o, 'r': o, 'l': o, 'm': o, 'i': o, 'k': o, 'Q': o, 'X': 1})
-        Z = lo['Z']
-        slices = []
-        for q in range(o):
+        ext.update({'p': o, 'r': o, 'l': o, 'm': o, 'i': o, 'k': o, 'Q': qc, 'X': 1})
+        Zs = (qc, o, o, v, v, v)
+        Z = lambda letters: Operand(lo['Z'], letters, Zs)
+        chunks = []
+        for q0 in range(0, o, qc):
+            nq = min(qc, o - q0)
+            ext_c = dict(ext, Q=nq)
             jobs = []
-            for p in range(o):
-                for r in range(o):
-                    case, (ci, cj, ck) = canonical_case(p, q, r)
-                    P = TP_T3[case]
-                    coef = np.zeros(6)
-                    for c, Qp in ((2.0, (0, 1, 2)), (-1.0, (2, 1, 0)), (-1.0, (1, 0, 2))):
-                        comp = (P[Qp[0]], P[Qp[1]], P[Qp[2]])
-                        coef[TP_T3.index(comp)] += c
-                    jobs.append(((p * o + r) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
-            fx = [dict(Q=q)]
-            r2_launches = [
-                plan_term('t3->r2 E_vovv q%d' % q, 'cpda,prdba->rbc',
-                          Operand(lo['Evovv'], 'cpda', (v, o, v, v)), Operand(Z, 'prdba', (o, o, v, v, v)),
-                          Operand(lo['r2'], 'Qrbc', (o, o, v, v)), 1.0, 1.0, fx, ext),
-                plan_term('t3->r2 E_ooov q%d' % q, 'lrpa,prabc->lbc',
-                          Operand(lo['Ecomb'], 'lrpa', (o, o, o, v)), Operand(Z, 'prabc', (o, o, v, v, v)),
-                          Operand(lo['r2'], 'Qlbc', (o, o, v, v)), 1.0, 1.0, fx, ext),
-                plan_term('t3->r1 q%d' % q, 'ikabc,kbcX->iaX',
-                          Operand(Z, 'ikabc', (o, o, v, v, v)), Operand(lo['Er1'], 'QkbcX', (o, o, v, v, 1)),
-                          Operand(lo['r1'], 'iaX', (o, v, 1)), 1.0, 1.0, fx, ext),
-            ]
-            w_launches = [
-                plan_term('t3->W_vooo q%d' % q, 'pread,mped->ram',
-                          Operand(Z, 'pread', (o, o, v, v, v)), Operand(lo['Eoovv_t'], 'mped', (o, o, v, v)),
-                          Operand(self.W_oovo, 'Qram', (o, o, v, o)), 1.0, 1.0, fx, ext),
-                plan_term('t3->W_vvvo q%d' % q, 'preba,rpde->abd',
-                          Operand(Z, 'preba', (o, o, v, v, v)), Operand(lo['Eoovv'], 'rpde', (o, o, v, v)),
-                          Operand(self.W_ovvv, 'Qabd', (o, v, v, v)), -1.0, 1.0, fx, ext),
-            ]
-            self._retarget(r2_launches + w_launches)
-            slices.append((DeviceArray.from_host(s3_jobs_array(jobs)), len(jobs), r2_launches, w_launches))
-        lo['slices'] = slices
+            for Q in range(nq):
+                for p in range(o):
+                    for r in range(o):
+                        case, (ci, cj, ck) = canonical_case(p, q0 + Q, r)
+                        P = TP_T3[case]
+                        coef = np.zeros(6)
+                        for c, Qp in ((2.0, (0, 1, 2)), (-1.0, (2, 1, 0)), (-1.0, (1, 0, 2))):
+                            comp = (P[Qp[0]], P[Qp[1]], P[Qp[2]])
+                            coef[TP_T3.index(comp)] += c
+                        jobs.append((((Q * o + p) * o + r) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
+            # --- T3 -> r2, r1 ---
+            a2 = []
+            e_split = [({'p': p}, {'p': p}, {'S': p}) for p in range(o)]
+            a2.append(plan_einsum('t3->r2 E_vovv q%d+%d' % (q0, nq), 'Qrdba,cda->Qrbc', Z('Qprdba'),
+                                  Operand(lo['Evovv'], 'cpda', (v, o, v, v)),
+                                  Operand(lo['P2'], 'SQrbc', (o, nq, o, v, v)), 1.0, 0.0, ext_c, e_split))
+            a2.append(('sum', lo['r2'], q0 * o * v * v, lo['P2'], o, nq * o * v * v, 1.0))
+            a2.append(plan_einsum('t3->r2 E_ooov q%d+%d' % (q0, nq), 'Qprabc,lrpa->Qlbc', Z('Qprabc'),
+                                  Operand(lo['Ecomb'], 'lrpa', (o, o, o, v)),
+                                  Operand(lo['r2'], 'Qlbc', (o, o, v, v)), 1.0, 1.0, ext_c,
+                                  [({}, {}, {'Q': q0})]))
+            r1_split = [({'Q': Q, 'k': k}, {'Q': q0 + Q, 'k': k}, {'S': Q * o + k})
+                        for Q in range(nq) for k in range(o)]
+            a2.append(plan_einsum('t3->r1 q%d+%d' % (q0, nq), 'iabc,bcX->iaX', Z('Qikabc'),
+                                  Operand(lo['Er1'], 'QkbcX', (o, o, v, v, 1)),
+                                  Operand(lo['P1'], 'SiaX', (qc * o, o, v, 1)), 1.0, 0.0, ext_c, r1_split))
+            a2.append(('sum', lo['r1'], 0, lo['P1'], nq * o, o * v, 1.0))
+            # --- T3 -> W_vooo, W_vvvo ---
+            a3 = []
+            a3.append(plan_einsum('t3->W_vooo q%d+%d' % (q0, nq), 'Qread,med->Qram', Z('Qpread'),
+                                  Operand(lo['Eoovv_t'], 'mped', (o, o, v, v)),
+                                  Operand(lo['P3'], 'SQram', (o, nq, o, v, o)), 1.0, 0.0, ext_c, e_split))
+            a3.append(('sum', self.W_oovo, q0 * o * v * o, lo['P3'], o, nq * o * v * o, 1.0))
+            a3.append(plan_einsum('t3->W_vvvo q%d+%d' % (q0, nq), 'Qpreba,rpde->Qabd', Z('Qpreba'),
+                                  Operand(lo['Eoovv'], 'rpde', (o, o, v, v)),
+                                  Operand(self.W_ovvv, 'Qabd', (o, v, v, v)), -1.0, 1.0, ext_c,
+                                  [({}, {}, {'Q': q0})]))
+            self._retarget([g for g in a2 + a3 if isinstance(g, GemmLaunch)])
+            chunks.append((DeviceArray.from_host(s3_jobs_array(jobs)), len(jobs), a2, a3))
+        lo['chunks'] = chunks
+        lo['z_valid'] = False
         self._lower = lo
 
     def set_t1_dressed(self, t1_fock_ov, E_oovv, E_ooov, E_vovv):
@@ -528,10 +554,19 @@ class TriplesEngine:
         L.call("dcc_oooo_prefactors_" + self.sfx, self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
                self.pfs.vp, s)
 
-    def _build_z(self, q):
-        jobs, njobs = self._lower['slices'][q][:2]
+    def _build_z(self, c):
+        jobs, njobs = self._lower['chunks'][c][:2]
         s = self.stream.handle if self.stream is not None else None
         L.call("dcc_s3_transform_" + self.sfx, self._lower['Z'].vp, self.t3.vp, self.nvir, 0.0, jobs.vp, njobs, s)
+
+    def _run_lower(self, steps):
+        s = self.stream.handle if self.stream is not None else None
+        for g in steps:
+            if isinstance(g, tuple):        # ('sum', dst, dst_offset, partials, nslices, n, alpha)
+                _, dst, off, part, ns, n, alpha = g
+                L.call("dcc_sum_slices_" + self.sfx, dst.offset_ptr(off), part.vp, ns, n, alpha, 1.0, s)
+            else:
+                g.run(self.stream)
 
     def t3_to_r1r2(self, download=True):
         """T3 contributions to r1 and the un-symmetrized r2 (rccsdt.py:253-321).
@@ -541,22 +576,26 @@ class TriplesEngine:
         lo = self._lower
         lo['r1'].fill(0.0, self.stream)
         lo['r2'].fill(0.0, self.stream)
-        for q in range(self.nocc):
-            self._build_z(q)
-            for g in lo['slices'][q][2]:
-                g.run(self.stream)
+        for c, chunk in enumerate(lo['chunks']):
+            self._build_z(c)
+            self._run_lower(chunk[2])
+        lo['z_valid'] = len(lo['chunks']) == 1
         if not download:
             return lo['r1'], lo['r2']
         return (lo['r1'].download(stream=self.stream).reshape(self.nocc, self.nvir).astype(np.float64),
                 lo['r2'].download(stream=self.stream).astype(np.float64))
 
-    def t3_to_w(self):
-        """Add the T3 terms of rccsdt.py:323-366 to the device W_vooo / W_vvvo (after set_intermediates)."""
+    def t3_to_w(self, reuse_z=False):
+        """Add the T3 terms of rccsdt.py:323-366 to the device W_vooo / W_vvvo (after set_intermediates).
+
+        ``reuse_z`` keeps the Z built by the preceding t3_to_r1r2 when T3 and the dressed integrals
+        have not changed since (one chunk only)."""
         lo = self._lower
-        for q in range(self.nocc):
-            self._build_z(q)
-            for g in lo['slices'][q][3]:
-                g.run(self.stream)
+        for c, chunk in enumerate(lo['chunks']):
+            if not (reuse_z and lo['z_valid']):
+                self._build_z(c)
+            self._run_lower(chunk[3])
+        lo['z_valid'] = False
 
     def get_w_t3_adjusted(self):
         """Device W_vooo, W_vvvo (incl. T3 terms) back in the reference layouts."""

@@ -253,6 +253,15 @@ int dcc_tensor_axpby_f32_f64(float *dst, const double *src, const dcc_strided *l
 int dcc_tensor_axpby_f64_f32(double *dst, const float *src, const dcc_strided *layout, double alpha, double beta,
                              dcc_stream_t stream);
 
+/*
+ * dst[i] = beta * dst[i] + alpha * sum_{s < nslices} src[s * n + i] (FP64 sum in slice order): the
+ * reduction of split-K partial products of the T3 -> r1/r2/W contractions (distr_cc/rccsdt.py:253-366).
+ */
+int dcc_sum_slices_f64(double *dst, const double *src, int64_t nslices, int64_t n, double alpha, double beta,
+                       dcc_stream_t stream);
+int dcc_sum_slices_f32(float *dst, const float *src, int64_t nslices, int64_t n, double alpha, double beta,
+                       dcc_stream_t stream);
+
 /* Number of doubles the `partials` scratch of dcc_dot_* must hold. */
 int64_t dcc_dot_partials(void);
 

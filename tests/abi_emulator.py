@@ -243,6 +243,19 @@ class Emulator:
         D[od] = val.astype(dtd)
         return 0
 
+    def dcc_sum_slices_f64(self, *a):
+        return self._sum_slices(np.float64, *a)
+
+    def dcc_sum_slices_f32(self, *a):
+        return self._sum_slices(np.float32, *a)
+
+    def _sum_slices(self, dt, dst, src, nslices, n, alpha, beta, stream):
+        n, ns = int(n), int(nslices)
+        D = self.dev.view(_ptr(dst), dt, n)
+        acc = self.dev.view(_ptr(src), dt, ns * n).reshape(ns, n).astype(np.float64).sum(axis=0) if ns else 0.0
+        D[:] = ((beta * D.astype(np.float64) if beta != 0.0 else 0.0) + alpha * acc).astype(dt)
+        return 0
+
     def dcc_dot_partials(self):
         return 1024
 

@@ -32,6 +32,16 @@ def test_engine_residual_and_update_plan(monkeypatch, o, v):
 
 @pytest.mark.parametrize("o,v", [(1, 3), (3, 4), (4, 3), (2, 5)])
 def test_engine_t3_to_r1r2_and_w_plan(monkeypatch, o, v):
+    _t3_to_r1r2_and_w(monkeypatch, o, v, None)
+
+
+@pytest.mark.parametrize("o,v,qc", [(4, 3, 1), (5, 3, 2), (3, 4, 2)])
+def test_engine_t3_to_r1r2_and_w_chunked(monkeypatch, o, v, qc):
+    """Middle index q processed in chunks of qc (the memory-bounded path of large systems)."""
+    _t3_to_r1r2_and_w(monkeypatch, o, v, qc)
+
+
+def _t3_to_r1r2_and_w(monkeypatch, o, v, qc):
     abi_emulator.install(monkeypatch)
     from distr_cc.synthetic import SyntheticSystem
     from distr_cc.t3_engine import TriplesEngine
@@ -43,6 +53,9 @@ def test_engine_t3_to_r1r2_and_w_plan(monkeypatch, o, v):
     E = 0.5 * (E + E.transpose(1, 0, 3, 2))
     F = s.fock + 0.01 * rng.normal(size=(nmo, nmo))
     eng = TriplesEngine(o, v)
+    if qc is not None:
+        eng._setup_lower(z_budget_bytes=qc * o * o * v ** 3 * 8)
+        assert eng._lower['qc'] == qc and len(eng._lower['chunks']) == -(-o // qc)
     eng.set_t3(t3)
     eng.set_intermediates(im, t2)
     eng.set_t1_dressed(F[:o, o:], E[:o, :o, o:, o:], E[:o, :o, :o, o:], E[o:, :o, o:, o:])
