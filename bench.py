@@ -101,6 +101,7 @@ def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01):
         eng = RCCSDTEngine(o, v, sysm.fock, pppp, sysm.mo_energy, stream=st, t3_dtype=dt,
                            triples_contract=contract)
         eng.set_amplitudes(t1, t2, t3)
+        eng.tr.autotune()
         st.synchronize()
         eng.update_amps()       # warm-up iteration (first-launch costs), then restart
         eng.set_amplitudes(t1, t2, t3)
@@ -272,6 +273,7 @@ def main():
     ap.add_argument("--nvir", type=int, default=90)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="print per-launch timings to stderr")
+    ap.add_argument("--autotune", type=int, default=1, help="autotune the contraction tiles first (0 = no)")
     ap.add_argument("--mixed-iters", type=int, default=3,
                     help="full mixed-vs-FP64 iterations to time and compare (0 = skip)")
     ap.add_argument("--adjoint", type=int, default=1, help="time the T2->T3 VJP (0 = skip)")
@@ -346,6 +348,10 @@ def main():
             e.record(stream)
             ev.setdefault(id(g), []).append((what, e))
 
+    tuned = None
+    if args.autotune:
+        print("bench: autotuning contraction tiles", file=sys.stderr, flush=True)
+        tuned = eng.autotune()
     print("bench: warmup %d" % args.warmup, file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()
@@ -405,7 +411,9 @@ def main():
                    "nocc": o, "nvir": v, "naux": 300, "seed": 20261202, "precision": "fp64",
                    "residual_tflops": F / (ms * 1e-3) / 1e12,
                    "residual_frac_fp64_peak": F / (ms * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,
-                   "r3_norm_sq": norms[0], "wall_s": wall},
+                   "r3_norm_sq": norms[0], "wall_s": wall,
+                   "autotuned_tiles": None if tuned is None else
+                   {k: v[0] for k, v in tuned.items() if k.split(" ", 1)[1].startswith(("W_vvvv", "ring"))}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_MFMA_TFLOPS, "traffic": measured_traffic(o, v)[0],
                      "traffic_source": measured_traffic(o, v)[1],

@@ -588,7 +588,8 @@ bool valid_map(const dcc_idxmap &m) {
 template <typename T>
 int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, const T *A, dcc_idxmap am,
                    dcc_idxmap ak, const T *B, dcc_idxmap bk, dcc_idxmap bn, T *C, dcc_idxmap cm, dcc_idxmap cn,
-                   const dcc_batch *batches, int64_t nbatch, int64_t max_rows, dcc_stream_t stream) {
+                   const dcc_batch *batches, int64_t nbatch, int64_t max_rows, dcc_stream_t stream,
+                   int64_t tile_shape = -1) {
     DCC_CHECK(M >= 0 && N >= 0 && K >= 0 && nbatch >= 0 && max_rows >= 0, "negative extent");
     DCC_CHECK(M < ((int64_t)1 << 31) && N < ((int64_t)1 << 31) && K < ((int64_t)1 << 31),
               "extent exceeds 2^31");
@@ -628,7 +629,12 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     // 64x64 keeps four waves per SIMD resident, which hides HBM latency better
     // than the larger register tiles (measured on MI355X for every triples GEMM).
     int tbm = 64, tbn = 64;
-    if (tile_env != nullptr && sscanf(tile_env, "%dx%d", &tbm, &tbn) != 2) {
+    if (tile_shape >= 0) {
+        DCC_CHECK(tile_shape <= 3, "tile shape must be 0..3");
+        const int shapes[4][2] = {{64, 64}, {128, 64}, {64, 128}, {128, 128}};
+        tbm = shapes[tile_shape][0];
+        tbn = shapes[tile_shape][1];
+    } else if (tile_env != nullptr && sscanf(tile_env, "%dx%d", &tbm, &tbn) != 2) {
         tbm = 64;
         tbn = 64;
     }
@@ -714,6 +720,22 @@ extern "C" int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, d
                                 int64_t nbatch, int64_t max_rows, dcc_stream_t stream) {
     return contract_entry<double>(M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch,
                                   max_rows, stream);
+}
+
+extern "C" int dcc_contract_tiled_f64(int64_t M, int64_t N, int64_t K, double alpha, double beta, const double *A,
+                                      dcc_idxmap am, dcc_idxmap ak, const double *B, dcc_idxmap bk, dcc_idxmap bn,
+                                      double *C, dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches,
+                                      int64_t nbatch, int64_t max_rows, int64_t tile_shape, dcc_stream_t stream) {
+    return contract_entry<double>(M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch,
+                                  max_rows, stream, tile_shape);
+}
+
+extern "C" int dcc_contract_tiled_f32(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
+                                      dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn,
+                                      float *C, dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches,
+                                      int64_t nbatch, int64_t max_rows, int64_t tile_shape, dcc_stream_t stream) {
+    return contract_entry<float>(M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch,
+                                 max_rows, stream, tile_shape);
 }
 
 extern "C" int dcc_contract_f32(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,

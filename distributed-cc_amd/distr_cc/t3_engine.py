@@ -113,10 +113,15 @@ class GemmLaunch:
         self.max_rows = int(max_rows)
         self.flops = float(flops)
         self.keep = keep  # device tables referenced by maps
+        self.tile = -1    # -1: library default; 0..3: 64x64, 128x64, 64x128, 128x128 (autotuned)
 
     def run(self, stream):
-        L.call(self.fn, *self.args, self.batches.vp, self.nbatch, self.max_rows,
-               stream.handle if stream is not None else None)
+        s = stream.handle if stream is not None else None
+        if self.tile >= 0 and self.fn in ("dcc_contract_f64", "dcc_contract_f32"):
+            L.call(self.fn.replace("contract_", "contract_tiled_"), *self.args, self.batches.vp, self.nbatch,
+                   self.max_rows, self.tile, s)
+        else:
+            L.call(self.fn, *self.args, self.batches.vp, self.nbatch, self.max_rows, s)
 
 
 def _fuse(letters, ops, ext):
@@ -666,6 +671,38 @@ class TriplesEngine:
         self.set_denominators(mo_energy, level_shift, nvir_diis)
         self.finalize_launch(r3_out)
         return self.read_norms()
+
+    def autotune(self, reps=2, include_lower=True):
+        """Time the four workgroup tiles for every contraction launch and keep the fastest.
+
+        Run it before a residual (it accumulates into R and the scratch buffers); returns
+        {launch name: (tile, ms)}.  The split-BF16x3 launches have one tile and are skipped."""
+        from distr_cc.device import Event
+        shapes = ("64x64", "128x64", "64x128", "128x128")
+        launches = list(self.launches)
+        if include_lower and self._lower is not None:
+            for chunk in self._lower['chunks']:
+                launches += [g for g in chunk[2] + chunk[3] if isinstance(g, GemmLaunch)]
+        out = {}
+        st = self.stream
+        for n, g in enumerate(launches):
+            if not isinstance(g, GemmLaunch) or g.fn not in ("dcc_contract_f64", "dcc_contract_f32"):
+                continue
+            best = None
+            for tile in range(4):
+                g.tile = tile
+                g.run(st)
+                a, b = Event(), Event()
+                a.record(st)
+                for _ in range(reps):
+                    g.run(st)
+                b.record(st)
+                ms = a.elapsed_ms(b) / reps
+                if best is None or ms < best[1]:
+                    best = (tile, ms)
+            g.tile = best[0]
+            out["%d %s" % (n, g.name)] = (shapes[best[0]], best[1])
+        return out
 
     def residual_flops(self):
         return sum(g.flops for g in self.launches)

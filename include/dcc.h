@@ -112,6 +112,19 @@ int dcc_contract_f32(int64_t M, int64_t N, int64_t K, double alpha, double beta,
                      dcc_stream_t stream);
 
 /*
+ * dcc_contract_f64 / _f32 with an explicit workgroup tile: tile_shape 0 = 64x64, 1 = 128x64,
+ * 2 = 64x128, 3 = 128x128 (M x N); chosen per launch by the engines' autotuner.
+ */
+int dcc_contract_tiled_f64(int64_t M, int64_t N, int64_t K, double alpha, double beta, const double *A,
+                           dcc_idxmap am, dcc_idxmap ak, const double *B, dcc_idxmap bk, dcc_idxmap bn, double *C,
+                           dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
+                           int64_t max_rows, int64_t tile_shape, dcc_stream_t stream);
+int dcc_contract_tiled_f32(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
+                           dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn, float *C,
+                           dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
+                           int64_t max_rows, int64_t tile_shape, dcc_stream_t stream);
+
+/*
  * dcc_contract_f32 with split-BF16x3 products: each FP32 operand x is split into hi = bf16(x) and
  * lo = bf16(x - hi) (round to nearest even) and every product is accumulated as
  * hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 in FP32 (relative product error ~2^-16).

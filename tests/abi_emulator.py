@@ -183,6 +183,12 @@ class Emulator:
     def dcc_contract_f32(self, *a):
         return self._contract(np.float32, *a)
 
+    def dcc_contract_tiled_f64(self, *a):
+        return self._contract(np.float64, *a[:-2], a[-1])
+
+    def dcc_contract_tiled_f32(self, *a):
+        return self._contract(np.float32, *a[:-2], a[-1])
+
     def dcc_contract_f32_bf16x3(self, *a):
         return self._contract("bf16x3", *a)
 

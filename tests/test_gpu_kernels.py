@@ -287,3 +287,26 @@ def test_batched_ensemble_vs_engines(dev):
         assert abs(e.get_t3() - t3ref).max() <= 1e-12 * abs(t3ref).max()
         assert abs(e.read_norms()[0] - nrm[0]) <= 1e-11 * nrm[0]
     bt.close()
+
+
+@pytest.mark.parametrize("o,v", [(3, 20), (5, 17)])
+def test_residual_autotuned_vs_oracle(dev, o, v):
+    """Every contraction on its autotuned tile (all four tile shapes get exercised) == oracle."""
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    from oracle import rccsdt_tri as tri
+    _say("residual autotuned o=%d v=%d" % (o, v))
+    s = SyntheticSystem(o, v, naux=30, seed=800 + o * v)
+    im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
+    eng = TriplesEngine(o, v)
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    tuned = eng.autotune()
+    _say("tiles %s" % sorted(set(t for t, _ in tuned.values())))
+    for tile in range(4):   # force each shape once on every launch, then the tuned choice
+        for g in eng.launches:
+            if hasattr(g, "tile"):
+                g.tile = tile
+        eng.residual()
+        Ro = tri.compute_r3_tri(o, v, im, t2, t3)
+        assert abs(eng.R.download() - Ro).max() <= 1e-12 * abs(Ro).max(), tile
