@@ -65,8 +65,8 @@ class Event:
         self.handle = ctypes.c_void_p()
         L.call("dcc_event_create", ctypes.byref(self.handle))
 
-    def record(self, stream):
-        L.call("dcc_event_record", self.handle, stream.handle)
+    def record(self, stream=None):
+        L.call("dcc_event_record", self.handle, stream.handle if stream is not None else None)
 
     def elapsed_ms(self, later):
         ms = ctypes.c_float()
