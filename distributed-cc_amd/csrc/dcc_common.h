@@ -15,6 +15,49 @@ void clear_error();
 
 inline hipStream_t as_stream(dcc_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
+// K index map of the gather-GEMMs, evaluated without integer division or branches: the two
+// quotients of the three-level mixed-radix decomposition use round-up reciprocals,
+// q = (n * mul) >> sh with sh = 31 + ceil(log2 d) and mul = ceil(2^sh / d), which is exact for every
+// n < 2^31 (the error n * (mul * d - 2^sh) / (d * 2^sh) stays below 1/d).
+struct KMap {
+    int64_t s0, s1, s2;   // strides of the three levels
+    uint32_t e2, e12;     // extents of the inner level and of the two inner levels
+    uint32_t mul2, mul12; // reciprocals of e2 and e12
+    uint32_t sh2, sh12;
+};
+
+__host__ __device__ inline void magic(uint32_t d, uint32_t &mul, uint32_t &sh) {
+    uint32_t l = 0;
+    while (((uint64_t)1 << l) < d) ++l;
+    sh = 31 + l;
+    mul = (uint32_t)((((uint64_t)1 << sh) + d - 1) / d);
+}
+
+__host__ __device__ inline KMap make_kmap(const dcc_idxmap &m) {
+    KMap k;
+    k.s0 = m.s0;
+    k.s1 = m.s1;
+    k.s2 = m.s2;
+    k.e2 = (uint32_t)m.e2;
+    k.e12 = (uint32_t)(m.e1 * m.e2);
+    magic(k.e2, k.mul2, k.sh2);
+    magic(k.e12, k.mul12, k.sh12);
+    return k;
+}
+
+__device__ __forceinline__ uint32_t magic_div(uint32_t n, uint32_t mul, uint32_t sh) {
+    return (uint32_t)(((uint64_t)n * mul) >> sh);
+}
+
+// Offset of index k (< 2^31) under the map.
+__device__ __forceinline__ int64_t koff(const KMap &m, uint32_t k) {
+    const uint32_t q0 = magic_div(k, m.mul12, m.sh12);
+    const uint32_t r0 = k - q0 * m.e12;
+    const uint32_t q1 = magic_div(r0, m.mul2, m.sh2);
+    const uint32_t q2 = r0 - q1 * m.e2;
+    return (int64_t)q0 * m.s0 + (int64_t)q1 * m.s1 + (int64_t)q2 * m.s2;
+}
+
 }  // namespace dcc
 
 // Return a nonzero status from an entry point when a HIP call fails.

@@ -14,12 +14,13 @@
 // (BM/2) x (BN/2) sub-tile as (BM/32) x (BN/32) MFMA 16x16 blocks; BK = 16
 // with a double-buffered LDS pipeline (one barrier per K step).  Every thread
 // loads its A and B elements at one fixed k row of the tile, so the K index
-// map is evaluated once per operand and then advanced by BK with carries --
-// no integer division inside the K loop.
+// map is evaluated once per operand and tile (by reciprocal multiplication),
+// and the loop body is free of branches.
 #include "dcc_common.h"
 
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 
 namespace {
 
@@ -60,44 +61,9 @@ __device__ __forceinline__ int64_t map_off(const dcc_idxmap &mp, uint32_t x, int
     return (int64_t)q0 * mp.s0 + (int64_t)q1 * mp.s1 + (int64_t)q2 * mp.s2;
 }
 
-// K index map in incremental form: offset of k, advanced by kBK per step.
-struct KMap {
-    int64_t s0, s1, s2;   // strides of the three levels
-    int64_t step;         // bk * s2
-    int64_t carry1;       // s1 - e2 * s2  (q2 wraps into q1)
-    int64_t carry0;       // s0 - e1 * s1  (q1 wraps into q0)
-    uint32_t e1, e2;
-    uint32_t bk;          // K advance per step
-};
-
-struct KIter {
-    uint32_t q1, q2;
-    int64_t off;
-};
-
-__device__ __forceinline__ KIter kinit(const KMap &m, uint32_t k) {
-    const uint32_t e12 = m.e1 * m.e2;
-    const uint32_t q0 = k / e12;
-    const uint32_t r0 = k - q0 * e12;
-    KIter it;
-    it.q1 = r0 / m.e2;
-    it.q2 = r0 - it.q1 * m.e2;
-    it.off = (int64_t)q0 * m.s0 + (int64_t)it.q1 * m.s1 + (int64_t)it.q2 * m.s2;
-    return it;
-}
-
-__device__ __forceinline__ void kadvance(const KMap &m, KIter &it) {
-    it.q2 += m.bk;
-    it.off += m.step;
-    while (it.q2 >= m.e2) {  // at most ceil(kBK / e2) passes
-        it.q2 -= m.e2;
-        it.off += m.carry1;
-        if (++it.q1 >= m.e1) {
-            it.q1 = 0;
-            it.off += m.carry0;
-        }
-    }
-}
+using dcc::KMap;
+using dcc::koff;
+using dcc::make_kmap;
 
 template <typename T>
 struct ContractArgs {
@@ -128,8 +94,13 @@ template <int a, int b> struct CMax { static constexpr int v = a > b ? a : b; };
 // C_MFAST selects the epilogue: false writes C straight from the accumulators (16 consecutive lanes
 // walk n, coalesced when C is contiguous along n); true stages the tile in LDS and writes it with
 // consecutive lanes walking m, for outputs whose contiguous index is carried by M.
-template <typename T, int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST>
-__global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T> p) {
+// Minimum resident waves per SIMD the register allocator must keep: the 64x64 tile keeps four (the
+// LDS of four workgroups fits a CU); larger tiles are limited by LDS anyway.
+template <int BM, int BN> struct MinWaves { static constexpr int v = (BM * BN <= 64 * 64) ? 4 : 1; };
+
+template <typename T, int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int STAGES>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MinWaves<BM, BN>::v)))
+void contract_kernel(const ContractArgs<T> p) {
     typedef Mfma<T> MF;
     typedef typename MF::acc_t acc_t;
     constexpr int WM = BM / 2, WN = BN / 2;
@@ -170,44 +141,45 @@ __global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T
     const int b_kk = B_NFAST ? (tid / 16) : (tid % kBK);
     const int b_n_base = B_NFAST ? (tid % 16) : (tid / kBK);
     int64_t a_row[A_PER_T];
-    uint32_t a_ok = 0;
 #pragma unroll
     for (int r = 0; r < A_PER_T; ++r) {
         const int64_t m = m0 + a_m_base + 16 * r;
-        const bool ok = m < m_end;
-        a_ok |= (ok ? 1u : 0u) << r;
-        a_row[r] = ok ? bt.a_off + map_off(p.am, (uint32_t)m, batch) : 0;
+        a_row[r] = m < m_end ? bt.a_off + map_off(p.am, (uint32_t)m, batch) : 0;
     }
     int64_t b_col[B_PER_T];
-    uint32_t b_ok = 0;
 #pragma unroll
     for (int r = 0; r < B_PER_T; ++r) {
         const int n = n0 + b_n_base + 16 * r;
-        const bool ok = n < p.N;
-        b_ok |= (ok ? 1u : 0u) << r;
-        b_col[r] = ok ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
+        b_col[r] = n < p.N ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
     }
-    KIter ka = kinit(p.ak, (uint32_t)a_kk);
-    KIter kb = kinit(p.bk, (uint32_t)b_kk);
-
-    T a_reg[A_PER_T], b_reg[B_PER_T];
-    auto load_tiles = [&](int k0) {
-        const bool ak_ok = k0 + a_kk < p.K;
-        const bool bk_ok = k0 + b_kk < p.K;
+    // STAGES = 2: two register sets hold the tiles k+1 and k+2 while tile k is multiplied out of LDS,
+    // so every global load has two K steps of MFMA work (plus the other resident waves) to hide
+    // behind; STAGES = 1 keeps one set (fewer registers, more resident waves).
+    // Loads are branch-free: rows/columns outside the operand read its offset 0 (they only feed
+    // outputs that are never stored), and k beyond K reads offset 0 and is replaced by zero.
+    T a_reg[STAGES][A_PER_T], b_reg[STAGES][B_PER_T];
+    auto load_tiles = [&](int s, int k0) {
+        const uint32_t ka = (uint32_t)(k0 + a_kk), kb = (uint32_t)(k0 + b_kk);
+        const bool ak_ok = ka < (uint32_t)p.K;
+        const bool bk_ok = kb < (uint32_t)p.K;
+        const int64_t aoff = koff(p.ak, ak_ok ? ka : 0u);
+        const int64_t boff = koff(p.bk, bk_ok ? kb : 0u);
 #pragma unroll
-        for (int r = 0; r < A_PER_T; ++r)
-            a_reg[r] = (ak_ok && ((a_ok >> r) & 1u)) ? p.A[a_row[r] + ka.off] : T(0);
+        for (int r = 0; r < A_PER_T; ++r) {
+            const T x = p.A[a_row[r] + aoff];
+            a_reg[s][r] = ak_ok ? x : T(0);
+        }
 #pragma unroll
-        for (int r = 0; r < B_PER_T; ++r)
-            b_reg[r] = (bk_ok && ((b_ok >> r) & 1u)) ? p.B[b_col[r] + kb.off] : T(0);
-        kadvance(p.ak, ka);
-        kadvance(p.bk, kb);
+        for (int r = 0; r < B_PER_T; ++r) {
+            const T x = p.B[b_col[r] + boff];
+            b_reg[s][r] = bk_ok ? x : T(0);
+        }
     };
-    auto store_tiles = [&](int buf) {
+    auto store_tiles = [&](int s, int buf) {
 #pragma unroll
-        for (int r = 0; r < A_PER_T; ++r) As[buf][a_kk][a_m_base + 16 * r] = a_reg[r];
+        for (int r = 0; r < A_PER_T; ++r) As[buf][a_kk][a_m_base + 16 * r] = a_reg[s][r];
 #pragma unroll
-        for (int r = 0; r < B_PER_T; ++r) Bs[buf][b_kk][b_n_base + 16 * r] = b_reg[r];
+        for (int r = 0; r < B_PER_T; ++r) Bs[buf][b_kk][b_n_base + 16 * r] = b_reg[s][r];
     };
 
     const int wave = tid / 64, lane = tid % 64;
@@ -218,14 +190,7 @@ __global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
-
-    const int nk = (p.K + kBK - 1) / kBK;
-    load_tiles(0);
-    store_tiles(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) load_tiles((kt + 1) * kBK);
+    auto compute = [&](int buf) {
 #pragma unroll
         for (int ks = 0; ks < kBK / 4; ++ks) {
             const int kr = ks * 4 + lk;
@@ -240,8 +205,40 @@ __global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T
                 for (int j = 0; j < FN; ++j)
                     acc[i][j] = MF::step(af[i], bf[j], acc[i][j]);
         }
-        if (kt + 1 < nk) store_tiles(buf ^ 1);
+    };
+
+    // Prefetches past the last tile read offset 0 and land in the idle buffer, which nothing reads
+    // afterwards.
+    const int nk = (p.K + kBK - 1) / kBK;
+    if constexpr (STAGES == 2) {
+        // Step kt multiplies LDS buffer kt&1 while register set kt&1 receives tile kt+2 and set
+        // (kt+1)&1 (tile kt+1) is written to the other buffer; one barrier per step.  The loop is
+        // unrolled by two so register sets are indexed statically.
+        load_tiles(0, 0);
+        load_tiles(1, kBK);
+        store_tiles(0, 0);
         __syncthreads();
+        for (int kt = 0; kt < nk; kt += 2) {
+            load_tiles(0, (kt + 2) * kBK);
+            compute(0);
+            store_tiles(1, 1);
+            __syncthreads();
+            if (kt + 1 >= nk) break;
+            load_tiles(1, (kt + 3) * kBK);
+            compute(1);
+            store_tiles(0, 0);
+            __syncthreads();
+        }
+    } else {
+        load_tiles(0, 0);
+        store_tiles(0, 0);
+        __syncthreads();
+        for (int kt = 0; kt < nk; ++kt) {
+            load_tiles(0, (kt + 1) * kBK);
+            compute(kt & 1);
+            store_tiles(0, (kt & 1) ^ 1);
+            __syncthreads();
+        }
     }
 
     if constexpr (C_MFAST) {
@@ -315,34 +312,84 @@ __global__ __launch_bounds__(kThreads) void contract_kernel(const ContractArgs<T
 
 // ---------------------------------------------------------------------------------------------
 // Split-BF16x3: FP32 operands split as x = hi + lo with hi = bf16(x), lo = bf16(x - hi) (round to
-// nearest even); each product is hi*hi + hi*lo + lo*hi on v_mfma_f32_16x16x32_bf16 with FP32
-// accumulation (the lo*lo term, ~2^-16 relative, is dropped).  Same index maps, batching and
-// epilogues as contract_kernel; BK = 32 and the hi/lo tiles are stored k-contiguous in LDS so a
-// lane reads its 8 consecutive k values of a row (A) or column (B) as one 16-byte load:
+// nearest even, v_cvt_pk_bf16_f32); each product is hi*hi + hi*lo + lo*hi on
+// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (the lo*lo term, ~2^-16 relative, is dropped).
+// Same index maps, batching and epilogues as contract_kernel; BK = 32 (one MFMA K step).
+//
+// MFMA operand layouts (wave64):
 //   A fragment: lane holds A[lane%16][8*(lane/16) + 0..7];  B fragment: B[8*(lane/16) + 0..7][lane%16];
 //   accumulator: lane holds D[4*(lane/16) + r][lane%16] (as v_mfma_f32_16x16x4f32).
+// Each operand is staged in LDS as hi and lo planes in one of two layouts, chosen by which of its
+// indices is contiguous in memory:
+//   K-contiguous ("row-major" [row][k], stride kKP3): every thread loads a pair (k, k+1) of R/16 rows
+//     and writes each pair as one 32-bit word per plane; a fragment is one 16-byte ds_read_b128.
+//   row-contiguous ([k][row], stride R + 16): every thread loads a pair of adjacent rows at R/16
+//     values of k and writes each pair as one word per plane; a fragment is two ds_read_b64_tr_b16,
+//     which on gfx950 gives lane 16g + i the four values at column i of the 4 x 16 block whose row j
+//     the lanes 16g + 4j .. 16g + 4j + 3 address (four 16-bit values each).
+// The K offsets of the next tile's 32 k values are computed once per step by wave 0 (reciprocal
+// multiplication) into a small LDS table, so the loaders need no division; k >= K is marked -1 and
+// masked only in the last tile.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
 constexpr int kBK3 = 32;
+constexpr int kKP3 = kBK3 + 8;  // [row][k] stride (16-bit units): 80-byte rows, conflict-free b128 reads
 
-__device__ __forceinline__ unsigned short bf16_rne(float x) {
-    uint32_t u = __float_as_uint(x);
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (unsigned short)(u >> 16);
+// hi/lo bf16 pairs of two floats, element 0 in the low half of each word.
+__device__ __forceinline__ void split_pair(float x0, float x1, uint32_t &hi, uint32_t &lo) {
+    const bf16x2 h = {(__bf16)x0, (__bf16)x1};
+    hi = __builtin_bit_cast(uint32_t, h);
+    const float h0 = __uint_as_float(hi << 16), h1 = __uint_as_float(hi & 0xffff0000u);
+    const bf16x2 l = {(__bf16)(x0 - h0), (__bf16)(x1 - h1)};
+    lo = __builtin_bit_cast(uint32_t, l);
 }
 
+// Staging geometry of one operand with R rows (m for A, n for B) per tile.
+template <int R, bool KFAST> struct Stage3 {
+    static constexpr int LD = KFAST ? kKP3 : R + 16;            // LDS row stride (16-bit units)
+    static constexpr int PLANE = KFAST ? R * kKP3 : kBK3 * (R + 16);
+    static constexpr int NR = KFAST ? R / 16 : 2;               // rows per thread
+    static constexpr int NK = KFAST ? 2 : R / 16;               // k values per thread
+    static constexpr int RP = R / 2;                            // row pairs (row-contiguous layout)
+    __device__ static int row(int tid, int r) { return KFAST ? tid / 16 + 16 * r : 2 * (tid % RP) + r; }
+    __device__ static int kidx(int tid, int j) {
+        return KFAST ? 2 * (tid % 16) + j : tid / RP + (kThreads / RP) * j;
+    }
+    // 16-bit element index of (row, k) in a plane
+    __device__ static int at(int row, int k) { return KFAST ? row * kKP3 + k : k * (R + 16) + row; }
+    // fragment of rows rbase..rbase+15 (16 rows x 8 k values per lane group) from one plane
+    __device__ static bf16x8 frag(const unsigned short *plane, int rbase, int lane) {
+        const int lrow = lane % 16, g = lane / 16;
+        if constexpr (KFAST) {
+            return *reinterpret_cast<const bf16x8 *>(&plane[(rbase + lrow) * kKP3 + 8 * g]);
+        } else {
+            const int kr = 8 * g + lrow / 4, c = rbase + 4 * (lrow % 4);
+            typedef __attribute__((address_space(3))) unsigned short lds_u16;
+            const lds_u16 *base = (const lds_u16 *)plane;
+            const s16x4 lo4 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(base + kr * (R + 16) + c));
+            const s16x4 hi4 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(base + (kr + 4) * (R + 16) + c));
+            const bf16x4 a = __builtin_bit_cast(bf16x4, lo4), b = __builtin_bit_cast(bf16x4, hi4);
+            return bf16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+        }
+    }
+};
+
 template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST>
-__global__ __launch_bounds__(kThreads) void contract_bf16x3_kernel(const ContractArgs<float> p) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+void contract_bf16x3_kernel(const ContractArgs<float> p) {
+    typedef Stage3<BM, A_KFAST> SA;
+    typedef Stage3<BN, !B_NFAST> SB;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
-    constexpr int KP = kBK3 + 8;                  // padded k stride: conflict-free 16-byte reads
-    constexpr int A_PER_T = BM * kBK3 / kThreads;
-    constexpr int B_PER_T = BN * kBK3 / kThreads;
-    constexpr int A_ELEMS = 2 * 2 * BM * KP, B_ELEMS = 2 * 2 * BN * KP;   // [buf][hi/lo][row][k]
-    constexpr int MAIN_BYTES = (A_ELEMS + B_ELEMS) * 2;
+    constexpr int MAIN_BYTES = 2 * (SA::PLANE + SB::PLANE) * 2;  // hi + lo planes, 16-bit
     constexpr int EPI_BYTES = C_MFAST ? BN * (BM + 1) * 4 : 0;
     __shared__ __align__(16) unsigned char smem[MAIN_BYTES > EPI_BYTES ? MAIN_BYTES : EPI_BYTES];
+    __shared__ int64_t ktab[2][2][kBK3];  // [tile parity][A, B][k]
     unsigned short *As = reinterpret_cast<unsigned short *>(smem);
-    unsigned short *Bs = As + A_ELEMS;
+    unsigned short *Bs = As + 2 * SA::PLANE;
 
     const int tid = threadIdx.x;
     const int64_t L = xcd_remap((int64_t)blockIdx.x, p.total_tiles);
@@ -362,66 +409,99 @@ __global__ __launch_bounds__(kThreads) void contract_bf16x3_kernel(const Contrac
     if (m0 >= bt.m_end) return;
     const int64_t m_end = bt.m_end;
     const int n0 = tn * BN;
-
-    // one k row per thread and operand; rows/columns base + 8r
-    const int a_kk = A_KFAST ? (tid % kBK3) : (tid / 8);
-    const int a_m_base = A_KFAST ? (tid / kBK3) : (tid % 8);
-    const int b_kk = B_NFAST ? (tid / 8) : (tid % kBK3);
-    const int b_n_base = B_NFAST ? (tid % 8) : (tid / kBK3);
-    int64_t a_row[A_PER_T];
-    uint32_t a_ok = 0;
-#pragma unroll
-    for (int r = 0; r < A_PER_T; ++r) {
-        const int64_t m = m0 + a_m_base + 8 * r;
-        const bool ok = m < m_end;
-        a_ok |= (ok ? 1u : 0u) << r;
-        a_row[r] = ok ? bt.a_off + map_off(p.am, (uint32_t)m, batch) : 0;
-    }
-    int64_t b_col[B_PER_T];
-    uint32_t b_ok = 0;
-#pragma unroll
-    for (int r = 0; r < B_PER_T; ++r) {
-        const int n = n0 + b_n_base + 8 * r;
-        const bool ok = n < p.N;
-        b_ok |= (ok ? 1u : 0u) << r;
-        b_col[r] = ok ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
-    }
-    KIter ka = kinit(p.ak, (uint32_t)a_kk);
-    KIter kb = kinit(p.bk, (uint32_t)b_kk);
-
-    float a_reg[A_PER_T], b_reg[B_PER_T];
-    auto load_tiles = [&](int k0) {
-        const bool ak_ok = k0 + a_kk < p.K;
-        const bool bk_ok = k0 + b_kk < p.K;
-#pragma unroll
-        for (int r = 0; r < A_PER_T; ++r)
-            a_reg[r] = (ak_ok && ((a_ok >> r) & 1u)) ? p.A[a_row[r] + ka.off] : 0.0f;
-#pragma unroll
-        for (int r = 0; r < B_PER_T; ++r)
-            b_reg[r] = (bk_ok && ((b_ok >> r) & 1u)) ? p.B[b_col[r] + kb.off] : 0.0f;
-        kadvance(p.ak, ka);
-        kadvance(p.bk, kb);
-    };
-    auto store_tiles = [&](int buf) {
-#pragma unroll
-        for (int r = 0; r < A_PER_T; ++r) {
-            const unsigned short hi = bf16_rne(a_reg[r]);
-            const unsigned short lo = bf16_rne(a_reg[r] - __uint_as_float((uint32_t)hi << 16));
-            const int row = a_m_base + 8 * r;
-            As[((buf * 2 + 0) * BM + row) * KP + a_kk] = hi;
-            As[((buf * 2 + 1) * BM + row) * KP + a_kk] = lo;
-        }
-#pragma unroll
-        for (int r = 0; r < B_PER_T; ++r) {
-            const unsigned short hi = bf16_rne(b_reg[r]);
-            const unsigned short lo = bf16_rne(b_reg[r] - __uint_as_float((uint32_t)hi << 16));
-            const int col = b_n_base + 8 * r;
-            Bs[((buf * 2 + 0) * BN + col) * KP + b_kk] = hi;
-            Bs[((buf * 2 + 1) * BN + col) * KP + b_kk] = lo;
-        }
-    };
-
     const int wave = tid / 64, lane = tid % 64;
+
+    // row offsets; rows outside the operand read offset 0 (they only feed outputs never stored)
+    int64_t a_row[SA::NR], b_col[SB::NR];
+#pragma unroll
+    for (int r = 0; r < SA::NR; ++r) {
+        const int64_t m = m0 + SA::row(tid, r);
+        a_row[r] = m < m_end ? bt.a_off + map_off(p.am, (uint32_t)m, batch) : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < SB::NR; ++r) {
+        const int n = n0 + SB::row(tid, r);
+        b_col[r] = n < p.N ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
+    }
+
+    // wave 0: K offsets of tile k0 into table slot t (-1 beyond K)
+    auto fill_table = [&](int t, int k0) {
+        if (wave == 0) {
+            const uint32_t k = (uint32_t)(k0 + (lane & 31));
+            const int64_t oa = koff(p.ak, k), ob = koff(p.bk, k);
+            ktab[t][lane >> 5][lane & 31] = k < (uint32_t)p.K ? (lane < 32 ? oa : ob) : int64_t(-1);
+        }
+    };
+
+    float a_reg[SA::NR][SA::NK], b_reg[SB::NR][SB::NK];
+    auto load_tiles = [&](int t, auto tail) {
+        constexpr bool TAIL = decltype(tail)::value;
+#pragma unroll
+        for (int j = 0; j < SA::NK; ++j) {
+            const int64_t o = ktab[t][0][SA::kidx(tid, j)];
+#pragma unroll
+            for (int r = 0; r < SA::NR; ++r) {
+                if constexpr (TAIL) {
+                    const float x = p.A[a_row[r] + (o < 0 ? 0 : o)];
+                    a_reg[r][j] = o < 0 ? 0.0f : x;
+                } else {
+                    a_reg[r][j] = p.A[a_row[r] + o];
+                }
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < SB::NK; ++j) {
+            const int64_t o = ktab[t][1][SB::kidx(tid, j)];
+#pragma unroll
+            for (int r = 0; r < SB::NR; ++r) {
+                if constexpr (TAIL) {
+                    const float x = p.B[b_col[r] + (o < 0 ? 0 : o)];
+                    b_reg[r][j] = o < 0 ? 0.0f : x;
+                } else {
+                    b_reg[r][j] = p.B[b_col[r] + o];
+                }
+            }
+        }
+    };
+    // pairs along k (K-contiguous layout) or along rows (row-contiguous layout), one word per plane
+    auto store_tiles = [&]() {
+        uint32_t hi, lo;
+        if constexpr (A_KFAST) {
+#pragma unroll
+            for (int r = 0; r < SA::NR; ++r) {
+                split_pair(a_reg[r][0], a_reg[r][1], hi, lo);
+                const int e = SA::at(SA::row(tid, r), SA::kidx(tid, 0));
+                *reinterpret_cast<uint32_t *>(&As[e]) = hi;
+                *reinterpret_cast<uint32_t *>(&As[SA::PLANE + e]) = lo;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SA::NK; ++j) {
+                split_pair(a_reg[0][j], a_reg[1][j], hi, lo);
+                const int e = SA::at(SA::row(tid, 0), SA::kidx(tid, j));
+                *reinterpret_cast<uint32_t *>(&As[e]) = hi;
+                *reinterpret_cast<uint32_t *>(&As[SA::PLANE + e]) = lo;
+            }
+        }
+        if constexpr (!B_NFAST) {
+#pragma unroll
+            for (int r = 0; r < SB::NR; ++r) {
+                split_pair(b_reg[r][0], b_reg[r][1], hi, lo);
+                const int e = SB::at(SB::row(tid, r), SB::kidx(tid, 0));
+                *reinterpret_cast<uint32_t *>(&Bs[e]) = hi;
+                *reinterpret_cast<uint32_t *>(&Bs[SB::PLANE + e]) = lo;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < SB::NK; ++j) {
+                split_pair(b_reg[0][j], b_reg[1][j], hi, lo);
+                const int e = SB::at(SB::row(tid, 0), SB::kidx(tid, j));
+                *reinterpret_cast<uint32_t *>(&Bs[e]) = hi;
+                *reinterpret_cast<uint32_t *>(&Bs[SB::PLANE + e]) = lo;
+            }
+        }
+    };
+
     const int wm = wave / 2, wn = wave % 2;
     const int lrow = lane % 16, lk = lane / 16;
     flt4 acc[FM][FN];
@@ -430,35 +510,43 @@ __global__ __launch_bounds__(kThreads) void contract_bf16x3_kernel(const Contrac
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = flt4{0, 0, 0, 0};
 
+    // One LDS buffer and one register stage: step kt loads tile kt+1 into registers, multiplies
+    // tile kt from LDS, then (after a barrier) writes tile kt+1 to LDS.  Table slot t&1 holds
+    // the K offsets of tile t; wave 0 fills slot kt&1 for tile kt+2 during step kt (its last
+    // reader, the load of tile kt, finished before the barriers of step kt-1).
     const int nk = (p.K + kBK3 - 1) / kBK3;
-    load_tiles(0);
-    store_tiles(0);
+    const int nfull = p.K / kBK3;  // tiles with every k < K
+    fill_table(0, 0);
+    fill_table(1, kBK3);
+    __syncthreads();
+    if (nfull > 0) load_tiles(0, std::false_type{});
+    else load_tiles(0, std::true_type{});
+    store_tiles();
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) load_tiles((kt + 1) * kBK3);
-        bf16x8 ah[FM], al[FM], bh[FN], bl[FN];
+        const int t1 = (kt + 1) & 1;
+        if (kt + 1 < nfull) load_tiles(t1, std::false_type{});
+        else load_tiles(t1, std::true_type{});
+        fill_table(kt & 1, (kt + 2) * kBK3);
+        bf16x8 ah[FM], al[FM];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-            const int row = wm * WM + i * 16 + lrow;
-            ah[i] = *reinterpret_cast<const bf16x8 *>(&As[((buf * 2 + 0) * BM + row) * KP + 8 * lk]);
-            al[i] = *reinterpret_cast<const bf16x8 *>(&As[((buf * 2 + 1) * BM + row) * KP + 8 * lk]);
+            ah[i] = SA::frag(As, wm * WM + i * 16, lane);
+            al[i] = SA::frag(As + SA::PLANE, wm * WM + i * 16, lane);
         }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
-            const int col = wn * WN + j * 16 + lrow;
-            bh[j] = *reinterpret_cast<const bf16x8 *>(&Bs[((buf * 2 + 0) * BN + col) * KP + 8 * lk]);
-            bl[j] = *reinterpret_cast<const bf16x8 *>(&Bs[((buf * 2 + 1) * BN + col) * KP + 8 * lk]);
-        }
+            const bf16x8 bh = SB::frag(Bs, wn * WN + j * 16, lane);
+            const bf16x8 bl = SB::frag(Bs + SB::PLANE, wn * WN + j * 16, lane);
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < FM; ++i) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
             }
-        if (kt + 1 < nk) store_tiles(buf ^ 1);
+        }
+        __syncthreads();
+        store_tiles();
         __syncthreads();
     }
 
@@ -535,34 +623,21 @@ int64_t inner_stride(const dcc_idxmap &m) {
     return m.s0;
 }
 
-KMap make_kmap(const dcc_idxmap &m, int bk = kBK) {
-    KMap k;
-    k.s0 = m.s0;
-    k.s1 = m.s1;
-    k.s2 = m.s2;
-    k.e1 = (uint32_t)m.e1;
-    k.e2 = (uint32_t)m.e2;
-    k.step = (int64_t)bk * m.s2;
-    k.bk = (uint32_t)bk;
-    k.carry1 = m.s1 - m.e2 * m.s2;
-    k.carry0 = m.s0 - m.e1 * m.s1;
-    return k;
-}
-
-template <typename T, int BM, int BN, bool C_MFAST>
+template <typename T, int BM, int BN, bool C_MFAST, int STAGES>
 void launch_variant(ContractArgs<T> &args, bool a_kfast, bool b_nfast, dim3 grid, dim3 block, hipStream_t stream) {
     if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, true, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, true, C_MFAST, STAGES>), grid, block, 0, stream, args);
     else if (a_kfast)
-        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, false, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, true, false, C_MFAST, STAGES>), grid, block, 0, stream, args);
     else if (b_nfast)
-        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, true, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, true, C_MFAST, STAGES>), grid, block, 0, stream, args);
     else
-        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, false, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_kernel<T, BM, BN, false, false, C_MFAST, STAGES>), grid, block, 0, stream, args);
 }
 
 template <typename T, int BM, int BN>
-int launch(ContractArgs<T> &args, int64_t max_rows, bool a_kfast, bool b_nfast, bool c_mfast, hipStream_t stream) {
+int launch(ContractArgs<T> &args, int64_t max_rows, bool a_kfast, bool b_nfast, bool c_mfast, int stages,
+           hipStream_t stream) {
     args.tiles_m = (int)((max_rows + BM - 1) / BM);
     args.tiles_n = (int)((args.N + BN - 1) / BN);
     const int64_t nbatch = args.total_tiles;  // caller passes nbatch here
@@ -572,10 +647,14 @@ int launch(ContractArgs<T> &args, int64_t max_rows, bool a_kfast, bool b_nfast, 
         return 2;
     }
     const dim3 grid((unsigned)args.total_tiles), block(kThreads);
-    if (c_mfast)
-        launch_variant<T, BM, BN, true>(args, a_kfast, b_nfast, grid, block, stream);
+    if (c_mfast && stages == 2)
+        launch_variant<T, BM, BN, true, 2>(args, a_kfast, b_nfast, grid, block, stream);
+    else if (c_mfast)
+        launch_variant<T, BM, BN, true, 1>(args, a_kfast, b_nfast, grid, block, stream);
+    else if (stages == 2)
+        launch_variant<T, BM, BN, false, 2>(args, a_kfast, b_nfast, grid, block, stream);
     else
-        launch_variant<T, BM, BN, false>(args, a_kfast, b_nfast, grid, block, stream);
+        launch_variant<T, BM, BN, false, 1>(args, a_kfast, b_nfast, grid, block, stream);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -624,43 +703,65 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     // C written along m (through LDS) unless it is contiguous along n
     const bool c_mfast = inner_stride(cn) != 1;
     hipStream_t s = dcc::as_stream(stream);
-    // Tile shape: DCC_CONTRACT_TILE=128x128|128x64|64x128|64x64 overrides the default.
+    // Workgroup tile: tile_shape 0..3 = 64x64, 128x64, 64x128, 128x128 with the two-stage register
+    // prefetch, 4..7 = the same shapes with one stage; -1 = 64x64 two-stage, or DCC_CONTRACT_TILE
+    // (e.g. "128x64" or "64x64s1") when set.
     static const char *tile_env = getenv("DCC_CONTRACT_TILE");
-    // 64x64 keeps four waves per SIMD resident, which hides HBM latency better
-    // than the larger register tiles (measured on MI355X for every triples GEMM).
-    int tbm = 64, tbn = 64;
+    int tbm = 64, tbn = 64, stages = 2;
     if (tile_shape >= 0) {
-        DCC_CHECK(tile_shape <= 3, "tile shape must be 0..3");
+        DCC_CHECK(tile_shape <= 7, "tile shape must be 0..7");
         const int shapes[4][2] = {{64, 64}, {128, 64}, {64, 128}, {128, 128}};
-        tbm = shapes[tile_shape][0];
-        tbn = shapes[tile_shape][1];
-    } else if (tile_env != nullptr && sscanf(tile_env, "%dx%d", &tbm, &tbn) != 2) {
-        tbm = 64;
-        tbn = 64;
+        tbm = shapes[tile_shape % 4][0];
+        tbn = shapes[tile_shape % 4][1];
+        stages = tile_shape < 4 ? 2 : 1;
+    } else if (tile_env != nullptr) {
+        int st = 2;
+        const int got = sscanf(tile_env, "%dx%ds%d", &tbm, &tbn, &st);
+        if (got < 2) {
+            tbm = 64;
+            tbn = 64;
+        }
+        stages = st == 1 ? 1 : 2;
     }
-    if (tbm == 128 && tbn == 128) return launch<T, 128, 128>(args, max_rows, a_kfast, b_nfast, c_mfast, s);
-    if (tbm == 128 && tbn == 64) return launch<T, 128, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, s);
-    if (tbm == 64 && tbn == 128) return launch<T, 64, 128>(args, max_rows, a_kfast, b_nfast, c_mfast, s);
-    return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, s);
+    if (tbm == 128 && tbn == 128) return launch<T, 128, 128>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
+    if (tbm == 128 && tbn == 64) return launch<T, 128, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
+    if (tbm == 64 && tbn == 128) return launch<T, 64, 128>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
+    return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
 }
 
-template <bool C_MFAST>
+template <int BM, int BN, bool C_MFAST>
 void launch_bf16x3(ContractArgs<float> &args, bool a_kfast, bool b_nfast, dim3 grid, dim3 block,
                    hipStream_t stream) {
     if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<64, 64, true, true, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST>), grid, block, 0, stream, args);
     else if (a_kfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<64, 64, true, false, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST>), grid, block, 0, stream, args);
     else if (b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<64, 64, false, true, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST>), grid, block, 0, stream, args);
     else
-        hipLaunchKernelGGL((contract_bf16x3_kernel<64, 64, false, false, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST>), grid, block, 0, stream, args);
+}
+
+template <int BM, int BN>
+int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool a_kfast, bool b_nfast,
+               bool c_mfast, hipStream_t s) {
+    args.tiles_m = (int)((max_rows + BM - 1) / BM);
+    args.tiles_n = (int)((args.N + BN - 1) / BN);
+    args.total_tiles = nbatch * args.tiles_m * args.tiles_n;
+    DCC_CHECK(args.total_tiles < ((int64_t)1 << 31), "too many tiles");
+    const dim3 grid((unsigned)args.total_tiles), block(kThreads);
+    if (c_mfast)
+        launch_bf16x3<BM, BN, true>(args, a_kfast, b_nfast, grid, block, s);
+    else
+        launch_bf16x3<BM, BN, false>(args, a_kfast, b_nfast, grid, block, s);
+    DCC_LAUNCH_CHECK();
+    return 0;
 }
 
 int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
                           dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn, float *C,
                           dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
-                          int64_t max_rows, dcc_stream_t stream) {
+                          int64_t max_rows, int64_t tile_shape, dcc_stream_t stream) {
     DCC_CHECK(M >= 0 && N >= 0 && K >= 0 && nbatch >= 0 && max_rows >= 0, "negative extent");
     DCC_CHECK(M < ((int64_t)1 << 31) && N < ((int64_t)1 << 31) && K < ((int64_t)1 << 31),
               "extent exceeds 2^31");
@@ -668,6 +769,7 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
                   valid_map(cn),
               "invalid index map");
     DCC_CHECK(ak.table == nullptr && bk.table == nullptr, "K index maps must be mixed-radix maps");
+    DCC_CHECK(tile_shape >= -1 && tile_shape <= 3, "tile shape must be -1..3");
     if (M == 0 || N == 0 || nbatch == 0 || max_rows == 0) return 0;
     DCC_CHECK(A != nullptr && B != nullptr && C != nullptr && batches != nullptr, "null pointer");
     ContractArgs<float> args;
@@ -683,24 +785,22 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
     args.bn = bn;
     args.cm = cm;
     args.cn = cn;
-    args.ak = make_kmap(ak, kBK3);
-    args.bk = make_kmap(bk, kBK3);
+    args.ak = make_kmap(ak);
+    args.bk = make_kmap(bk);
     args.batches = batches;
-    args.tiles_m = (int)((max_rows + 63) / 64);
-    args.tiles_n = (int)((N + 63) / 64);
-    args.total_tiles = nbatch * args.tiles_m * args.tiles_n;
-    DCC_CHECK(args.total_tiles < ((int64_t)1 << 31), "too many tiles");
     const bool a_kfast = inner_stride(ak) == 1 || inner_stride(am) != 1;
     const bool b_nfast = inner_stride(bn) == 1 || inner_stride(bk) != 1;
     const bool c_mfast = inner_stride(cn) != 1;
-    const dim3 grid((unsigned)args.total_tiles), block(kThreads);
     hipStream_t s = dcc::as_stream(stream);
-    if (c_mfast)
-        launch_bf16x3<true>(args, a_kfast, b_nfast, grid, block, s);
-    else
-        launch_bf16x3<false>(args, a_kfast, b_nfast, grid, block, s);
-    DCC_LAUNCH_CHECK();
-    return 0;
+    // default: 128x128 unless one side of the problem is small
+    int shape = (int)tile_shape;
+    if (shape < 0) shape = (max_rows <= 64 ? 0 : 1) + (N <= 64 ? 0 : 2);
+    switch (shape) {
+        case 0: return run_bf16x3<64, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, s);
+        case 1: return run_bf16x3<128, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, s);
+        case 2: return run_bf16x3<64, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, s);
+        default: return run_bf16x3<128, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, s);
+    }
 }
 
 }  // namespace
@@ -711,7 +811,16 @@ extern "C" int dcc_contract_f32_bf16x3(int64_t M, int64_t N, int64_t K, double a
                                        const dcc_batch *batches, int64_t nbatch, int64_t max_rows,
                                        dcc_stream_t stream) {
     return contract_bf16x3_entry(M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
-                                 stream);
+                                 -1, stream);
+}
+
+extern "C" int dcc_contract_tiled_f32_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, double beta,
+                                             const float *A, dcc_idxmap am, dcc_idxmap ak, const float *B,
+                                             dcc_idxmap bk, dcc_idxmap bn, float *C, dcc_idxmap cm, dcc_idxmap cn,
+                                             const dcc_batch *batches, int64_t nbatch, int64_t max_rows,
+                                             int64_t tile_shape, dcc_stream_t stream) {
+    return contract_bf16x3_entry(M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
+                                 tile_shape, stream);
 }
 
 extern "C" int dcc_contract_f64(int64_t M, int64_t N, int64_t K, double alpha, double beta, const double *A,

@@ -28,42 +28,6 @@ __device__ __forceinline__ int64_t gmap(const dcc_idxmap &mp, uint32_t x, int64_
     return (int64_t)q0 * mp.s0 + (int64_t)q1 * mp.s1 + (int64_t)q2 * mp.s2;
 }
 
-// incremental K map (as in dcc_contract.hip)
-struct GKIter {
-    uint32_t q1, q2, e1, e2;
-    int64_t off, s1, s2, carry1, carry0;
-};
-
-__device__ __forceinline__ GKIter gkinit(const dcc_idxmap &m, uint32_t k) {
-    GKIter it;
-    it.e1 = (uint32_t)m.e1;
-    it.e2 = (uint32_t)m.e2;
-    it.s1 = m.s1;
-    it.s2 = m.s2;
-    it.carry1 = m.s1 - m.e2 * m.s2;
-    it.carry0 = m.s0 - m.e1 * m.s1;
-    const uint32_t e12 = it.e1 * it.e2;
-    const uint32_t q0 = k / e12;
-    const uint32_t r0 = k - q0 * e12;
-    it.q1 = r0 / it.e2;
-    it.q2 = r0 - it.q1 * it.e2;
-    it.off = (int64_t)q0 * m.s0 + (int64_t)it.q1 * m.s1 + (int64_t)it.q2 * m.s2;
-    return it;
-}
-
-__device__ __forceinline__ void gkadvance(GKIter &it) {
-    it.q2 += kBK;
-    it.off += (int64_t)kBK * it.s2;
-    while (it.q2 >= it.e2) {
-        it.q2 -= it.e2;
-        it.off += it.carry1;
-        if (++it.q1 >= it.e1) {
-            it.q1 = 0;
-            it.off += it.carry0;
-        }
-    }
-}
-
 template <int BM, int BN>
 __global__ __launch_bounds__(kThreads) void grouped_kernel(const dcc_gemm_problem *__restrict__ probs, int nprob,
                                                            int64_t total_tiles) {
@@ -135,20 +99,27 @@ __global__ __launch_bounds__(kThreads) void grouped_kernel(const dcc_gemm_proble
             b_ok |= (ok ? 1u : 0u) << r;
             b_col[r] = ok ? bt.b_off + gmap(p.bn, (uint32_t)n, batch) : 0;
         }
-        GKIter ka = gkinit(p.ak, (uint32_t)a_kk);
-        GKIter kb = gkinit(p.bk, (uint32_t)b_kk);
+        const dcc::KMap akm = dcc::make_kmap(p.ak), bkm = dcc::make_kmap(p.bk);
         double a_reg[A_PER_T], b_reg[B_PER_T];
+        // branch-free loads: out-of-range elements read offset 0 and are replaced by zero
         auto load_tiles = [&](int k0) {
-            const bool ak_ok = k0 + a_kk < K;
-            const bool bk_ok = k0 + b_kk < K;
+            const uint32_t ka = (uint32_t)(k0 + a_kk), kb = (uint32_t)(k0 + b_kk);
+            const bool ak_ok = ka < (uint32_t)K;
+            const bool bk_ok = kb < (uint32_t)K;
+            const int64_t aoff = dcc::koff(akm, ak_ok ? ka : 0u);
+            const int64_t boff = dcc::koff(bkm, bk_ok ? kb : 0u);
 #pragma unroll
-            for (int r = 0; r < A_PER_T; ++r)
-                a_reg[r] = (ak_ok && ((a_ok >> r) & 1u)) ? A[a_row[r] + ka.off] : 0.0;
+            for (int r = 0; r < A_PER_T; ++r) {
+                const bool ok = ak_ok && ((a_ok >> r) & 1u);
+                const double x = A[ok ? a_row[r] + aoff : 0];
+                a_reg[r] = ok ? x : 0.0;
+            }
 #pragma unroll
-            for (int r = 0; r < B_PER_T; ++r)
-                b_reg[r] = (bk_ok && ((b_ok >> r) & 1u)) ? B[b_col[r] + kb.off] : 0.0;
-            gkadvance(ka);
-            gkadvance(kb);
+            for (int r = 0; r < B_PER_T; ++r) {
+                const bool ok = bk_ok && ((b_ok >> r) & 1u);
+                const double x = B[ok ? b_col[r] + boff : 0];
+                b_reg[r] = ok ? x : 0.0;
+            }
         };
         auto store_tiles = [&](int buf) {
 #pragma unroll
@@ -168,7 +139,7 @@ __global__ __launch_bounds__(kThreads) void grouped_kernel(const dcc_gemm_proble
         __syncthreads();
         for (int kt = 0; kt < nk; ++kt) {
             const int buf = kt & 1;
-            if (kt + 1 < nk) load_tiles((kt + 1) * kBK);
+            load_tiles((kt + 1) * kBK);  // past the last tile: offset-0 reads into the idle buffer
 #pragma unroll
             for (int ks = 0; ks < kBK / 4; ++ks) {
                 const int kr = ks * 4 + lk;
@@ -183,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void grouped_kernel(const dcc_gemm_proble
                     for (int j = 0; j < FN; ++j)
                         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
             }
-            if (kt + 1 < nk) store_tiles(buf ^ 1);
+            store_tiles(buf ^ 1);
             __syncthreads();
         }
         if (c_mfast) {

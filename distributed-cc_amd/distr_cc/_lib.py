@@ -87,6 +87,8 @@ _SIGS = {
                                 c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_contract_f32_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap, IdxMap,
                                  c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_vp], ctypes.c_int),
+    "dcc_contract_tiled_f32_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap,
+                                       IdxMap, c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_transform_f64": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_transform_f32": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_r3_finalize_tiles": ([c_i64], c_i64),

@@ -113,11 +113,11 @@ class GemmLaunch:
         self.max_rows = int(max_rows)
         self.flops = float(flops)
         self.keep = keep  # device tables referenced by maps
-        self.tile = -1    # -1: library default; 0..3: 64x64, 128x64, 64x128, 128x128 (autotuned)
+        self.tile = -1    # -1: library default; 0..7: dcc_contract_tiled_* tile_shape (autotuned)
 
     def run(self, stream):
         s = stream.handle if stream is not None else None
-        if self.tile >= 0 and self.fn in ("dcc_contract_f64", "dcc_contract_f32"):
+        if self.tile >= 0:
             L.call(self.fn.replace("contract_", "contract_tiled_"), *self.args, self.batches.vp, self.nbatch,
                    self.max_rows, self.tile, s)
         else:
@@ -673,12 +673,13 @@ class TriplesEngine:
         return self.read_norms()
 
     def autotune(self, reps=2, include_lower=True):
-        """Time the four workgroup tiles for every contraction launch and keep the fastest.
+        """Time the workgroup tiles of every contraction launch and keep the fastest: eight for the
+        FP64/FP32 kernels (four shapes x one- or two-stage prefetch), four shapes for split-BF16x3.
 
         Run it before a residual (it accumulates into R and the scratch buffers); returns
-        {launch name: (tile, ms)}.  The split-BF16x3 launches have one tile and are skipped."""
+        {launch name: (tile, ms)}."""
         from distr_cc.device import Event
-        shapes = ("64x64", "128x64", "64x128", "128x128")
+        shapes = ("64x64", "128x64", "64x128", "128x128", "64x64s1", "128x64s1", "64x128s1", "128x128s1")
         launches = list(self.launches)
         if include_lower and self._lower is not None:
             for chunk in self._lower['chunks']:
@@ -686,10 +687,10 @@ class TriplesEngine:
         out = {}
         st = self.stream
         for n, g in enumerate(launches):
-            if not isinstance(g, GemmLaunch) or g.fn not in ("dcc_contract_f64", "dcc_contract_f32"):
+            if not isinstance(g, GemmLaunch):
                 continue
             best = None
-            for tile in range(4):
+            for tile in range(4 if g.fn.endswith("bf16x3") else len(shapes)):
                 g.tile = tile
                 g.run(st)
                 a, b = Event(), Event()

@@ -113,7 +113,8 @@ int dcc_contract_f32(int64_t M, int64_t N, int64_t K, double alpha, double beta,
 
 /*
  * dcc_contract_f64 / _f32 with an explicit workgroup tile: tile_shape 0 = 64x64, 1 = 128x64,
- * 2 = 64x128, 3 = 128x128 (M x N); chosen per launch by the engines' autotuner.
+ * 2 = 64x128, 3 = 128x128 (M x N) with a two-stage register prefetch of the K tiles, 4..7 = the
+ * same shapes with a one-stage prefetch; chosen per launch by the engines' autotuner.
  */
 int dcc_contract_tiled_f64(int64_t M, int64_t N, int64_t K, double alpha, double beta, const double *A,
                            dcc_idxmap am, dcc_idxmap ak, const double *B, dcc_idxmap bk, dcc_idxmap bn, double *C,
@@ -136,6 +137,13 @@ int dcc_contract_f32_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, doubl
                             dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn, float *C,
                             dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
                             int64_t max_rows, dcc_stream_t stream);
+/* dcc_contract_f32_bf16x3 with an explicit workgroup tile: tile_shape 0 = 64x64, 1 = 128x64,
+ * 2 = 64x128, 3 = 128x128 (M x N); -1 picks 128 on each side unless that side has at most 64 rows.
+ * No reference counterpart (as dcc_contract_f32_bf16x3). */
+int dcc_contract_tiled_f32_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
+                                  dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn,
+                                  float *C, dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
+                                  int64_t max_rows, int64_t tile_shape, dcc_stream_t stream);
 
 /* ---------------------------------------- S3 block transforms (HBM-bound) */
 

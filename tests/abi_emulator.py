@@ -192,6 +192,9 @@ class Emulator:
     def dcc_contract_f32_bf16x3(self, *a):
         return self._contract("bf16x3", *a)
 
+    def dcc_contract_tiled_f32_bf16x3(self, *a):
+        return self._contract("bf16x3", *a[:-2], a[-1])
+
     def _contract(self, dt, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
                   stream):
         split = dt == "bf16x3"

@@ -241,6 +241,46 @@ def test_contract_bf16x3_plain_gemm(dev, M, N, K):
     assert abs(got - exact).max() < 1e-4 * scale * np.sqrt(K)         # split error ~2^-16 per product
 
 
+@pytest.mark.parametrize("tile", [0, 1, 2, 3])
+@pytest.mark.parametrize("layout", ["kfast-nfast", "mfast-nfast", "kfast-kfast", "mfast-kfast-ct"])
+def test_contract_bf16x3_layouts(dev, layout, tile):
+    """Every staging layout of the split-BF16x3 kernel (A contiguous along k or m, B along n or k),
+    the LDS-staged epilogue (C contiguous along m), partial M/N tiles, a K tail and a batch offset,
+    for each workgroup tile; checked against the same split evaluated exactly on the host."""
+    from abi_emulator import bf16_split
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    M, N, K = 150, 83, 75
+    _say("contract bf16x3 layout %s tile %d" % (layout, tile))
+    rng = np.random.default_rng(11 + tile)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    C0 = rng.normal(size=(M, N)).astype(np.float32)
+    lin = lambda s: L.IdxMap(None, 0, 1, 1 << 20, 0, 0, s)
+    a_mfast, b_kfast, c_mt = layout.startswith("mfast"), "-kfast" in layout, layout.endswith("-ct")
+    pad = 7   # batch offset of every operand
+    hA = np.concatenate([np.zeros(pad, np.float32), (A.T if a_mfast else A).ravel()])
+    hB = np.concatenate([np.zeros(pad, np.float32), (B.T if b_kfast else B).ravel()])
+    hC = np.concatenate([np.zeros(pad, np.float32), (C0.T if c_mt else C0).ravel()])
+    dA, dB, dC = DeviceArray.from_host(hA), DeviceArray.from_host(hB), DeviceArray.from_host(hC)
+    am, ak = (lin(1), lin(M)) if a_mfast else (lin(K), lin(1))
+    bk, bn = (lin(1), lin(K)) if b_kfast else (lin(N), lin(1))
+    cm, cn = (lin(1), lin(M)) if c_mt else (lin(N), lin(1))
+    bt = np.zeros(1, dtype=L.BATCH_DTYPE)
+    bt[0] = (pad, pad, pad, 0, M)
+    dbt = DeviceArray.from_host(bt)
+    L.call("dcc_contract_tiled_f32_bf16x3", M, N, K, 1.5, 0.5, dA.vp, am, ak, dB.vp, bk, bn, dC.vp, cm, cn,
+           dbt.vp, 1, M, tile, None)
+    out = dC.download()[pad:].astype(np.float64)
+    got = out.reshape(N, M).T if c_mt else out.reshape(M, N)
+    ah, al = bf16_split(A)
+    bh, bl = bf16_split(B)
+    split_ref = 0.5 * C0.astype(np.float64) + 1.5 * (ah @ bh + ah @ bl + al @ bh)
+    scale = max(1.0, abs(split_ref).max())
+    assert abs(got - split_ref).max() < 3e-6 * scale * np.sqrt(K)
+    assert np.all(dC.download()[:pad] == 0)
+
+
 @pytest.mark.parametrize("o,v", [(3, 9), (5, 17)])
 def test_residual_bf16x3_vs_oracle(dev, o, v):
     from distr_cc.synthetic import SyntheticSystem
