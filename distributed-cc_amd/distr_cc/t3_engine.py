@@ -4,7 +4,8 @@ the T3 -> r1/r2/W contributions, as launches of libdcc kernels.
 Data in HBM (FP64, C order):
   t3, R          (n3, v, v, v)     canonical blocks, lexicographic i<=j<=k
   Y              (npairs, o, v, v, v)  Y[(p,q)][l] = T[l,p,q] for p<=q (ring B operand)
-  WR1/WR3a/WR3b  (o, v, o, v)      ring weights, rows (r,x), cols (l,d)
+  Z              (npairs, o, v, v, v)  Y[l][b,d,c] + Y[l][c,b,d] stored as [d,b,c], right after Y
+  WR1, WR3[2]    (o, v, o, v)      ring weights, rows (r,x), cols (l,d)
   pfs            (6, n3, n3)       W_oooo/F_oo prefactors
   intermediates  shapes of distr_cc/rccsdt.py:380,445,625
 
@@ -18,7 +19,8 @@ batched gather-GEMMs (dcc_contract_f64):
                         - 1/2 (W_ovov[l,a,r,d] + W_ovvo[l,a,d,r]) (Y[l][b,d,c] + Y[l][c,b,d]) )
     part 3 of target (i,j,k) is G[i;jk], part 4 is G[j;ik] with (a,b) swapped and
     part 5 is G[k;ij] read as [c,a,b]; one launch per (part, term), batched over
-    the pairs p<=q with rows (r, x) restricted to the part's range of r;
+    the pairs p<=q with rows (r, x) restricted to the part's range of r; with Z the
+    last three terms are one launch over K = (term, l, d) with B = [Y; Z];
   * W_oooo/F_oo x T3 (rccsdt.py:616-732): 6 GEMMs M = n3 targets, K = n3 sources,
     N = v^3, with the prefactors built on the device.
 Then dcc_r3_finalize_f64 fuses rccsdt.py:947-971.
@@ -244,11 +246,16 @@ def s3_jobs_array(jobs):
 class TriplesEngine:
     """Holds the device state of one RCCSDT triples update for (nocc, nvir)."""
 
-    def __init__(self, nocc, nvir, stream=None, device=0, dtype=np.float64, contract="native"):
+    def __init__(self, nocc, nvir, stream=None, device=0, dtype=np.float64, contract="native", ring_fused=None):
         """``dtype`` float64 is the plain FP64 path; float32 stores T3, R, Y and the
         intermediates in FP32 and contracts on FP32 MFMA, keeping the denominator,
         the projection arithmetic and the norms in FP64.  ``contract="bf16x3"`` (FP32 storage
-        only) runs every contraction as split-BF16x3 products on BF16 MFMA instead."""
+        only) runs every contraction as split-BF16x3 products on BF16 MFMA instead.
+
+        ``ring_fused`` (default: when the device has room for a second Y-sized buffer) stores
+        Z[(p,q)][l][d,b,c] = Y[l][b,d,c] + Y[l][c,b,d] next to Y, so the two terms that share the
+        W_ovov + W_ovvo weight become one, fused along K with the W_ovvo term: three ring
+        GEMMs per part instead of five, and 4/5 of the ring flops."""
         device_init(device)
         o, v = int(nocc), int(nvir)
         self.nocc, self.nvir = o, v
@@ -264,7 +271,17 @@ class TriplesEngine:
         # --- device buffers ---
         self.t3 = DeviceArray((n3, v, v, v), dt)
         self.R = DeviceArray((n3, v, v, v), dt)
-        self.Y = DeviceArray((self.npairs, o, v, v, v), dt)
+        ysize = self.npairs * o * v3
+        if ring_fused is None:
+            from distr_cc.device import mem_get_info
+            free = mem_get_info()[0]
+            # room for Z plus the rest of the engine (W_oooo scratch, lower-order chunks) and a margin
+            ring_fused = free > (ysize * 2 + n3 * v3 * 2) * dt.itemsize + (24 << 30)
+        self.ring_fused = bool(ring_fused)
+        self.Yz = DeviceArray((2 if self.ring_fused else 1, self.npairs, o, v, v, v), dt)
+        self.Y = DeviceArray((self.npairs, o, v, v, v), dt, ptr=self.Yz.ptr, owner=False)
+        self.Z = (DeviceArray((self.npairs, o, v, v, v), dt, ptr=self.Yz.ptr + ysize * dt.itemsize, owner=False)
+                  if self.ring_fused else None)
         self.t2 = DeviceArray((o, o, v, v), dt)
         self.F_oo = DeviceArray((o, o), dt)
         self.F_vv = DeviceArray((v, v), dt)
@@ -277,8 +294,10 @@ class TriplesEngine:
         self.W_oovo = DeviceArray((o, o, v, o), dt)   # W_vooo[a,l,i,j] stored as [i,j,a,l]
         self.W_vvvv = DeviceArray((v, v, v, v), dt)
         self.WR1 = DeviceArray((o, v, o, v), dt)
-        self.WR3a = DeviceArray((o, v, o, v), dt)
-        self.WR3b = DeviceArray((o, v, o, v), dt)
+        self.WR3 = DeviceArray((2, o, v, o, v), dt)   # [W_ovvo term, (W_ovov + W_ovvo) term]
+        w3 = (o * v) ** 2 * dt.itemsize
+        self.WR3a = DeviceArray((o, v, o, v), dt, ptr=self.WR3.ptr, owner=False)
+        self.WR3b = DeviceArray((o, v, o, v), dt, ptr=self.WR3.ptr + w3, owner=False)
         self.pfs = DeviceArray((6, n3, n3), dt)
         self.Xp = DeviceArray((n3, v, v, v), dt)     # one W_oooo/F_oo term before its permutation
         self.eia = DeviceArray((o, v))
@@ -295,6 +314,18 @@ class TriplesEngine:
                 jobs.append(((pq * o + l) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
         self.y_jobs = DeviceArray.from_host(s3_jobs_array(jobs))
         self.n_y_jobs = len(jobs)
+        # --- Z jobs: Z[(p,q)][l] = Y[(p,q)][l].transpose(1,0,2) + Y[(p,q)][l].transpose(2,1,0), from T ---
+        if self.ring_fused:
+            jobs = []
+            for pq, (p, q) in enumerate(self.pairs):
+                for l in range(o):
+                    case, (ci, cj, ck) = canonical_case(l, p, q)
+                    a = TP_T3[case]
+                    coef = np.zeros(6)
+                    for b in ((1, 0, 2), (2, 1, 0)):
+                        coef[TP_T3.index((a[b[0]], a[b[1]], a[b[2]]))] += 1.0
+                    jobs.append(((pq * o + l) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
+            self.z_jobs = DeviceArray.from_host(s3_jobs_array(jobs))
         if contract not in ("native", "bf16x3") or (contract == "bf16x3" and dt != np.float32):
             raise ValueError("contract must be 'native', or 'bf16x3' with float32 storage")
         self.contract = contract
@@ -387,10 +418,16 @@ class TriplesEngine:
         terms = [
             ('WR1', 1, (0, 2), v, v2, 1),    # -W_ovov[l,b,r,d] Y[l][a,d,c]
             ('WR1', 2, (0, 1), 1, v2, v),    # -W_ovov[l,c,r,d] Y[l][a,b,d]
-            ('WR3a', 0, (1, 2), v2, v, 1),   #  W_ovvo[l,a,d,r] Y[l][d,b,c]
-            ('WR3b', 0, (1, 2), v, v2, 1),   # -1/2(..) Y[l][b,d,c]
-            ('WR3b', 0, (1, 2), 1, v, v2),   # -1/2(..) Y[l][c,b,d]
         ]
+        if self.ring_fused:
+            # W_ovvo[l,a,d,r] Y[l][d,b,c] - 1/2(..) Z[l][d,b,c] as one GEMM over K = (term, l, d)
+            terms.append(('WR3', 0, (1, 2), v2, v, 1))
+        else:
+            terms += [
+                ('WR3a', 0, (1, 2), v2, v, 1),   #  W_ovvo[l,a,d,r] Y[l][d,b,c]
+                ('WR3b', 0, (1, 2), v, v2, 1),   # -1/2(..) Y[l][b,d,c]
+                ('WR3b', 0, (1, 2), 1, v, v2),   # -1/2(..) Y[l][c,b,d]
+            ]
         launches = []
         for part in (3, 4, 5):
             S = place[part]
@@ -410,16 +447,18 @@ class TriplesEngine:
                 tdev = DeviceArray.from_host(tbl)
                 self._tables.append(tdev)
                 W = getattr(self, wname)
+                nt = 2 if wname == 'WR3' else 1                  # terms fused along K
                 am = L.IdxMap(None, 0, 1, ov, 0, 0, ov)          # row (r,x) of the (ov x ov) weight
-                ak = L.IdxMap(None, 0, 1, ov, 0, 0, 1)
-                bk = L.IdxMap(None, 0, o, v, 0, v3, sd)          # (l, d)
+                ak = L.IdxMap(None, 0, 1, ov, ov * ov, 0, 1)     # (term, (l,d))
+                zoff = self.npairs * o * v3                      # Z follows Y in one buffer
+                bk = L.IdxMap(None, 0, o, v, zoff, v3, sd)       # (term, l, d)
                 bn = L.IdxMap(None, 0, v, v, 0, sn0, sn1)        # (n_outer, n_inner)
                 cm = table_map(tdev, ov)
                 cn = L.IdxMap(None, 0, v, v, 0, S[n0], S[n1])
                 nrows = sum(b[4] - b[3] for b in batches)
-                launches.append(GemmLaunch('ring part%d %s M%d' % (part, wname, mx), ov, v2, ov, 1.0, 1.0,
+                launches.append(GemmLaunch('ring part%d %s M%d' % (part, wname, mx), ov, v2, nt * ov, 1.0, 1.0,
                                            W, am, ak, self.Y, bk, bn, self.R, cm, cn, batches,
-                                           max(b[4] - b[3] for b in batches), 2.0 * nrows * v2 * ov,
+                                           max(b[4] - b[3] for b in batches), 2.0 * nrows * v2 * ov * nt,
                                            keep=(tdev,)))
         return launches
 
@@ -631,6 +670,9 @@ class TriplesEngine:
         s = self.stream.handle if self.stream is not None else None
         L.call("dcc_s3_transform_" + self.sfx, self.Y.vp, self.t3.vp, self.nvir, 0.0, self.y_jobs.vp,
                self.n_y_jobs, s)
+        if self.ring_fused:
+            L.call("dcc_s3_transform_" + self.sfx, self.Z.vp, self.t3.vp, self.nvir, 0.0, self.z_jobs.vp,
+                   self.n_y_jobs, s)
 
     def residual(self, timer=None):
         """R <- r3 of distr_cc/rccsdt.py:368-433 for the current t3 and intermediates.

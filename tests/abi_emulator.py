@@ -148,6 +148,11 @@ class Emulator:
     def dcc_free(self, p):
         return 0
 
+    def dcc_mem_get_info(self, pfree, ptotal):
+        # an MI355X-sized device: 288 GB, all of it free
+        pfree._obj.value = ptotal._obj.value = 288 << 30
+        return 0
+
     def dcc_memcpy_htod(self, dst, src, nbytes, stream):
         d = self.dev.view(_ptr(dst), np.uint8, int(nbytes))
         d[:] = np.ctypeslib.as_array(ctypes.cast(src, ctypes.POINTER(ctypes.c_uint8)), (int(nbytes),))
