@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--breakdown", action="store_true", help="print per-launch timings to stderr")
     ap.add_argument("--autotune", type=int, default=1, help="autotune the contraction tiles first (0 = no)")
+    ap.add_argument("--tiles", metavar="JSON", default=None,
+                    help="tile choice file: loaded (no autotune) if it exists, else written after autotuning")
     ap.add_argument("--mixed-iters", type=int, default=3,
                     help="full mixed-vs-FP64 iterations to time and compare (0 = skip)")
     ap.add_argument("--adjoint", type=int, default=1, help="time the T2->T3 VJP (0 = skip)")
@@ -349,9 +351,14 @@ def main():
             ev.setdefault(id(g), []).append((what, e))
 
     tuned = None
-    if args.autotune:
+    if args.tiles and os.path.exists(args.tiles):
+        eng.set_tile_config(json.load(open(args.tiles)))
+        print("bench: tiles loaded from %s" % args.tiles, file=sys.stderr, flush=True)
+    elif args.autotune:
         print("bench: autotuning contraction tiles", file=sys.stderr, flush=True)
         tuned = eng.autotune()
+        if args.tiles:
+            json.dump(eng.tile_config(), open(args.tiles, "w"))
     print("bench: warmup %d" % args.warmup, file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         step()

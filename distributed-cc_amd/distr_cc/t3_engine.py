@@ -714,6 +714,26 @@ class TriplesEngine:
         self.finalize_launch(r3_out)
         return self.read_norms()
 
+    def _tunable(self, include_lower=True):
+        launches = list(self.launches)
+        if include_lower and self._lower is not None:
+            for chunk in self._lower['chunks']:
+                launches += [g for g in chunk[2] + chunk[3] if isinstance(g, GemmLaunch)]
+        return launches
+
+    def tile_config(self):
+        """Workgroup tile of every contraction launch, in autotune order (None for other launches)."""
+        return [g.tile if isinstance(g, GemmLaunch) else None for g in self._tunable()]
+
+    def set_tile_config(self, tiles):
+        """Apply a tile_config() list (for example one saved from an earlier autotune)."""
+        launches = self._tunable()
+        if len(tiles) != len(launches):
+            raise ValueError("tile list has %d entries, the engine %d launches" % (len(tiles), len(launches)))
+        for g, t in zip(launches, tiles):
+            if isinstance(g, GemmLaunch) and t is not None:
+                g.tile = int(t)
+
     def autotune(self, reps=2, include_lower=True):
         """Time the workgroup tiles of every contraction launch and keep the fastest: eight for the
         FP64/FP32 kernels (four shapes x one- or two-stage prefetch), four shapes for split-BF16x3.
@@ -722,10 +742,7 @@ class TriplesEngine:
         {launch name: (tile, ms)}."""
         from distr_cc.device import Event
         shapes = ("64x64", "128x64", "64x128", "128x128", "64x64s1", "128x64s1", "64x128s1", "128x128s1")
-        launches = list(self.launches)
-        if include_lower and self._lower is not None:
-            for chunk in self._lower['chunks']:
-                launches += [g for g in chunk[2] + chunk[3] if isinstance(g, GemmLaunch)]
+        launches = self._tunable(include_lower)
         out = {}
         st = self.stream
         for n, g in enumerate(launches):
