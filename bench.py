@@ -11,6 +11,7 @@ in HBM before the timed region.  Prints one JSON line.
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -83,27 +84,54 @@ def measured_traffic(o, v):
     return None, None
 
 
-def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01):
+def _tiles_from_small(dt, contract, seed, sigma_b, o=10, v=90):
+    """Autotune the engine of one precision at nocc=10, nvir=90 and return its {launch name: tile}."""
+    from distr_cc.device import Stream
+    from distr_cc.rccsdt_engine import RCCSDTEngine
+    from distr_cc.synthetic import SyntheticSystem
+    sysm = SyntheticSystem(o, v, naux=300, seed=seed, sigma_b=sigma_b)
+    st = Stream()
+    eng = RCCSDTEngine(o, v, sysm.fock, sysm.pppp(), sysm.mo_energy, stream=st, t3_dtype=dt,
+                       triples_contract=contract)
+    eng.set_amplitudes(np.zeros((o, v)), sysm.t2(), sysm.t3_tri())
+    eng.update_amps()
+    eng.tr.autotune()
+    st.synchronize()
+    tiles = eng.tr.tiles_by_name()
+    del eng
+    gc.collect()
+    return tiles
+
+
+def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01, autotune="direct"):
     """Full device RCCSDT iterations (T1 dressing, lower order, T3 -> r1/r2/W, triples residual,
-    update; no DIIS) from the synthetic amplitudes, once with FP64 triples and once with FP32 triples
-    (FP32 MFMA contractions, FP64 elsewhere).  Returns per-iteration |E_mixed - E_fp64| and the mean
-    time of one iteration in each precision (HIP events on the engine stream)."""
+    update; no DIIS) from the synthetic amplitudes with FP64, FP32 and split-BF16x3 triples
+    contractions (FP64 elsewhere).  Returns per-iteration |E_mixed - E_fp64| and the mean time of
+    one iteration in each precision (HIP events on the engine stream).  ``autotune``: "direct"
+    tunes each engine, "transfer" applies the tiles tuned at nocc=10, nvir=90 in the same
+    precision (by launch name), "none" keeps the library defaults."""
     from distr_cc.device import Event, Stream
     from distr_cc.rccsdt_engine import RCCSDTEngine
     from distr_cc.synthetic import SyntheticSystem
     sysm = SyntheticSystem(o, v, naux=300, seed=seed, sigma_b=sigma_b)
     pppp = sysm.pppp()
     t1, t2, t3 = np.zeros((o, v)), sysm.t2(), sysm.t3_tri()
+    print("bench: mixed inputs ready (T3 %.1f GB)" % (t3.nbytes / 1e9), file=sys.stderr, flush=True)
     out = {}
     for name, dt, contract in (("fp64", np.float64, "native"), ("fp32", np.float32, "native"),
                                ("bf16x3", np.float32, "bf16x3")):
+        print("bench: mixed %s engine nocc=%d nvir=%d" % (name, o, v), file=sys.stderr, flush=True)
         st = Stream()
         eng = RCCSDTEngine(o, v, sysm.fock, pppp, sysm.mo_energy, stream=st, t3_dtype=dt,
                            triples_contract=contract)
         eng.set_amplitudes(t1, t2, t3)
-        eng.tr.autotune()
+        if autotune == "direct":
+            eng.tr.autotune()
+        elif autotune == "transfer":
+            eng.tr.set_tiles_by_name(_tiles_from_small(dt, contract, seed, sigma_b))
         st.synchronize()
         eng.update_amps()       # warm-up iteration (first-launch costs), then restart
+        print("bench: mixed %s warm-up done" % name, file=sys.stderr, flush=True)
         eng.set_amplitudes(t1, t2, t3)
         energies, times = [eng.energy()], []
         phase_ev = []
@@ -113,13 +141,15 @@ def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01):
             e.record(st)
             phase_ev.append((ph, what, e))
 
-        for _ in range(iters):
+        for it in range(iters):
             a, b = Event(), Event()
             a.record(st)
             eng.update_amps(timer=timer)
             b.record(st)
             times.append(a.elapsed_ms(b))
             energies.append(eng.energy())
+            print("bench: mixed %s iteration %d: %.1f ms, E_corr %.12f" % (name, it, times[-1], energies[-1]),
+                  file=sys.stderr, flush=True)
         phases = {}
         starts = {}
         for ph, what, e in phase_ev:
@@ -128,11 +158,14 @@ def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01):
             else:
                 phases.setdefault(ph, []).append(starts[ph].elapsed_ms(e))
         out[name] = (energies, times, {k: float(np.mean(x)) for k, x in phases.items()})
-        del eng
+        del eng, timer, phase_ev
+        gc.collect()   # the engine's programs hold reference cycles; free its device memory now
     e64 = out["fp64"][0]
     t64 = float(np.mean(out["fp64"][1]))
     res = {"workload": "synthetic RCCSDT iterations nocc=%d nvir=%d, B-factor sigma=%g (weakly correlated, "
-                       "so the Jacobi iteration converges), no DIIS, %d iterations" % (o, v, sigma_b, iters),
+                       "so the Jacobi iteration converges), no DIIS, %d iterations, tiles %s"
+                       % (o, v, sigma_b, iters, {"direct": "autotuned", "none": "library defaults",
+                                                 "transfer": "autotuned at nocc=10 nvir=90"}[autotune]),
            "fp64_ms_per_iteration": t64, "e_corr_fp64": e64, "fp64_phase_ms": out["fp64"][2]}
     for name in ("fp32", "bf16x3"):
         de = [abs(x - y) for x, y in zip(out[name][0], e64)]
@@ -276,8 +309,12 @@ def main():
     ap.add_argument("--autotune", type=int, default=1, help="autotune the contraction tiles first (0 = no)")
     ap.add_argument("--tiles", metavar="JSON", default=None,
                     help="tile choice file: loaded (no autotune) if it exists, else written after autotuning")
-    ap.add_argument("--mixed-iters", type=int, default=3,
+    ap.add_argument("--mixed-iters", type=int, default=2,
                     help="full mixed-vs-FP64 iterations to time and compare (0 = skip)")
+    ap.add_argument("--mixed-nocc", type=int, default=16)
+    ap.add_argument("--mixed-nvir", type=int, default=160)
+    ap.add_argument("--mixed-autotune", default="transfer", choices=("direct", "transfer", "none"),
+                    help="tiles of the mixed-vs-FP64 engines: tuned on them, tuned at nocc=10 nvir=90, or defaults")
     ap.add_argument("--adjoint", type=int, default=1, help="time the T2->T3 VJP (0 = skip)")
     ap.add_argument("--batched", type=int, default=96, help="systems in the batched ensemble (0 = skip)")
     ap.add_argument("--q-nocc", type=int, default=5)
@@ -429,8 +466,9 @@ def main():
     }
     if args.mixed_iters > 0:
         print("bench: mixed vs FP64 full iterations", file=sys.stderr, flush=True)
-        del eng
-        out["config"]["mixed_vs_fp64"] = mixed_vs_fp64(o, v, args.mixed_iters)
+        del eng, step, dom, timer
+        gc.collect()
+        out["config"]["mixed_vs_fp64"] = mixed_vs_fp64(args.mixed_nocc, args.mixed_nvir, args.mixed_iters, autotune=args.mixed_autotune)
     if args.batched > 0:
         print("bench: batched ensemble of %d systems" % args.batched, file=sys.stderr, flush=True)
         out["config"]["batched"] = batched_ensemble(args.batched)

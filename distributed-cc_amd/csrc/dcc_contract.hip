@@ -77,6 +77,7 @@ struct ContractArgs {
     const dcc_batch *batches;
     int tiles_m, tiles_n;
     int64_t total_tiles;
+    int vec;  // split-BF16x3 only: bit 0 (A), bit 1 (B): element pairs of a thread may be loaded as 8 bytes
 };
 
 // Spread consecutive workgroups of one XCD over a contiguous range of tiles:
@@ -321,8 +322,9 @@ void contract_kernel(const ContractArgs<T> p) {
 //   accumulator: lane holds D[4*(lane/16) + r][lane%16] (as v_mfma_f32_16x16x4f32).
 // Each operand is staged in LDS as hi and lo planes in one of two layouts, chosen by which of its
 // indices is contiguous in memory:
-//   K-contiguous ("row-major" [row][k], stride kKP3): every thread loads a pair (k, k+1) of R/16 rows
-//     and writes each pair as one 32-bit word per plane; a fragment is one 16-byte ds_read_b128.
+//   K-contiguous ([row][k], 16-byte chunks XOR-swizzled per row pair): every thread loads a pair
+//     (k, k+1) of R/16 rows and writes each pair as one 32-bit word per plane; a fragment is one
+//     16-byte ds_read_b128.
 //   row-contiguous ([k][row], stride R + 16): every thread loads a pair of adjacent rows at R/16
 //     values of k and writes each pair as one word per plane; a fragment is two ds_read_b64_tr_b16,
 //     which on gfx950 gives lane 16g + i the four values at column i of the 4 x 16 block whose row j
@@ -336,7 +338,6 @@ typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
 constexpr int kBK3 = 32;
-constexpr int kKP3 = kBK3 + 8;  // [row][k] stride (16-bit units): 80-byte rows, conflict-free b128 reads
 
 // hi/lo bf16 pairs of two floats, element 0 in the low half of each word.
 __device__ __forceinline__ void split_pair(float x0, float x1, uint32_t &hi, uint32_t &lo) {
@@ -347,10 +348,20 @@ __device__ __forceinline__ void split_pair(float x0, float x1, uint32_t &hi, uin
     lo = __builtin_bit_cast(uint32_t, l);
 }
 
+// Packed operands hold one element per 32-bit word: hi = bf16(x) in the low half, lo = bf16(x - hi) in
+// the high half (written by dcc_pack_bf16x3 with the same conversion as split_pair).  Two words give
+// the hi pair and the lo pair directly.
+__device__ __forceinline__ void split_pair(uint32_t w0, uint32_t w1, uint32_t &hi, uint32_t &lo) {
+    hi = __builtin_amdgcn_perm(w1, w0, 0x05040100u);
+    lo = __builtin_amdgcn_perm(w1, w0, 0x07060302u);
+}
+
 // Staging geometry of one operand with R rows (m for A, n for B) per tile.
 template <int R, bool KFAST> struct Stage3 {
-    static constexpr int LD = KFAST ? kKP3 : R + 16;            // LDS row stride (16-bit units)
-    static constexpr int PLANE = KFAST ? R * kKP3 : kBK3 * (R + 16);
+    // K-contiguous planes are [row][32 k] with the four 16-byte chunks of a row XOR-swizzled by
+    // (row >> 1) & 3: the 16-byte fragment reads of 8 consecutive rows and the 32-bit pair writes
+    // of a half-wave then touch distinct banks, without padding.
+    static constexpr int PLANE = KFAST ? R * kBK3 : kBK3 * (R + 16);
     static constexpr int NR = KFAST ? R / 16 : 2;               // rows per thread
     static constexpr int NK = KFAST ? 2 : R / 16;               // k values per thread
     static constexpr int RP = R / 2;                            // row pairs (row-contiguous layout)
@@ -359,12 +370,16 @@ template <int R, bool KFAST> struct Stage3 {
         return KFAST ? 2 * (tid % 16) + j : tid / RP + (kThreads / RP) * j;
     }
     // 16-bit element index of (row, k) in a plane
-    __device__ static int at(int row, int k) { return KFAST ? row * kKP3 + k : k * (R + 16) + row; }
+    __device__ static int at(int row, int k) {
+        if constexpr (KFAST) return row * kBK3 + 8 * ((k >> 3) ^ ((row >> 1) & 3)) + (k & 7);
+        else return k * (R + 16) + row;
+    }
     // fragment of rows rbase..rbase+15 (16 rows x 8 k values per lane group) from one plane
     __device__ static bf16x8 frag(const unsigned short *plane, int rbase, int lane) {
         const int lrow = lane % 16, g = lane / 16;
         if constexpr (KFAST) {
-            return *reinterpret_cast<const bf16x8 *>(&plane[(rbase + lrow) * kKP3 + 8 * g]);
+            const int row = rbase + lrow;
+            return *reinterpret_cast<const bf16x8 *>(&plane[row * kBK3 + 8 * (g ^ ((row >> 1) & 3))]);
         } else {
             const int kr = 8 * g + lrow / 4, c = rbase + 4 * (lrow % 4);
             typedef __attribute__((address_space(3))) unsigned short lds_u16;
@@ -377,19 +392,20 @@ template <int R, bool KFAST> struct Stage3 {
     }
 };
 
-template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST>
+// PACKED: A and B hold packed hi/lo words instead of FP32 values (the pointers of p are reinterpreted).
+template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, bool PACKED>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
 void contract_bf16x3_kernel(const ContractArgs<float> p) {
     typedef Stage3<BM, A_KFAST> SA;
     typedef Stage3<BN, !B_NFAST> SB;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
-    constexpr int MAIN_BYTES = 2 * (SA::PLANE + SB::PLANE) * 2;  // hi + lo planes, 16-bit
+    constexpr int BUF = 2 * (SA::PLANE + SB::PLANE);            // one buffer: hi + lo planes, 16-bit units
+    constexpr int MAIN_BYTES = 2 * BUF * 2;                      // two buffers
     constexpr int EPI_BYTES = C_MFAST ? BN * (BM + 1) * 4 : 0;
     __shared__ __align__(16) unsigned char smem[MAIN_BYTES > EPI_BYTES ? MAIN_BYTES : EPI_BYTES];
     __shared__ int64_t ktab[2][2][kBK3];  // [tile parity][A, B][k]
-    unsigned short *As = reinterpret_cast<unsigned short *>(smem);
-    unsigned short *Bs = As + 2 * SA::PLANE;
+    unsigned short *const lds = reinterpret_cast<unsigned short *>(smem);
 
     const int tid = threadIdx.x;
     const int64_t L = xcd_remap((int64_t)blockIdx.x, p.total_tiles);
@@ -424,28 +440,70 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
         b_col[r] = n < p.N ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
     }
 
-    // wave 0: K offsets of tile k0 into table slot t (-1 beyond K)
+    // K offsets of tile k0 into table slot t (-1 beyond K): lanes 0..15 of waves 0, 1 fill the A
+    // entries, of waves 2, 3 the B entries
     auto fill_table = [&](int t, int k0) {
-        if (wave == 0) {
-            const uint32_t k = (uint32_t)(k0 + (lane & 31));
-            const int64_t oa = koff(p.ak, k), ob = koff(p.bk, k);
-            ktab[t][lane >> 5][lane & 31] = k < (uint32_t)p.K ? (lane < 32 ? oa : ob) : int64_t(-1);
+        if (lane < 16) {
+            const int e = (wave & 1) * 16 + lane;
+            const uint32_t k = (uint32_t)(k0 + e);
+            const int64_t o = koff(wave < 2 ? p.ak : p.bk, k);
+            ktab[t][wave >> 1][e] = k < (uint32_t)p.K ? o : int64_t(-1);
         }
     };
 
-    float a_reg[SA::NR][SA::NK], b_reg[SB::NR][SB::NK];
-    auto load_tiles = [&](int t, auto tail) {
+    typedef typename std::conditional<PACKED, uint32_t, float>::type E;
+    const E *Ap = reinterpret_cast<const E *>(p.A);
+    const E *Bp = reinterpret_cast<const E *>(p.B);
+    E a_reg[SA::NR][SA::NK], b_reg[SB::NR][SB::NK];
+    typedef E E2 __attribute__((ext_vector_type(2)));
+    auto load_tiles = [&](int t, auto tail, auto vec) {
         constexpr bool TAIL = decltype(tail)::value;
+        if constexpr (decltype(vec)::value && !TAIL) {
+            // each thread's pairs are adjacent and 8-byte aligned: one 8-byte load per pair
+            if constexpr (A_KFAST) {
+                const int64_t o = ktab[t][0][SA::kidx(tid, 0)];
+#pragma unroll
+                for (int r = 0; r < SA::NR; ++r) {
+                    const E2 x = *reinterpret_cast<const E2 *>(Ap + a_row[r] + o);
+                    a_reg[r][0] = x[0];
+                    a_reg[r][1] = x[1];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < SA::NK; ++j) {
+                    const E2 x = *reinterpret_cast<const E2 *>(Ap + a_row[0] + ktab[t][0][SA::kidx(tid, j)]);
+                    a_reg[0][j] = x[0];
+                    a_reg[1][j] = x[1];
+                }
+            }
+            if constexpr (!B_NFAST) {
+                const int64_t o = ktab[t][1][SB::kidx(tid, 0)];
+#pragma unroll
+                for (int r = 0; r < SB::NR; ++r) {
+                    const E2 x = *reinterpret_cast<const E2 *>(Bp + b_col[r] + o);
+                    b_reg[r][0] = x[0];
+                    b_reg[r][1] = x[1];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < SB::NK; ++j) {
+                    const E2 x = *reinterpret_cast<const E2 *>(Bp + b_col[0] + ktab[t][1][SB::kidx(tid, j)]);
+                    b_reg[0][j] = x[0];
+                    b_reg[1][j] = x[1];
+                }
+            }
+            return;
+        }
 #pragma unroll
         for (int j = 0; j < SA::NK; ++j) {
             const int64_t o = ktab[t][0][SA::kidx(tid, j)];
 #pragma unroll
             for (int r = 0; r < SA::NR; ++r) {
                 if constexpr (TAIL) {
-                    const float x = p.A[a_row[r] + (o < 0 ? 0 : o)];
-                    a_reg[r][j] = o < 0 ? 0.0f : x;
+                    const E x = Ap[a_row[r] + (o < 0 ? 0 : o)];
+                    a_reg[r][j] = o < 0 ? E(0) : x;
                 } else {
-                    a_reg[r][j] = p.A[a_row[r] + o];
+                    a_reg[r][j] = Ap[a_row[r] + o];
                 }
             }
         }
@@ -455,16 +513,18 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
 #pragma unroll
             for (int r = 0; r < SB::NR; ++r) {
                 if constexpr (TAIL) {
-                    const float x = p.B[b_col[r] + (o < 0 ? 0 : o)];
-                    b_reg[r][j] = o < 0 ? 0.0f : x;
+                    const E x = Bp[b_col[r] + (o < 0 ? 0 : o)];
+                    b_reg[r][j] = o < 0 ? E(0) : x;
                 } else {
-                    b_reg[r][j] = p.B[b_col[r] + o];
+                    b_reg[r][j] = Bp[b_col[r] + o];
                 }
             }
         }
     };
     // pairs along k (K-contiguous layout) or along rows (row-contiguous layout), one word per plane
-    auto store_tiles = [&]() {
+    auto store_tiles = [&](int buf) {
+        unsigned short *As = lds + buf * BUF;
+        unsigned short *Bs = As + 2 * SA::PLANE;
         uint32_t hi, lo;
         if constexpr (A_KFAST) {
 #pragma unroll
@@ -510,24 +570,31 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = flt4{0, 0, 0, 0};
 
-    // One LDS buffer and one register stage: step kt loads tile kt+1 into registers, multiplies
-    // tile kt from LDS, then (after a barrier) writes tile kt+1 to LDS.  Table slot t&1 holds
-    // the K offsets of tile t; wave 0 fills slot kt&1 for tile kt+2 during step kt (its last
-    // reader, the load of tile kt, finished before the barriers of step kt-1).
+    // Two LDS buffers and one register stage: step kt loads tile kt+1 into registers, multiplies
+    // tile kt from buffer kt&1 and writes tile kt+1 into the other buffer (last read in step kt-1);
+    // one barrier per step.  Table slot t&1 holds the K offsets of tile t; slot kt&1 is refilled
+    // for tile kt+2 during step kt (its last reader, the load of tile kt, ran in step kt-1).
     const int nk = (p.K + kBK3 - 1) / kBK3;
     const int nfull = p.K / kBK3;  // tiles with every k < K
+    // 8-byte pair loads: the host checked strides, extents and pointer alignment; the batch offsets
+    // (and, for row pairs of A, the row range) are checked here, uniformly per workgroup
+    const bool vec = (p.vec & 3) == 3 && ((bt.a_off | bt.b_off) & 1) == 0 &&
+                     (A_KFAST || ((bt.m_begin | bt.m_end) & 1) == 0);
+    auto mainloop = [&](auto vtag) {
     fill_table(0, 0);
     fill_table(1, kBK3);
     __syncthreads();
-    if (nfull > 0) load_tiles(0, std::false_type{});
-    else load_tiles(0, std::true_type{});
-    store_tiles();
+    if (nfull > 0) load_tiles(0, std::false_type{}, vtag);
+    else load_tiles(0, std::true_type{}, vtag);
+    store_tiles(0);
     __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
         const int t1 = (kt + 1) & 1;
-        if (kt + 1 < nfull) load_tiles(t1, std::false_type{});
-        else load_tiles(t1, std::true_type{});
+        if (kt + 1 < nfull) load_tiles(t1, std::false_type{}, vtag);
+        else load_tiles(t1, std::true_type{}, vtag);
         fill_table(kt & 1, (kt + 2) * kBK3);
+        const unsigned short *As = lds + (kt & 1) * BUF;
+        const unsigned short *Bs = As + 2 * SA::PLANE;
         bf16x8 ah[FM], al[FM];
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
@@ -545,10 +612,12 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
             }
         }
-        __syncthreads();
-        store_tiles();
+        store_tiles(t1);
         __syncthreads();
     }
+    };
+    if (vec) mainloop(std::true_type{});
+    else mainloop(std::false_type{});
 
     if constexpr (C_MFAST) {
         float *Cs = reinterpret_cast<float *>(smem);
@@ -696,6 +765,7 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     args.bk = make_kmap(bk);
     args.batches = batches;
     args.total_tiles = nbatch;
+    args.vec = 0;
 
     // choose the contiguous index of each operand as the lane-fastest one
     const bool a_kfast = inner_stride(ak) == 1 || inner_stride(am) != 1;
@@ -729,39 +799,67 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
 }
 
-template <int BM, int BN, bool C_MFAST>
+template <int BM, int BN, bool C_MFAST, bool PACKED>
 void launch_bf16x3(ContractArgs<float> &args, bool a_kfast, bool b_nfast, dim3 grid, dim3 block,
                    hipStream_t stream) {
     if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, PACKED>), grid, block, 0, stream,
+                           args);
     else if (a_kfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, PACKED>), grid, block, 0, stream,
+                           args);
     else if (b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, PACKED>), grid, block, 0, stream,
+                           args);
     else
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST>), grid, block, 0, stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, PACKED>), grid, block, 0, stream,
+                           args);
 }
 
 template <int BM, int BN>
 int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool a_kfast, bool b_nfast,
-               bool c_mfast, hipStream_t s) {
+               bool c_mfast, bool packed, hipStream_t s) {
     args.tiles_m = (int)((max_rows + BM - 1) / BM);
     args.tiles_n = (int)((args.N + BN - 1) / BN);
     args.total_tiles = nbatch * args.tiles_m * args.tiles_n;
     DCC_CHECK(args.total_tiles < ((int64_t)1 << 31), "too many tiles");
     const dim3 grid((unsigned)args.total_tiles), block(kThreads);
-    if (c_mfast)
-        launch_bf16x3<BM, BN, true>(args, a_kfast, b_nfast, grid, block, s);
+    if (c_mfast && packed)
+        launch_bf16x3<BM, BN, true, true>(args, a_kfast, b_nfast, grid, block, s);
+    else if (c_mfast)
+        launch_bf16x3<BM, BN, true, false>(args, a_kfast, b_nfast, grid, block, s);
+    else if (packed)
+        launch_bf16x3<BM, BN, false, true>(args, a_kfast, b_nfast, grid, block, s);
     else
-        launch_bf16x3<BM, BN, false>(args, a_kfast, b_nfast, grid, block, s);
+        launch_bf16x3<BM, BN, false, false>(args, a_kfast, b_nfast, grid, block, s);
     DCC_LAUNCH_CHECK();
     return 0;
+}
+
+// Whether the element pairs a split-BF16x3 loader thread holds -- (k, k+1) of one row when the
+// operand is staged K-contiguous, rows (2i, 2i+1) at one k otherwise -- are adjacent in memory and
+// 8-byte aligned for every row and k (batch offsets and, for A, the row range parity are checked
+// in the kernel).  nrows_even: the row extent (N for B) must be even so no pair straddles its end
+// (0 = checked per batch in the kernel).
+bool pairs_contiguous(const void *base, const dcc_idxmap &rows, const dcc_idxmap &k, bool kfast,
+                      int64_t nrows) {
+    if (reinterpret_cast<uintptr_t>(base) % 8 != 0 || rows.table != nullptr || k.table != nullptr) return false;
+    const dcc_idxmap &pair = kfast ? k : rows;   // the map whose inner index the pair runs along
+    const dcc_idxmap &other = kfast ? rows : k;
+    if (inner_stride(pair) != 1) return false;
+    // pairs start at even indices: the inner level must have an even extent, or be the only level
+    if (pair.e2 > 1 ? pair.e2 % 2 != 0 : false) return false;
+    if (pair.e2 > 1 && ((pair.s0 | pair.s1) & 1)) return false;
+    if (pair.e2 <= 1 && pair.e1 > 1 && (pair.e1 % 2 != 0 || (pair.s0 & 1))) return false;
+    if ((other.s0 | other.s1 | other.s2) & 1) return false;
+    if (!kfast && nrows % 2 != 0) return false;
+    return true;
 }
 
 int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
                           dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn, float *C,
                           dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
-                          int64_t max_rows, int64_t tile_shape, dcc_stream_t stream) {
+                          int64_t max_rows, int64_t tile_shape, dcc_stream_t stream, bool packed = false) {
     DCC_CHECK(M >= 0 && N >= 0 && K >= 0 && nbatch >= 0 && max_rows >= 0, "negative extent");
     DCC_CHECK(M < ((int64_t)1 << 31) && N < ((int64_t)1 << 31) && K < ((int64_t)1 << 31),
               "extent exceeds 2^31");
@@ -791,15 +889,17 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
     const bool a_kfast = inner_stride(ak) == 1 || inner_stride(am) != 1;
     const bool b_nfast = inner_stride(bn) == 1 || inner_stride(bk) != 1;
     const bool c_mfast = inner_stride(cn) != 1;
+    args.vec = (pairs_contiguous(A, am, ak, a_kfast, 0) ? 1 : 0) |
+               (pairs_contiguous(B, bn, bk, !b_nfast, N) ? 2 : 0);
     hipStream_t s = dcc::as_stream(stream);
     // default: 128x128 unless one side of the problem is small
     int shape = (int)tile_shape;
     if (shape < 0) shape = (max_rows <= 64 ? 0 : 1) + (N <= 64 ? 0 : 2);
     switch (shape) {
-        case 0: return run_bf16x3<64, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, s);
-        case 1: return run_bf16x3<128, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, s);
-        case 2: return run_bf16x3<64, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, s);
-        default: return run_bf16x3<128, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, s);
+        case 0: return run_bf16x3<64, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, packed, s);
+        case 1: return run_bf16x3<128, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, packed, s);
+        case 2: return run_bf16x3<64, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, packed, s);
+        default: return run_bf16x3<128, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, packed, s);
     }
 }
 
@@ -812,6 +912,16 @@ extern "C" int dcc_contract_f32_bf16x3(int64_t M, int64_t N, int64_t K, double a
                                        dcc_stream_t stream) {
     return contract_bf16x3_entry(M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
                                  -1, stream);
+}
+
+extern "C" int dcc_contract_packed_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, double beta,
+                                          const uint32_t *A, dcc_idxmap am, dcc_idxmap ak, const uint32_t *B,
+                                          dcc_idxmap bk, dcc_idxmap bn, float *C, dcc_idxmap cm, dcc_idxmap cn,
+                                          const dcc_batch *batches, int64_t nbatch, int64_t max_rows,
+                                          int64_t tile_shape, dcc_stream_t stream) {
+    return contract_bf16x3_entry(M, N, K, alpha, beta, reinterpret_cast<const float *>(A), am, ak,
+                                 reinterpret_cast<const float *>(B), bk, bn, C, cm, cn, batches, nbatch, max_rows,
+                                 tile_shape, stream, true);
 }
 
 extern "C" int dcc_contract_tiled_f32_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, double beta,

@@ -87,6 +87,9 @@ _SIGS = {
                                 c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_contract_f32_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap, IdxMap,
                                  c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_vp], ctypes.c_int),
+    "dcc_pack_bf16x3": ([c_vp, c_vp, c_i64, c_vp], ctypes.c_int),
+    "dcc_contract_packed_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap,
+                                    IdxMap, c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_contract_tiled_f32_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap,
                                        IdxMap, c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_transform_f64": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),

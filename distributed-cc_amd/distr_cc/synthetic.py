@@ -59,7 +59,9 @@ class SyntheticSystem:
         o, v = self.nocc, self.nvir
         eia = self.mo_energy[:o, None] - self.mo_energy[None, o:]
         rng = np.random.default_rng(self.seed + 7919)
-        blocks = []
+        n3 = o * (o + 1) * (o + 2) // 6
+        out = np.empty((n3, v, v, v), dtype=self.dtype)
+        n = 0
         for i in range(o):
             for j in range(i, o):
                 for k in range(j, o):
@@ -72,8 +74,9 @@ class SyntheticSystem:
                     elif j == k:
                         x = 0.5 * (x + x.transpose(0, 2, 1))
                     d = eia[i, :, None, None] + eia[j, None, :, None] + eia[k, None, None, :]
-                    blocks.append(x / d)
-        return np.ascontiguousarray(np.stack(blocks).astype(self.dtype))
+                    out[n] = x / d
+                    n += 1
+        return out
 
     def bare_intermediates(self):
         """Leading (T1 = T2 = 0) values of the triples intermediates: ERI slices.

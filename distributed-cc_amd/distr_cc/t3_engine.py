@@ -119,7 +119,9 @@ class GemmLaunch:
 
     def run(self, stream):
         s = stream.handle if stream is not None else None
-        if self.tile >= 0:
+        if self.fn == "dcc_contract_packed_bf16x3":
+            L.call(self.fn, *self.args, self.batches.vp, self.nbatch, self.max_rows, self.tile, s)
+        elif self.tile >= 0:
             L.call(self.fn.replace("contract_", "contract_tiled_"), *self.args, self.batches.vp, self.nbatch,
                    self.max_rows, self.tile, s)
         else:
@@ -332,7 +334,45 @@ class TriplesEngine:
         self._tables = []
         self.launches = self._plan_residual()
         self._retarget(self.launches)
+        self._packs = self._setup_packing() if contract == "bf16x3" else []
         self._lower = None
+
+    def _setup_packing(self):
+        """Split-BF16x3 residual: point every GEMM operand at a packed (hi | lo << 16) copy of its
+        tensor, so each element is split once per residual instead of once per tile that reads it.
+        Y and Z are packed in place (they are rebuilt from T3 for every residual and read by nothing
+        else); the others get packed copies.  Returns the (source, packed, count) passes to run."""
+        sources = [self.t3, self.Yz, self.W_vvvv, self.F_vv, self.W_ovvv, self.W_oovo, self.t2, self.WR1,
+                   self.WR3, self.pfs]
+        packed = {}
+
+        def find(ptr):
+            for src in sources:
+                if src.ptr <= ptr < src.ptr + src.nbytes:
+                    if id(src) not in packed:
+                        dst = src if src is self.Yz else DeviceArray(src.shape, np.uint32)
+                        packed[id(src)] = (src, dst)
+                    return packed[id(src)]
+            return None
+
+        for g in self.launches:
+            if not isinstance(g, GemmLaunch):
+                continue
+            a, b = find(g.args[5].value), find(g.args[8].value)
+            if a is None or b is None:
+                continue
+            args = list(g.args)
+            args[5] = ctypes.c_void_p(a[1].ptr + (g.args[5].value - a[0].ptr))
+            args[8] = ctypes.c_void_p(b[1].ptr + (g.args[8].value - b[0].ptr))
+            g.args = tuple(args)
+            g.fn = "dcc_contract_packed_bf16x3"
+        return [(src, dst, int(np.prod(src.shape))) for src, dst in packed.values()]
+
+    def pack_operands(self):
+        """Run the split-BF16x3 packing passes (after T3, Y/Z and the intermediates are final)."""
+        s = self.stream.handle if self.stream is not None else None
+        for src, dst, n in self._packs:
+            L.call("dcc_pack_bf16x3", src.vp, dst.vp, n, s)
 
     def _retarget(self, launches):
         if self.contract == "bf16x3":
@@ -682,6 +722,8 @@ class TriplesEngine:
         """
         self.R.fill(0.0, self.stream)
         self.materialize_y()
+        if self._packs:
+            self.pack_operands()
         for g in self.launches:
             if timer is not None:
                 timer(g, 'start')
@@ -733,6 +775,16 @@ class TriplesEngine:
         for g, t in zip(launches, tiles):
             if isinstance(g, GemmLaunch) and t is not None:
                 g.tile = int(t)
+
+    def tiles_by_name(self):
+        """{launch name: tile} of the contraction launches (for transfer to an engine of another size)."""
+        return {g.name: g.tile for g in self._tunable() if isinstance(g, GemmLaunch)}
+
+    def set_tiles_by_name(self, tiles):
+        """Apply a tiles_by_name() map; launches without an entry keep their tile."""
+        for g in self._tunable():
+            if isinstance(g, GemmLaunch) and g.name in tiles:
+                g.tile = int(tiles[g.name])
 
     def autotune(self, reps=2, include_lower=True):
         """Time the workgroup tiles of every contraction launch and keep the fastest: eight for the

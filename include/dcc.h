@@ -137,6 +137,19 @@ int dcc_contract_f32_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, doubl
                             dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn, float *C,
                             dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
                             int64_t max_rows, dcc_stream_t stream);
+/* Split-BF16x3 packing of an FP32 tensor: dst[i] = hi | lo << 16 with hi = bf16(src[i]) and
+ * lo = bf16(src[i] - hi), both rounded to nearest even; dst may alias src (in-place packing).
+ * No reference counterpart (the reference is FP64 throughout). */
+int dcc_pack_bf16x3(const float *src, uint32_t *dst, int64_t n, dcc_stream_t stream);
+
+/* dcc_contract_tiled_f32_bf16x3 on packed operands: A and B hold dcc_pack_bf16x3 words (same index
+ * maps, element offsets as for the FP32 tensors), so the GEMM does no splitting; C is FP32.
+ * No reference counterpart (as dcc_contract_f32_bf16x3). */
+int dcc_contract_packed_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, double beta, const uint32_t *A,
+                               dcc_idxmap am, dcc_idxmap ak, const uint32_t *B, dcc_idxmap bk, dcc_idxmap bn,
+                               float *C, dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
+                               int64_t max_rows, int64_t tile_shape, dcc_stream_t stream);
+
 /* dcc_contract_f32_bf16x3 with an explicit workgroup tile: tile_shape 0 = 64x64, 1 = 128x64,
  * 2 = 64x128, 3 = 128x128 (M x N); -1 picks 128 on each side unless that side has at most 64 rows.
  * No reference counterpart (as dcc_contract_f32_bf16x3). */

@@ -200,13 +200,30 @@ class Emulator:
     def dcc_contract_tiled_f32_bf16x3(self, *a):
         return self._contract("bf16x3", *a[:-2], a[-1])
 
+    def dcc_contract_packed_bf16x3(self, *a):
+        return self._contract("packed", *a[:-2], a[-1])
+
+    def dcc_pack_bf16x3(self, src, dst, n, stream):
+        n = int(n)
+        x = self.dev.view(_ptr(src), np.float32, n).copy()
+        hi, lo = bf16_split(x)
+        w = (hi.astype(np.float32).view(np.uint32) >> 16) | (lo.astype(np.float32).view(np.uint32) & 0xFFFF0000)
+        self.dev.view(_ptr(dst), np.uint32, n)[:] = w
+        return 0
+
     def _contract(self, dt, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
                   stream):
-        split = dt == "bf16x3"
+        split, packed = dt in ("bf16x3", "packed"), dt == "packed"
         if split:
             dt = np.float32
         dev = self.dev
         Av, Bv, Cv = dev.view(_ptr(A), dt), dev.view(_ptr(B), dt), dev.view(_ptr(C), dt)
+        if packed:
+            Av, Bv = dev.view(_ptr(A), np.uint32), dev.view(_ptr(B), np.uint32)
+
+        def parts(w):   # hi, lo of packed words
+            return ((w << 16).astype(np.uint32).view(np.float32).astype(np.float64),
+                    (w & 0xFFFF0000).astype(np.uint32).view(np.float32).astype(np.float64))
         bt = dev.view(_ptr(batches), np.int64, 5 * int(nbatch)).reshape(-1, 5)
         ks = np.arange(K)
         ns = np.arange(N)
@@ -219,8 +236,8 @@ class Emulator:
             bi = b_off + _map(dev, bk, ks, b)[:, None] + _map(dev, bn, ns, b)[None, :]
             ci = c_off + _map(dev, cm, ms, b)[:, None] + _map(dev, cn, ns, b)[None, :]
             if split and K > 0:
-                ah, al = bf16_split(Av[ai])
-                bh, bl = bf16_split(Bv[bi])
+                ah, al = parts(Av[ai]) if packed else bf16_split(Av[ai])
+                bh, bl = parts(Bv[bi]) if packed else bf16_split(Bv[bi])
                 prod = ah @ bh + ah @ bl + al @ bh
             else:
                 prod = (Av[ai].astype(np.float64) @ Bv[bi].astype(np.float64)) if K > 0 else np.zeros(ci.shape)

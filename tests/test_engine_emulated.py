@@ -211,7 +211,8 @@ def test_bf16x3_residual_plan(monkeypatch):
     s = SyntheticSystem(o, v, naux=20, seed=31)
     im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
     eng = TriplesEngine(o, v, dtype=np.float32, contract="bf16x3")
-    assert all(g.fn == "dcc_contract_f32_bf16x3" for g in eng.launches if hasattr(g, "args"))
+    # every residual GEMM reads packed (pre-split) operands
+    assert all(g.fn == "dcc_contract_packed_bf16x3" for g in eng.launches if hasattr(g, "args"))
     eng.set_t3(t3)
     eng.set_intermediates(im, t2)
     eng.residual()

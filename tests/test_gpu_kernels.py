@@ -241,24 +241,28 @@ def test_contract_bf16x3_plain_gemm(dev, M, N, K):
     assert abs(got - exact).max() < 1e-4 * scale * np.sqrt(K)         # split error ~2^-16 per product
 
 
+@pytest.mark.parametrize("even", [False, True])
+@pytest.mark.parametrize("packed", [False, True])
 @pytest.mark.parametrize("tile", [0, 1, 2, 3])
 @pytest.mark.parametrize("layout", ["kfast-nfast", "mfast-nfast", "kfast-kfast", "mfast-kfast-ct"])
-def test_contract_bf16x3_layouts(dev, layout, tile):
+def test_contract_bf16x3_layouts(dev, layout, tile, packed, even):
     """Every staging layout of the split-BF16x3 kernel (A contiguous along k or m, B along n or k),
     the LDS-staged epilogue (C contiguous along m), partial M/N tiles, a K tail and a batch offset,
-    for each workgroup tile; checked against the same split evaluated exactly on the host."""
+    for each workgroup tile, with FP32 operands or operands packed by dcc_pack_bf16x3; checked
+    against the same split evaluated exactly on the host.  ``even`` sizes and offsets let the
+    kernel load element pairs as 8-byte vectors; odd ones force the per-element loads."""
     from abi_emulator import bf16_split
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
-    M, N, K = 150, 83, 75
-    _say("contract bf16x3 layout %s tile %d" % (layout, tile))
+    M, N, K = (150, 84, 74) if even else (150, 83, 75)
+    _say("contract bf16x3 layout %s tile %d packed %d even %d" % (layout, tile, packed, even))
     rng = np.random.default_rng(11 + tile)
     A = rng.normal(size=(M, K)).astype(np.float32)
     B = rng.normal(size=(K, N)).astype(np.float32)
     C0 = rng.normal(size=(M, N)).astype(np.float32)
     lin = lambda s: L.IdxMap(None, 0, 1, 1 << 20, 0, 0, s)
     a_mfast, b_kfast, c_mt = layout.startswith("mfast"), "-kfast" in layout, layout.endswith("-ct")
-    pad = 7   # batch offset of every operand
+    pad = 8 if even else 7   # batch offset of every operand
     hA = np.concatenate([np.zeros(pad, np.float32), (A.T if a_mfast else A).ravel()])
     hB = np.concatenate([np.zeros(pad, np.float32), (B.T if b_kfast else B).ravel()])
     hC = np.concatenate([np.zeros(pad, np.float32), (C0.T if c_mt else C0).ravel()])
@@ -269,8 +273,20 @@ def test_contract_bf16x3_layouts(dev, layout, tile):
     bt = np.zeros(1, dtype=L.BATCH_DTYPE)
     bt[0] = (pad, pad, pad, 0, M)
     dbt = DeviceArray.from_host(bt)
-    L.call("dcc_contract_tiled_f32_bf16x3", M, N, K, 1.5, 0.5, dA.vp, am, ak, dB.vp, bk, bn, dC.vp, cm, cn,
-           dbt.vp, 1, M, tile, None)
+    if packed:
+        pA, pB = DeviceArray(hA.shape, np.uint32), DeviceArray(hB.shape, np.uint32)
+        L.call("dcc_pack_bf16x3", dA.vp, pA.vp, hA.size, None)
+        L.call("dcc_pack_bf16x3", dB.vp, pB.vp, hB.size, None)
+        L.call("dcc_contract_packed_bf16x3", M, N, K, 1.5, 0.5, pA.vp, am, ak, pB.vp, bk, bn, dC.vp, cm, cn,
+               dbt.vp, 1, M, tile, None)
+        # the packing itself: hi in the low half, lo in the high half, as the host split
+        w = pA.download()[pad:]
+        hi, lo = bf16_split(hA[pad:])
+        assert np.array_equal((w << 16).view(np.float32), hi.astype(np.float32))
+        assert np.array_equal((w & 0xFFFF0000).view(np.float32), lo.astype(np.float32))
+    else:
+        L.call("dcc_contract_tiled_f32_bf16x3", M, N, K, 1.5, 0.5, dA.vp, am, ak, dB.vp, bk, bn, dC.vp, cm, cn,
+               dbt.vp, 1, M, tile, None)
     out = dC.download()[pad:].astype(np.float64)
     got = out.reshape(N, M).T if c_mt else out.reshape(M, N)
     ah, al = bf16_split(A)
