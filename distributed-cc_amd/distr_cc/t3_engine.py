@@ -6,6 +6,7 @@ Data in HBM (FP64, C order):
   Y              (npairs, o, v, v, v)  Y[(p,q)][l] = T[l,p,q] for p<=q (ring B operand)
   Z              (npairs, o, v, v, v)  Y[l][b,d,c] + Y[l][c,b,d] stored as [d,b,c], right after Y
   WR1, WR3[2]    (o, v, o, v)      ring weights, rows (r,x), cols (l,d)
+  Gs             (pairs per chunk, o, v, v, v)  G[r;pq][a,b,c] before its placement
   pfs            (6, n3, n3)       W_oooo/F_oo prefactors
   intermediates  shapes of distr_cc/rccsdt.py:380,445,625
 
@@ -18,9 +19,9 @@ batched gather-GEMMs (dcc_contract_f64):
                         + W_ovvo[l,a,d,r] Y[l][d,b,c]
                         - 1/2 (W_ovov[l,a,r,d] + W_ovvo[l,a,d,r]) (Y[l][b,d,c] + Y[l][c,b,d]) )
     part 3 of target (i,j,k) is G[i;jk], part 4 is G[j;ik] with (a,b) swapped and
-    part 5 is G[k;ij] read as [c,a,b]; one launch per (part, term), batched over
-    the pairs p<=q with rows (r, x) restricted to the part's range of r; with Z the
-    last three terms are one launch over K = (term, l, d) with B = [Y; Z];
+    part 5 is G[k;ij] read as [c,a,b]; G is computed once for all r (GEMMs batched over
+    the pairs p<=q, rows (r, x)) into a scratch buffer and placed by one S3 pass per
+    part; with Z the last three terms are one GEMM over K = (term, l, d), B = [Y; Z];
   * W_oooo/F_oo x T3 (rccsdt.py:616-732): 6 GEMMs M = n3 targets, K = n3 sources,
     N = v^3, with the prefactors built on the device.
 Then dcc_r3_finalize_f64 fuses rccsdt.py:947-971.
@@ -248,7 +249,8 @@ def s3_jobs_array(jobs):
 class TriplesEngine:
     """Holds the device state of one RCCSDT triples update for (nocc, nvir)."""
 
-    def __init__(self, nocc, nvir, stream=None, device=0, dtype=np.float64, contract="native", ring_fused=None):
+    def __init__(self, nocc, nvir, stream=None, device=0, dtype=np.float64, contract="native", ring_fused=None,
+                 ring_g_bytes=8 << 30):
         """``dtype`` float64 is the plain FP64 path; float32 stores T3, R, Y and the
         intermediates in FP32 and contracts on FP32 MFMA, keeping the denominator,
         the projection arithmetic and the norms in FP64.  ``contract="bf16x3"`` (FP32 storage
@@ -257,7 +259,8 @@ class TriplesEngine:
         ``ring_fused`` (default: when the device has room for a second Y-sized buffer) stores
         Z[(p,q)][l][d,b,c] = Y[l][b,d,c] + Y[l][c,b,d] next to Y, so the two terms that share the
         W_ovov + W_ovvo weight become one, fused along K with the W_ovvo term: three ring
-        GEMMs per part instead of five, and 4/5 of the ring flops."""
+        GEMMs per part instead of five, and 4/5 of the ring flops.  ``ring_g_bytes`` bounds the
+        scratch buffer of the ring operator G (pairs are processed in chunks that fit)."""
         device_init(device)
         o, v = int(nocc), int(nvir)
         self.nocc, self.nvir = o, v
@@ -332,6 +335,7 @@ class TriplesEngine:
             raise ValueError("contract must be 'native', or 'bf16x3' with float32 storage")
         self.contract = contract
         self._tables = []
+        self.ring_g_bytes = int(ring_g_bytes)
         self.launches = self._plan_residual()
         self._retarget(self.launches)
         self._packs = self._setup_packing() if contract == "bf16x3" else []
@@ -439,13 +443,22 @@ class TriplesEngine:
         return launches
 
     def _plan_ring(self):
-        """Ring terms (rccsdt.py:540-596) as 5 GEMMs x 3 parts, batched over pairs."""
-        o, v, n3 = self.nocc, self.nvir, self.n3
+        """Ring terms (rccsdt.py:540-596).
+
+        G[r;pq] (the part-3 operator of the module docstring) is the same for the three parts; only
+        its placement differs (part 3 of target (r,p,q) is G[r;pq], part 4 of (p,r,q) is G[r;pq] with
+        (a,b) swapped, part 5 of (p,q,r) is G[r;pq] read as [c,a,b]).  So G is computed once for every
+        r into a scratch buffer Gs[pair][r][a,b,c] -- GEMMs with M = (r, x) over all o*v rows,
+        batched over pairs, plain strided output -- and then added into R by one S3 placement pass
+        per part.  Pairs are processed in chunks whose G fits ``ring_g_bytes``."""
+        o, v = self.nocc, self.nvir
         v2, v3 = v * v, v ** 3
         ov = o * v
-        # where element [x0,x1,x2] of G[r;pq] lands in the target block, per part
-        place = {3: (v2, v, 1), 4: (v, v2, 1), 5: (1, v2, v)}
-        # rows r of pair (p,q) owned by each part, and the target of row r
+        dt = self.dtype
+        pc = int(max(1, min(self.npairs, self.ring_g_bytes // max(1, o * v3 * dt.itemsize))))
+        self.Gs = DeviceArray((pc, o, v, v, v), dt)
+        Sg = (v2, v, 1)   # strides of [a,b,c] in a G block
+
         def rows(part, p, q):
             return {3: (0, p + 1), 4: (p, q + 1), 5: (q, o)}[part]
 
@@ -453,12 +466,10 @@ class TriplesEngine:
             t = {3: (r, p, q), 4: (p, r, q), 5: (p, q, r)}[part]
             return ijk_to_linear(t[0], t[1], t[2], o)
 
-        # B operand per term: Y[l][...] strides for (l,d) and the two N axes
+        # placement of G[a,b,c] in the target block, as the S3 permutation of the source block
+        perm = {3: (0, 1, 2), 4: (1, 0, 2), 5: (1, 2, 0)}
         # term: (weight, G axis carried by M, N axes (outer, inner), B strides of d, n_outer, n_inner)
-        terms = [
-            ('WR1', 1, (0, 2), v, v2, 1),    # -W_ovov[l,b,r,d] Y[l][a,d,c]
-            ('WR1', 2, (0, 1), 1, v2, v),    # -W_ovov[l,c,r,d] Y[l][a,b,d]
-        ]
+        terms = []
         if self.ring_fused:
             # W_ovvo[l,a,d,r] Y[l][d,b,c] - 1/2(..) Z[l][d,b,c] as one GEMM over K = (term, l, d)
             terms.append(('WR3', 0, (1, 2), v2, v, 1))
@@ -468,24 +479,15 @@ class TriplesEngine:
                 ('WR3b', 0, (1, 2), v, v2, 1),   # -1/2(..) Y[l][b,d,c]
                 ('WR3b', 0, (1, 2), 1, v, v2),   # -1/2(..) Y[l][c,b,d]
             ]
+        terms += [
+            ('WR1', 1, (0, 2), v, v2, 1),    # -W_ovov[l,b,r,d] Y[l][a,d,c]
+            ('WR1', 2, (0, 1), 1, v2, v),    # -W_ovov[l,c,r,d] Y[l][a,b,d]
+        ]
         launches = []
-        for part in (3, 4, 5):
-            S = place[part]
-            for wname, mx, (n0, n1), sd, sn0, sn1 in terms:
-                # innermost N index = the one contiguous in R (coalesced C), unless that breaks the
-                # coalescing of the B (Y) loads, which are re-read once per M tile and cost more
-                if S[n0] < S[n1] and (sd == 1 or sn0 == 1):
-                    n0, n1, sn0, sn1 = n1, n0, sn1, sn0
-                # per-pair row table: cm[pair][(r,x)] = target(r)*v^3 + x*S[mx]
-                tbl = np.zeros((self.npairs, ov), dtype=np.int64)
-                batches = []
-                for pq, (p, q) in enumerate(self.pairs):
-                    r0, r1 = rows(part, p, q)
-                    for r in range(r0, r1):
-                        tbl[pq, r * v:(r + 1) * v] = target(part, r, p, q) * v3 + np.arange(v) * S[mx]
-                    batches.append((0, pq * o * v3, 0, r0 * v, r1 * v))
-                tdev = DeviceArray.from_host(tbl)
-                self._tables.append(tdev)
+        for c0 in range(0, self.npairs, pc):
+            chunk = list(enumerate(self.pairs))[c0:c0 + pc]
+            batches = [(0, pq * o * v3, (pq - c0) * o * v3, 0, ov) for pq, _ in chunk]
+            for n, (wname, mx, (n0, n1), sd, sn0, sn1) in enumerate(terms):
                 W = getattr(self, wname)
                 nt = 2 if wname == 'WR3' else 1                  # terms fused along K
                 am = L.IdxMap(None, 0, 1, ov, 0, 0, ov)          # row (r,x) of the (ov x ov) weight
@@ -493,13 +495,20 @@ class TriplesEngine:
                 zoff = self.npairs * o * v3                      # Z follows Y in one buffer
                 bk = L.IdxMap(None, 0, o, v, zoff, v3, sd)       # (term, l, d)
                 bn = L.IdxMap(None, 0, v, v, 0, sn0, sn1)        # (n_outer, n_inner)
-                cm = table_map(tdev, ov)
-                cn = L.IdxMap(None, 0, v, v, 0, S[n0], S[n1])
-                nrows = sum(b[4] - b[3] for b in batches)
-                launches.append(GemmLaunch('ring part%d %s M%d' % (part, wname, mx), ov, v2, nt * ov, 1.0, 1.0,
-                                           W, am, ak, self.Y, bk, bn, self.R, cm, cn, batches,
-                                           max(b[4] - b[3] for b in batches), 2.0 * nrows * v2 * ov * nt,
-                                           keep=(tdev,)))
+                cm = L.IdxMap(None, 0, o, v, 0, v3, Sg[mx])      # (r, x)
+                cn = L.IdxMap(None, 0, v, v, 0, Sg[n0], Sg[n1])
+                launches.append(GemmLaunch('ring %s M%d' % (wname, mx), ov, v2, nt * ov, 1.0,
+                                           0.0 if n == 0 else 1.0, W, am, ak, self.Y, bk, bn, self.Gs, cm, cn,
+                                           batches, ov, 2.0 * len(chunk) * ov * v2 * ov * nt))
+            for part in (3, 4, 5):
+                coef = np.zeros(6)
+                coef[TP_T3.index(perm[part])] = 1.0
+                jobs = []
+                for pq, (p, q) in chunk:
+                    r0, r1 = rows(part, p, q)
+                    for r in range(r0, r1):
+                        jobs.append((target(part, r, p, q) * v3, ((pq - c0) * o + r) * v3, coef))
+                launches.append(S3Launch('ring place part%d' % part, self.R, self.Gs, v, 1.0, jobs))
         return launches
 
 

@@ -30,6 +30,26 @@ def test_engine_residual_and_update_plan(monkeypatch, o, v):
     assert abs(n2 - n2o) <= 1e-12 * n2o
 
 
+@pytest.mark.parametrize("o,v,pairs,fused", [(4, 5, 3, True), (5, 3, 1, True), (4, 4, 4, False)])
+def test_engine_ring_chunked(monkeypatch, o, v, pairs, fused):
+    """The ring operator G computed for a few pairs at a time (the memory-bounded path of large
+    systems), with and without the fused W_ovvo + Z GEMM."""
+    abi_emulator.install(monkeypatch)
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    s = SyntheticSystem(o, v, naux=20, seed=19 + o)
+    rng = np.random.default_rng(4)
+    im = {k: x + 1e-3 * rng.normal(size=x.shape) for k, x in s.bare_intermediates().items()}
+    t2, t3 = s.t2(), s.t3_tri()
+    eng = TriplesEngine(o, v, ring_fused=fused, ring_g_bytes=pairs * o * v ** 3 * 8)
+    assert eng.Gs.shape[0] == pairs
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.residual()
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3, batch_size=7)
+    assert abs(eng.R.download() - Ro).max() <= 1e-12 * abs(Ro).max()
+
+
 @pytest.mark.parametrize("o,v", [(1, 3), (3, 4), (4, 3), (2, 5)])
 def test_engine_t3_to_r1r2_and_w_plan(monkeypatch, o, v):
     _t3_to_r1r2_and_w(monkeypatch, o, v, None)
