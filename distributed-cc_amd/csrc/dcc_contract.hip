@@ -95,6 +95,19 @@ template <int a, int b> struct CMax { static constexpr int v = a > b ? a : b; };
 // C_MFAST selects the epilogue: false writes C straight from the accumulators (16 consecutive lanes
 // walk n, coalesced when C is contiguous along n); true stages the tile in LDS and writes it with
 // consecutive lanes walking m, for outputs whose contiguous index is carried by M.
+// Loader geometry of one operand with R rows per tile (BK = 16, 256 threads), in element pairs:
+// K-contiguous (KFAST): pair (2 kp, 2 kp + 1) with kp = tid % 8, rows tid / 8 + 32 r (NR = R/32, NK = 2);
+// row-contiguous: rows (2 i, 2 i + 1) with i = tid % (R/2), k = tid / (R/2) + (512 / R) j (NR = 2,
+// NK = R/32).  reg[r][j] holds row row(tid, r) at k kat(tid, r, j).
+template <int R, bool KFAST> struct StageP {
+    static constexpr int NR = KFAST ? R / 32 : 2;
+    static constexpr int NK = KFAST ? 2 : R / 32;
+    __device__ static int row(int tid, int r) { return KFAST ? tid / 8 + 32 * r : 2 * (tid % (R / 2)) + r; }
+    __device__ static int kidx(int tid, int j) { return KFAST ? 2 * (tid % 8) + j : tid / (R / 2) + (512 / R) * j; }
+    __device__ static int kat(int tid, int r, int j) { return kidx(tid, j); }
+    __device__ static int rat(int tid, int r, int j) { return row(tid, r); }
+};
+
 // Minimum resident waves per SIMD the register allocator must keep: the 64x64 tile keeps four (the
 // LDS of four workgroups fits a CU); larger tiles are limited by LDS anyway.
 template <int BM, int BN> struct MinWaves { static constexpr int v = (BM * BN <= 64 * 64) ? 4 : 1; };
@@ -107,8 +120,6 @@ void contract_kernel(const ContractArgs<T> p) {
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int LDA = BM + 1, LDB = BN + 1;
-    constexpr int A_PER_T = BM * kBK / kThreads;
-    constexpr int B_PER_T = BN * kBK / kThreads;
     constexpr int MAIN = 2 * kBK * LDA + 2 * kBK * LDB;
     constexpr int SMEM = C_MFAST ? CMax<MAIN, BN * (BM + 1)>::v : MAIN;
     __shared__ T smem[SMEM];
@@ -135,52 +146,96 @@ void contract_kernel(const ContractArgs<T> p) {
     const int64_t m_end = bt.m_end;
     const int n0 = tn * BN;
 
-    // ---- loader coordinates: one k row per thread and operand ----
-    // K-fast: 16 consecutive lanes walk k; M/N-fast: 16 consecutive lanes walk m/n.
-    const int a_kk = A_KFAST ? (tid % kBK) : (tid / 16);
-    const int a_m_base = A_KFAST ? (tid / kBK) : (tid % 16);
-    const int b_kk = B_NFAST ? (tid / 16) : (tid % kBK);
-    const int b_n_base = B_NFAST ? (tid % 16) : (tid / kBK);
-    int64_t a_row[A_PER_T];
+    // ---- loader coordinates: every thread holds element pairs ----
+    // K-contiguous operand: pair (k, k+1), kp = tid % 8, rows tid / 8 + 32 r (R/32 rows);
+    // row-contiguous operand: rows (2 i, 2 i + 1), i = tid % (R/2), at R/32 values of k.
+    // A pair is loaded as one vector when the host and the batch checks allow (vec below).
+    typedef StageP<BM, A_KFAST> SA;
+    typedef StageP<BN, !B_NFAST> SB;
+    int64_t a_row[SA::NR];
 #pragma unroll
-    for (int r = 0; r < A_PER_T; ++r) {
-        const int64_t m = m0 + a_m_base + 16 * r;
+    for (int r = 0; r < SA::NR; ++r) {
+        const int64_t m = m0 + SA::row(tid, r);
         a_row[r] = m < m_end ? bt.a_off + map_off(p.am, (uint32_t)m, batch) : 0;
     }
-    int64_t b_col[B_PER_T];
+    int64_t b_col[SB::NR];
 #pragma unroll
-    for (int r = 0; r < B_PER_T; ++r) {
-        const int n = n0 + b_n_base + 16 * r;
+    for (int r = 0; r < SB::NR; ++r) {
+        const int n = n0 + SB::row(tid, r);
         b_col[r] = n < p.N ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
     }
+    const bool vec = (p.vec & 3) == 3 && ((bt.a_off | bt.b_off) & 1) == 0 &&
+                     (A_KFAST || ((bt.m_begin | bt.m_end) & 1) == 0);
     // STAGES = 2: two register sets hold the tiles k+1 and k+2 while tile k is multiplied out of LDS,
     // so every global load has two K steps of MFMA work (plus the other resident waves) to hide
     // behind; STAGES = 1 keeps one set (fewer registers, more resident waves).
     // Loads are branch-free: rows/columns outside the operand read its offset 0 (they only feed
     // outputs that are never stored), and k beyond K reads offset 0 and is replaced by zero.
-    T a_reg[STAGES][A_PER_T], b_reg[STAGES][B_PER_T];
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    T a_reg[STAGES][SA::NR][SA::NK], b_reg[STAGES][SB::NR][SB::NK];
+    // one operand: X with row offsets rows[], K map km, into reg[NR][NK]
+    auto load_op = [&](const T *X, const int64_t *rows, const KMap &km, int k0, bool vfull, auto &reg,
+                       auto kfast_tag, auto geom) {
+        typedef decltype(geom) G;
+        if constexpr (decltype(kfast_tag)::value) {
+            const uint32_t k = (uint32_t)(k0 + G::kidx(tid, 0));
+            if (vfull) {
+                const int64_t o = koff(km, k);
+#pragma unroll
+                for (int r = 0; r < G::NR; ++r) {
+                    const T2 x = *reinterpret_cast<const T2 *>(X + rows[r] + o);
+                    reg[r][0] = x[0];
+                    reg[r][1] = x[1];
+                }
+            } else {
+#pragma unroll
+                for (int jj = 0; jj < 2; ++jj) {
+                    const uint32_t kj = k + jj;
+                    const bool ok = kj < (uint32_t)p.K;
+                    const int64_t o = koff(km, ok ? kj : 0u);
+#pragma unroll
+                    for (int r = 0; r < G::NR; ++r) {
+                        const T x = X[rows[r] + o];
+                        reg[r][jj] = ok ? x : T(0);
+                    }
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < G::NK; ++j) {
+                const uint32_t k = (uint32_t)(k0 + G::kidx(tid, j));
+                const bool ok = k < (uint32_t)p.K;
+                const int64_t o = koff(km, ok ? k : 0u);
+                if (vfull) {
+                    const T2 x = *reinterpret_cast<const T2 *>(X + rows[0] + o);
+                    reg[0][j] = x[0];
+                    reg[1][j] = x[1];
+                } else {
+#pragma unroll
+                    for (int r = 0; r < 2; ++r) {
+                        const T x = X[rows[r] + o];
+                        reg[r][j] = ok ? x : T(0);
+                    }
+                }
+            }
+        }
+    };
     auto load_tiles = [&](int s, int k0) {
-        const uint32_t ka = (uint32_t)(k0 + a_kk), kb = (uint32_t)(k0 + b_kk);
-        const bool ak_ok = ka < (uint32_t)p.K;
-        const bool bk_ok = kb < (uint32_t)p.K;
-        const int64_t aoff = koff(p.ak, ak_ok ? ka : 0u);
-        const int64_t boff = koff(p.bk, bk_ok ? kb : 0u);
-#pragma unroll
-        for (int r = 0; r < A_PER_T; ++r) {
-            const T x = p.A[a_row[r] + aoff];
-            a_reg[s][r] = ak_ok ? x : T(0);
-        }
-#pragma unroll
-        for (int r = 0; r < B_PER_T; ++r) {
-            const T x = p.B[b_col[r] + boff];
-            b_reg[s][r] = bk_ok ? x : T(0);
-        }
+        const bool vfull = vec && k0 + kBK <= p.K;   // the last, partial K tile loads per element
+        load_op(p.A, a_row, p.ak, k0, vfull, a_reg[s], std::integral_constant<bool, A_KFAST>{}, SA{});
+        load_op(p.B, b_col, p.bk, k0, vfull, b_reg[s], std::integral_constant<bool, !B_NFAST>{}, SB{});
     };
     auto store_tiles = [&](int s, int buf) {
 #pragma unroll
-        for (int r = 0; r < A_PER_T; ++r) As[buf][a_kk][a_m_base + 16 * r] = a_reg[s][r];
+        for (int r = 0; r < SA::NR; ++r)
 #pragma unroll
-        for (int r = 0; r < B_PER_T; ++r) Bs[buf][b_kk][b_n_base + 16 * r] = b_reg[s][r];
+            for (int j = 0; j < SA::NK; ++j)
+                As[buf][SA::kat(tid, r, j)][SA::rat(tid, r, j)] = a_reg[s][r][j];
+#pragma unroll
+        for (int r = 0; r < SB::NR; ++r)
+#pragma unroll
+            for (int j = 0; j < SB::NK; ++j)
+                Bs[buf][SB::kat(tid, r, j)][SB::rat(tid, r, j)] = b_reg[s][r][j];
     };
 
     const int wave = tid / 64, lane = tid % 64;
@@ -733,6 +788,27 @@ bool valid_map(const dcc_idxmap &m) {
     return m.e1 >= 1 && m.e2 >= 1 && m.e1 * m.e2 < (int64_t)1 << 32;
 }
 
+// Whether the element pairs a loader thread holds -- (k, k+1) of one row when the operand is staged
+// K-contiguous, rows (2i, 2i+1) at one k otherwise -- are adjacent in memory and aligned to the pair
+// size for every row and k (batch offsets and, for A, the row range parity are checked in the
+// kernel).  nrows: the row extent (N for B) must be even so no pair straddles its end (0 = checked
+// per batch in the kernel).
+bool pairs_contiguous(const void *base, const dcc_idxmap &rows, const dcc_idxmap &k, bool kfast,
+                      int64_t nrows, size_t pair_bytes) {
+    if (reinterpret_cast<uintptr_t>(base) % pair_bytes != 0 || rows.table != nullptr || k.table != nullptr)
+        return false;
+    const dcc_idxmap &pair = kfast ? k : rows;   // the map whose inner index the pair runs along
+    const dcc_idxmap &other = kfast ? rows : k;
+    if (inner_stride(pair) != 1) return false;
+    // pairs start at even indices: the inner level must have an even extent, or be the only level
+    if (pair.e2 > 1 ? pair.e2 % 2 != 0 : false) return false;
+    if (pair.e2 > 1 && ((pair.s0 | pair.s1) & 1)) return false;
+    if (pair.e2 <= 1 && pair.e1 > 1 && (pair.e1 % 2 != 0 || (pair.s0 & 1))) return false;
+    if ((other.s0 | other.s1 | other.s2) & 1) return false;
+    if (!kfast && nrows % 2 != 0) return false;
+    return true;
+}
+
 template <typename T>
 int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, const T *A, dcc_idxmap am,
                    dcc_idxmap ak, const T *B, dcc_idxmap bk, dcc_idxmap bn, T *C, dcc_idxmap cm, dcc_idxmap cn,
@@ -765,11 +841,12 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     args.bk = make_kmap(bk);
     args.batches = batches;
     args.total_tiles = nbatch;
-    args.vec = 0;
 
     // choose the contiguous index of each operand as the lane-fastest one
     const bool a_kfast = inner_stride(ak) == 1 || inner_stride(am) != 1;
     const bool b_nfast = inner_stride(bn) == 1 || inner_stride(bk) != 1;
+    args.vec = (pairs_contiguous(A, am, ak, a_kfast, 0, 2 * sizeof(T)) ? 1 : 0) |
+               (pairs_contiguous(B, bn, bk, !b_nfast, N, 2 * sizeof(T)) ? 2 : 0);
     // C written along m (through LDS) unless it is contiguous along n
     const bool c_mfast = inner_stride(cn) != 1;
     hipStream_t s = dcc::as_stream(stream);
@@ -836,26 +913,6 @@ int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool
     return 0;
 }
 
-// Whether the element pairs a split-BF16x3 loader thread holds -- (k, k+1) of one row when the
-// operand is staged K-contiguous, rows (2i, 2i+1) at one k otherwise -- are adjacent in memory and
-// 8-byte aligned for every row and k (batch offsets and, for A, the row range parity are checked
-// in the kernel).  nrows_even: the row extent (N for B) must be even so no pair straddles its end
-// (0 = checked per batch in the kernel).
-bool pairs_contiguous(const void *base, const dcc_idxmap &rows, const dcc_idxmap &k, bool kfast,
-                      int64_t nrows) {
-    if (reinterpret_cast<uintptr_t>(base) % 8 != 0 || rows.table != nullptr || k.table != nullptr) return false;
-    const dcc_idxmap &pair = kfast ? k : rows;   // the map whose inner index the pair runs along
-    const dcc_idxmap &other = kfast ? rows : k;
-    if (inner_stride(pair) != 1) return false;
-    // pairs start at even indices: the inner level must have an even extent, or be the only level
-    if (pair.e2 > 1 ? pair.e2 % 2 != 0 : false) return false;
-    if (pair.e2 > 1 && ((pair.s0 | pair.s1) & 1)) return false;
-    if (pair.e2 <= 1 && pair.e1 > 1 && (pair.e1 % 2 != 0 || (pair.s0 & 1))) return false;
-    if ((other.s0 | other.s1 | other.s2) & 1) return false;
-    if (!kfast && nrows % 2 != 0) return false;
-    return true;
-}
-
 int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
                           dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn, float *C,
                           dcc_idxmap cm, dcc_idxmap cn, const dcc_batch *batches, int64_t nbatch,
@@ -889,8 +946,8 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
     const bool a_kfast = inner_stride(ak) == 1 || inner_stride(am) != 1;
     const bool b_nfast = inner_stride(bn) == 1 || inner_stride(bk) != 1;
     const bool c_mfast = inner_stride(cn) != 1;
-    args.vec = (pairs_contiguous(A, am, ak, a_kfast, 0) ? 1 : 0) |
-               (pairs_contiguous(B, bn, bk, !b_nfast, N) ? 2 : 0);
+    args.vec = (pairs_contiguous(A, am, ak, a_kfast, 0, 8) ? 1 : 0) |
+               (pairs_contiguous(B, bn, bk, !b_nfast, N, 8) ? 2 : 0);
     hipStream_t s = dcc::as_stream(stream);
     // default: 128x128 unless one side of the problem is small
     int shape = (int)tile_shape;

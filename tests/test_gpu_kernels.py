@@ -35,18 +35,25 @@ def test_contract_plain_gemm(dev, M, N, K):
     bt[0] = (0, 0, 0, 0, M)
     dbt = DeviceArray.from_host(bt)
     lin = lambda s: L.IdxMap(None, 0, 1, 1 << 20, 0, 0, s)
-    L.call("dcc_contract_f64", M, N, K, 1.5, 0.5, dA.vp, lin(K), lin(1), dB.vp, lin(N), lin(1),
-           dC.vp, lin(N), lin(1), dbt.vp, 1, M, None)
-    C = dC.download()
     ref = 0.5 * C0 + 1.5 * A @ B
-    assert abs(C - ref).max() < 1e-12 * max(1.0, abs(ref).max())
+    for tile in range(-1, 8):   # default, then the four shapes x two prefetch depths
+        dC.upload(C0)
+        if tile < 0:
+            L.call("dcc_contract_f64", M, N, K, 1.5, 0.5, dA.vp, lin(K), lin(1), dB.vp, lin(N), lin(1),
+                   dC.vp, lin(N), lin(1), dbt.vp, 1, M, None)
+        else:
+            L.call("dcc_contract_tiled_f64", M, N, K, 1.5, 0.5, dA.vp, lin(K), lin(1), dB.vp, lin(N), lin(1),
+                   dC.vp, lin(N), lin(1), dbt.vp, 1, M, tile, None)
+        C = dC.download()
+        assert abs(C - ref).max() < 1e-12 * max(1.0, abs(ref).max()), tile
 
 
-def test_contract_transposed_operands(dev):
-    """A stored K-major and B stored N-major-transposed exercise the other two loaders."""
+@pytest.mark.parametrize("M,N,K", [(75, 41, 33), (76, 42, 34), (130, 138, 64)])
+def test_contract_transposed_operands(dev, M, N, K):
+    """A stored K-major and B stored N-major-transposed exercise the other two loaders (odd sizes:
+    per-element loads; even sizes: pair vector loads)."""
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
-    M, N, K = 75, 41, 33
     _say("contract transposed %dx%dx%d" % (M, N, K))
     rng = np.random.default_rng(5)
     At = rng.normal(size=(K, M))  # A[m,k] = At[k,m]
@@ -57,10 +64,16 @@ def test_contract_transposed_operands(dev):
     bt[0] = (0, 0, 0, 0, M)
     dbt = DeviceArray.from_host(bt)
     lin = lambda s: L.IdxMap(None, 0, 1, 1 << 20, 0, 0, s)
-    L.call("dcc_contract_f64", M, N, K, 1.0, 0.0, dA.vp, lin(1), lin(M), dB.vp, lin(1), lin(K),
-           dC.vp, lin(N), lin(1), dbt.vp, 1, M, None)
     ref = At.T @ Bt.T
-    assert abs(dC.download() - ref).max() < 1e-12 * abs(ref).max()
+    for tile in range(-1, 8):   # default, then the four shapes x two prefetch depths
+        dC.fill(0.0)
+        if tile < 0:
+            L.call("dcc_contract_f64", M, N, K, 1.0, 0.0, dA.vp, lin(1), lin(M), dB.vp, lin(1), lin(K),
+                   dC.vp, lin(N), lin(1), dbt.vp, 1, M, None)
+        else:
+            L.call("dcc_contract_tiled_f64", M, N, K, 1.0, 0.0, dA.vp, lin(1), lin(M), dB.vp, lin(1), lin(K),
+                   dC.vp, lin(N), lin(1), dbt.vp, 1, M, tile, None)
+        assert abs(dC.download() - ref).max() < 1e-12 * abs(ref).max(), tile
 
 
 @pytest.mark.parametrize("v", [1, 5, 8, 13, 20])
