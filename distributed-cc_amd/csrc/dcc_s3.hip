@@ -111,6 +111,59 @@ __global__ __launch_bounds__(kThreads) void s3_transform_kernel(T *__restrict__ 
     }
 }
 
+// Multi-source S3 transform: out[x] = beta*out[x] + sum_p c[p] * src_p.transpose(p)[x], with its own
+// source block src_p per permutation (for example the six permuted base blocks X[pi(ijk)] of a
+// symmetrised term).  The orbit tiles of each source are staged in LDS in turn and the six output
+// tiles accumulate in registers, so the output block is read and written once.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void s3_gather_kernel(T *__restrict__ out, const T *__restrict__ src,
+                                                             int nvir, double beta,
+                                                             const dcc_s3_gather_job *__restrict__ jobs,
+                                                             int norb) {
+    __shared__ T tiles[6][TE];
+    const int64_t job = blockIdx.x / norb;
+    const int orbit = blockIdx.x - (int)(job * norb);
+    int R[3];
+    decode_orbit(orbit, R[0], R[1], R[2]);
+    const dcc_s3_gather_job jb = jobs[job];
+    double acc[6][kPer];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) acc[s][h] = 0.0;
+    int64_t loaded = -1;
+    for (int p = 0; p < 6; ++p) {
+        if (jb.coef[p] == 0.0) continue;   // uniform across the workgroup
+        if (jb.src[p] != loaded) {
+            __syncthreads();                // the previous source's tiles are no longer read
+            load_orbit(tiles, src + jb.src[p], nvir, R);
+            __syncthreads();
+            loaded = jb.src[p];
+        }
+#pragma unroll
+        for (int s = 0; s < 6; ++s)
+#pragma unroll
+            for (int h = 0; h < kPer; ++h)
+                acc[s][h] += jb.coef[p] * permuted(tiles, s, p, threadIdx.x + h * kThreads);
+    }
+    const int64_t v = nvir, v2 = v * v;
+    T *dst = out + jb.dst;
+    for (int s = 0; s < 6; ++s) {
+        if (duplicate_tile(s, R)) continue;
+        const int t0 = R[kP[s][0]], t1 = R[kP[s][1]], t2 = R[kP[s][2]];
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) {
+            const int e = threadIdx.x + h * kThreads;
+            const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
+            if (g0 >= nvir || g1 >= nvir || g2 >= nvir) continue;
+            double val = acc[s][h];
+            T *o = dst + g0 * v2 + g1 * v + g2;
+            if (beta != 0.0) val += beta * (double)(*o);
+            *o = (T)val;
+        }
+    }
+}
+
 // Fused P3_full projection + denominator + t3 update + norms (one tile orbit).
 // Storage in T, arithmetic (projection, division, update, norms) in FP64.
 template <typename T>
@@ -204,6 +257,21 @@ __global__ __launch_bounds__(256) void reduce_pairs_kernel(const double *__restr
 namespace {
 
 template <typename T>
+int s3_gather(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_gather_job *jobs, int64_t njobs,
+              dcc_stream_t stream) {
+    DCC_CHECK(nvir >= 0 && njobs >= 0, "negative extent");
+    if (nvir == 0 || njobs == 0) return 0;
+    DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
+    const int64_t nt = (nvir + E - 1) / E;
+    const int64_t norb = n_orbits(nt);
+    DCC_CHECK(norb * njobs < ((int64_t)1 << 31), "grid too large");
+    hipLaunchKernelGGL(s3_gather_kernel<T>, dim3((unsigned)(norb * njobs)), dim3(kThreads), 0,
+                       dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)norb);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+template <typename T>
 int s3_transform(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_job *jobs, int64_t njobs,
                  dcc_stream_t stream) {
     DCC_CHECK(nvir >= 0 && njobs >= 0, "negative extent");
@@ -248,6 +316,16 @@ int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double be
 int dcc_s3_transform_f32(float *out, const float *src, int64_t nvir, double beta, const dcc_s3_job *jobs,
                          int64_t njobs, dcc_stream_t stream) {
     return s3_transform<float>(out, src, nvir, beta, jobs, njobs, stream);
+}
+
+int dcc_s3_gather_f64(double *out, const double *src, int64_t nvir, double beta, const dcc_s3_gather_job *jobs,
+                      int64_t njobs, dcc_stream_t stream) {
+    return s3_gather<double>(out, src, nvir, beta, jobs, njobs, stream);
+}
+
+int dcc_s3_gather_f32(float *out, const float *src, int64_t nvir, double beta, const dcc_s3_gather_job *jobs,
+                      int64_t njobs, dcc_stream_t stream) {
+    return s3_gather<float>(out, src, nvir, beta, jobs, njobs, stream);
 }
 
 int64_t dcc_r3_finalize_tiles(int64_t nvir) {

@@ -53,6 +53,7 @@ class GemmProblem(ctypes.Structure):
 BATCH_DTYPE = np.dtype([("a_off", np.int64), ("b_off", np.int64), ("c_off", np.int64),
                         ("m_begin", np.int64), ("m_end", np.int64)])
 S3JOB_DTYPE = np.dtype([("dst", np.int64), ("src", np.int64), ("coef", np.float64, (6,))])
+S3GATHER_DTYPE = np.dtype([("dst", np.int64), ("src", np.int64, (6,)), ("coef", np.float64, (6,))])
 assert BATCH_DTYPE.itemsize == ctypes.sizeof(Batch)
 assert S3JOB_DTYPE.itemsize == ctypes.sizeof(S3Job)
 
@@ -94,6 +95,8 @@ _SIGS = {
                                        IdxMap, c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_transform_f64": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_transform_f32": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
+    "dcc_s3_gather_f64": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
+    "dcc_s3_gather_f32": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_r3_finalize_tiles": ([c_i64], c_i64),
     "dcc_r3_finalize_f64": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp],
                             ctypes.c_int),

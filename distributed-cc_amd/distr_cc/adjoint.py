@@ -54,7 +54,7 @@ class T2T3Adjoint:
         self.n3 = nocc3(o)
         self.eng = TriplesEngine(o, v, stream=stream, device=device)
         self.fwd = [g for g in self.eng.launches if g.name.startswith("T2T3:")]
-        assert len(self.fwd) == 12
+        assert self.fwd
         f64 = np.float64
         self.lam = DeviceArray((self.n3, v, v, v), f64)
         self.lam_w = DeviceArray((o, o, o, v, v, v), f64)
@@ -105,7 +105,8 @@ class T2T3Adjoint:
         self.eng.W_oovo.upload(np.ascontiguousarray(np.asarray(W_vooo).transpose(2, 3, 0, 1)), self.stream)
 
     def forward(self):
-        """R <- T2->T3 part of r3 (canonical blocks) for the current inputs."""
+        """R <- T2->T3 part of r3 (canonical blocks) for the current inputs (the engine's T2->T3
+        launches: the base-term GEMMs and the permutation passes)."""
         self.eng.R.fill(0.0, self.stream)
         for g in self.fwd:
             g.run(self.stream)

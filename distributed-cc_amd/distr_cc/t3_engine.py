@@ -12,7 +12,8 @@ Data in HBM (FP64, C order):
 
 The residual follows distr_cc/rccsdt.py:368-433 term by term, grouped into
 batched gather-GEMMs (dcc_contract_f64):
-  * T2->T3 (rccsdt.py:388-399): 12 launches batched over the n3 blocks;
+  * T2->T3 (rccsdt.py:388-399): the base term X[ijk] for every ordered (i,j,k) (2 GEMMs), then one
+    S3 pass per simultaneous (ijk, abc) permutation into the canonical blocks;
   * F_vv and W_vvvv x T3 (rccsdt.py:400-402,424-426): 6 launches, N = (block, ...);
   * W_ovov/W_ovvo x T3 (rccsdt.py:540-596): with G the part-3 operator
         G[r;pq][a,b,c] = sum_{l,d} ( -W_ovov[l,b,r,d] Y[l][a,d,c] - W_ovov[l,c,r,d] Y[l][a,b,d]
@@ -237,6 +238,26 @@ class S3Launch:
                stream.handle if stream is not None else None)
 
 
+class S3GatherLaunch:
+    """One dcc_s3_gather call: out[dst] = beta*out[dst] + sum_p coef[p] src[src_p]^{TP_T3[p]} per job."""
+
+    def __init__(self, name, out, src, nvir, beta, jobs):
+        self.name, self.flops = name, 0.0
+        self.fn = "dcc_s3_gather_" + _suffix(out.dtype)
+        self.out, self.src, self.nvir, self.beta = out, src, int(nvir), float(beta)
+        arr = np.zeros(len(jobs), dtype=L.S3GATHER_DTYPE)
+        for n, (dst, srcs, coef) in enumerate(jobs):
+            arr[n]['dst'] = dst
+            arr[n]['src'] = srcs
+            arr[n]['coef'] = coef
+        self.jobs = DeviceArray.from_host(arr)
+        self.njobs = len(jobs)
+
+    def run(self, stream):
+        L.call(self.fn, self.out.vp, self.src.vp, self.nvir, self.beta, self.jobs.vp, self.njobs,
+               stream.handle if stream is not None else None)
+
+
 def s3_jobs_array(jobs):
     arr = np.zeros(len(jobs), dtype=L.S3JOB_DTYPE)
     for n, (dst, src, coef) in enumerate(jobs):
@@ -394,24 +415,7 @@ class TriplesEngine:
         blocks = [dict(I=i, J=j, K=k, B=n) for n, (i, j, k) in enumerate(self.triples)]
         # T2 -> T3 (rccsdt.py:388-399)
         Rb = Operand(self.R, 'Babc', (n3, v, v, v))
-        vvvo_terms = [('abd,dc->abc', 'J', 'IK'), ('acd,db->abc', 'K', 'IJ'), ('bad,dc->abc', 'I', 'JK'),
-                      ('bcd,da->abc', 'K', 'JI'), ('cad,db->abc', 'I', 'KJ'), ('cbd,da->abc', 'J', 'KI')]
-        for sub, wocc, tocc in vvvo_terms:
-            sa, rest = sub.split(',')
-            sb = rest.split('->')[0]
-            A = Operand(self.W_ovvv, 'w' + sa, (o, v, v, v))
-            B = Operand(self.t2, 'st' + sb, (o, o, v, v))
-            fixed = [dict(w=f[wocc], s=f[tocc[0]], t=f[tocc[1]], B=f['B']) for f in blocks]
-            launches.append(plan_term('T2T3:W_vvvo ' + sub, sub, A, B, Rb, 1.0, 1.0, fixed, ext))
-        vooo_terms = [('al,lbc->abc', 'IJ', 'K'), ('al,lcb->abc', 'IK', 'J'), ('bl,lac->abc', 'JI', 'K'),
-                      ('bl,lca->abc', 'JK', 'I'), ('cl,lab->abc', 'KI', 'J'), ('cl,lba->abc', 'KJ', 'I')]
-        for sub, wocc, tocc in vooo_terms:
-            sa, rest = sub.split(',')
-            sb = rest.split('->')[0]
-            A = Operand(self.W_oovo, 'uw' + sa, (o, o, v, o))
-            B = Operand(self.t2, sb[0] + 's' + sb[1:], (o, o, v, v))
-            fixed = [dict(u=f[wocc[0]], w=f[wocc[1]], s=f[tocc], B=f['B']) for f in blocks]
-            launches.append(plan_term('T2T3:W_vooo ' + sub, sub, A, B, Rb, -1.0, 1.0, fixed, ext))
+        launches += self._plan_t2t3()
         # F_vv and W_vvvv x T3 with the block index in N (rccsdt.py:400-402, 424-426)
         one = [dict()]
         for sub, tl in [('ad,Bdbc->Babc', 'Bdbc'), ('bd,Badc->Babc', 'Badc'), ('cd,Babd->Babc', 'Babd')]:
@@ -440,6 +444,52 @@ class TriplesEngine:
             coef[TP_T3.index({1: (0, 2, 1), 2: (1, 0, 2), 3: (2, 0, 1), 4: (1, 2, 0), 5: (2, 1, 0)}[s])] = 1.0
             jobs = [(q * v ** 3, q * v ** 3, coef) for q in range(n3)]
             launches.append(S3Launch('W_oooo/F_oo add s%d' % s, self.R, self.Xp, v, 1.0, jobs))
+        return launches
+
+    def _plan_t2t3(self):
+        """T2 -> T3 terms (rccsdt.py:388-399).
+
+        The twelve terms are the six simultaneous (ijk, abc) permutations of one base term
+            X[i,j,k][a,b,c] = sum_d W_vvvo[a,b,d,j] t2[i,k,d,c] - sum_l W_vooo[a,l,i,j] t2[l,k,b,c],
+        r3[ijk][abc] += sum_pi X[pi(ijk)][pi(abc)].  X is built for every ordered (i,j,k) by two GEMMs
+        (M = (j,a,b), N = (i,k,c), K = d and M = (i,j,a), N = (k,b,c), K = l; no batch tables), and
+        each permutation pi is one S3 pass adding the permuted source blocks into the canonical
+        targets.  Ordered triples are processed in chunks of the first index that fit
+        ``ring_g_bytes``."""
+        o, v = self.nocc, self.nvir
+        v3 = v ** 3
+        dt = self.dtype
+        ic = int(max(1, min(o, self.ring_g_bytes // max(1, o * o * v3 * dt.itemsize))))
+        self.Xt = DeviceArray((ic, o, o, v, v, v), dt)
+        ext = {x: v for x in 'abcd'}
+        ext.update({x: o for x in 'ijkl'})
+        one = [({}, {}, {})]
+        # source-position permutations of (i,j,k); the abc permutation of the S3 job is its inverse
+        pis = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+        launches = []
+        for i0 in range(0, o, ic):
+            n = min(ic, o - i0)
+            ext['i'] = n
+            X = Operand(self.Xt, 'ijkabc', (n, o, o, v, v, v))
+            A1 = Operand(self.W_ovvv, 'jabd', (o, v, v, v))
+            B1 = Operand(self.t2, 'ikdc', (n, o, v, v), base=i0 * o * v * v)
+            launches.append(plan_einsum('T2T3: X W_vvvo', 'jabd,ikdc->ijkabc', A1, B1, X, 1.0, 0.0, ext, one))
+            A2 = Operand(self.W_oovo, 'ijal', (n, o, v, o), base=i0 * o * v * o)
+            B2 = Operand(self.t2, 'lkbc', (o, o, v, v))
+            launches.append(plan_einsum('T2T3: X W_vooo', 'ijal,lkbc->ijkabc', A2, B2, X, -1.0, 1.0, ext, one))
+            # one gather pass: target q takes sum_pi X[pi(ijk)]^{pi^-1} over the sources in this chunk
+            jobs = []
+            for q, t in enumerate(self.triples):
+                srcs, coef = [0] * 6, np.zeros(6)
+                for pos in pis:
+                    src = (t[pos[0]], t[pos[1]], t[pos[2]])
+                    if i0 <= src[0] < i0 + n:
+                        p = TP_T3.index(tuple(pos.index(u) for u in range(3)))
+                        srcs[p] = (((src[0] - i0) * o + src[1]) * o + src[2]) * v3
+                        coef[p] = 1.0
+                if coef.any():
+                    jobs.append((q * v3, srcs, coef))
+            launches.append(S3GatherLaunch('T2T3: symmetrise', self.R, self.Xt, v, 1.0, jobs))
         return launches
 
     def _plan_ring(self):
@@ -500,15 +550,22 @@ class TriplesEngine:
                 launches.append(GemmLaunch('ring %s M%d' % (wname, mx), ov, v2, nt * ov, 1.0,
                                            0.0 if n == 0 else 1.0, W, am, ak, self.Y, bk, bn, self.Gs, cm, cn,
                                            batches, ov, 2.0 * len(chunk) * ov * v2 * ov * nt))
+            # one gather pass: each target takes its (up to three) part contributions of this chunk
+            contrib = {}
             for part in (3, 4, 5):
-                coef = np.zeros(6)
-                coef[TP_T3.index(perm[part])] = 1.0
-                jobs = []
+                pidx = TP_T3.index(perm[part])
                 for pq, (p, q) in chunk:
                     r0, r1 = rows(part, p, q)
                     for r in range(r0, r1):
-                        jobs.append((target(part, r, p, q) * v3, ((pq - c0) * o + r) * v3, coef))
-                launches.append(S3Launch('ring place part%d' % part, self.R, self.Gs, v, 1.0, jobs))
+                        contrib.setdefault(target(part, r, p, q), []).append((pidx, ((pq - c0) * o + r) * v3))
+            jobs = []
+            for tq, lst in sorted(contrib.items()):
+                srcs, coef = [0] * 6, np.zeros(6)
+                for pidx, off in lst:
+                    assert coef[pidx] == 0.0   # the three parts use three different permutations
+                    srcs[pidx], coef[pidx] = off, 1.0
+                jobs.append((tq * v3, srcs, coef))
+            launches.append(S3GatherLaunch('ring place', self.R, self.Gs, v, 1.0, jobs))
         return launches
 
 

@@ -188,6 +188,23 @@ int dcc_s3_transform_f32(float *out, const float *src, int64_t nvir, double beta
                          const dcc_s3_job *jobs, int64_t njobs, dcc_stream_t stream);
 
 /*
+ * Multi-source S3 transform: each job adds sum_p c[p] * (block at src[p]).transpose(p) to the out
+ * block at dst (src[p] ignored where c[p] == 0), reading the out block once.  Used for the
+ * symmetrisation sum_pi X[pi(ijk)][pi(abc)] of the T2->T3 base term and for placing the ring
+ * operator, which the reference evaluates as separate permuted einsums (distr_cc/rccsdt.py:388-399,
+ * 540-596).  Jobs of one call must have distinct dst blocks; out and src must not overlap.
+ */
+typedef struct dcc_s3_gather_job {
+    int64_t dst;
+    int64_t src[6];
+    double coef[6];
+} dcc_s3_gather_job;
+int dcc_s3_gather_f64(double *out, const double *src, int64_t nvir, double beta,
+                      const dcc_s3_gather_job *jobs, int64_t njobs, dcc_stream_t stream);
+int dcc_s3_gather_f32(float *out, const float *src, int64_t nvir, double beta,
+                      const dcc_s3_gather_job *jobs, int64_t njobs, dcc_stream_t stream);
+
+/*
  * Fused end of the triples update (distr_cc/rccsdt.py:947-971):
  *   r = (R - (1/6) sum_p R.transpose(p)) / (eia[i,a] + eia[j,b] + eia[k,c])
  *   t3 += r;  optionally r3_out = r;

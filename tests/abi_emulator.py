@@ -330,9 +330,33 @@ class Emulator:
         S = self.dev.view(_ptr(src), dt)
         raw = self.dev.view(_ptr(jobs), np.uint8, int(njobs) * 64)
         jb = raw.view(np.dtype([("dst", np.int64), ("src", np.int64), ("coef", np.float64, (6,))]))
+        # the device runs the jobs of one call concurrently: destinations must be distinct
+        assert len(set(int(d) for d in jb["dst"])) == len(jb), "S3 jobs of one call share a destination"
         for j in jb:
             x = S[int(j["src"]): int(j["src"]) + v ** 3].reshape(v, v, v).astype(np.float64)
             val = sum(c * x.transpose(p) for c, p in zip(j["coef"], _PERMS) if c != 0.0)
+            o = O[int(j["dst"]): int(j["dst"]) + v ** 3].reshape(v, v, v)
+            o[...] = ((beta * o.astype(np.float64) if beta != 0.0 else 0.0) + val).astype(dt)
+        return 0
+
+    def dcc_s3_gather_f64(self, *a):
+        return self._s3_gather(np.float64, *a)
+
+    def dcc_s3_gather_f32(self, *a):
+        return self._s3_gather(np.float32, *a)
+
+    def _s3_gather(self, dt, out, src, nvir, beta, jobs, njobs, stream):
+        v = int(nvir)
+        O = self.dev.view(_ptr(out), dt)
+        S = self.dev.view(_ptr(src), dt)
+        gdt = np.dtype([("dst", np.int64), ("src", np.int64, (6,)), ("coef", np.float64, (6,))])
+        jb = self.dev.view(_ptr(jobs), np.uint8, int(njobs) * gdt.itemsize).view(gdt)
+        assert len(set(int(d) for d in jb["dst"])) == len(jb), "S3 jobs of one call share a destination"
+        for j in jb:
+            val = 0.0
+            for c, p, so in zip(j["coef"], _PERMS, j["src"]):
+                if c != 0.0:
+                    val = val + c * S[int(so): int(so) + v ** 3].reshape(v, v, v).astype(np.float64).transpose(p)
             o = O[int(j["dst"]): int(j["dst"]) + v ** 3].reshape(v, v, v)
             o[...] = ((beta * o.astype(np.float64) if beta != 0.0 else 0.0) + val).astype(dt)
         return 0

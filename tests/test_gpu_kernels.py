@@ -100,6 +100,33 @@ def test_s3_transform_all_perms(dev, v):
         assert abs(out[n] - ref).max() < 1e-13, (n, v)
 
 
+@pytest.mark.parametrize("v", [1, 5, 8, 13, 20])
+def test_s3_gather_multi_source(dev, v):
+    """dcc_s3_gather_f64: a different source block per permutation, repeated sources, zero
+    coefficients and beta accumulation, against NumPy."""
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    _say("s3 gather v=%d" % v)
+    rng = np.random.default_rng(40 + v)
+    X = rng.normal(size=(5, v, v, v))
+    O0 = rng.normal(size=(3, v, v, v))
+    perms = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
+    v3 = v ** 3
+    jobs = [(0, [0, 1, 2, 3, 4, 0], np.array([1.0, -1.0, 0.5, 2.0, 0.0, 3.0])),
+            (2 * v3, [4, 4, 4, 4, 4, 4], np.ones(6)),
+            (1 * v3, [2, 0, 0, 0, 0, 1], np.array([1.5, 0.0, 0.0, 0.0, 0.0, -2.0]))]
+    arr = np.zeros(len(jobs), dtype=L.S3GATHER_DTYPE)
+    for n, (dst, srcs, coef) in enumerate(jobs):
+        arr[n]['dst'], arr[n]['src'], arr[n]['coef'] = dst, [x * v3 for x in srcs], coef
+    dX, dO, dj = DeviceArray.from_host(X), DeviceArray.from_host(O0), DeviceArray.from_host(arr)
+    L.call("dcc_s3_gather_f64", dO.vp, dX.vp, v, 0.5, dj.vp, len(jobs), None)
+    out = dO.download()
+    for dst, srcs, coef in jobs:
+        q = dst // v3
+        ref = 0.5 * O0[q] + sum(c * X[sx].transpose(p) for c, sx, p in zip(coef, srcs, perms) if c != 0.0)
+        assert abs(out[q] - ref).max() < 1e-13 * max(1.0, abs(ref).max()), (q, v)
+
+
 @pytest.mark.parametrize("o,v", [(1, 3), (3, 4), (4, 9), (5, 17), (3, 24)])
 def test_residual_and_update_vs_oracle(dev, o, v):
     from distr_cc.synthetic import SyntheticSystem
