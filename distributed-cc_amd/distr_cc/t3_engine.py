@@ -325,7 +325,9 @@ class TriplesEngine:
         self.WR3a = DeviceArray((o, v, o, v), dt, ptr=self.WR3.ptr, owner=False)
         self.WR3b = DeviceArray((o, v, o, v), dt, ptr=self.WR3.ptr + w3, owner=False)
         self.pfs = DeviceArray((6, n3, n3), dt)
-        self.Xp = DeviceArray((n3, v, v, v), dt)     # one W_oooo/F_oo term before its permutation
+        # W_oooo/F_oo terms s = 1..5 before their permutation, for a chunk of target blocks
+        qc = int(max(1, min(n3, (ring_g_bytes // max(1, 5 * v3 * dt.itemsize)))))
+        self.Xp = DeviceArray((5, qc, v, v, v), dt)
         self.eia = DeviceArray((o, v))
         self.norb = int(L._lib.dcc_r3_finalize_tiles(v))
         self.partials = DeviceArray((n3 * self.norb, 2))
@@ -437,13 +439,26 @@ class TriplesEngine:
         B = Operand(self.t3, 'pabc', (n3, v, v, v))
         Rq = Operand(self.R, 'qabc', (n3, v, v, v))
         launches.append(plan_term('W_oooo/F_oo s0', 'pq,pabc->qabc', A, B, Rq, 1.0, 1.0, [dict(S=0)], ext))
-        X = Operand(self.Xp, 'qabc', (n3, v, v, v))
-        for s in range(1, 6):
-            launches.append(plan_term('W_oooo/F_oo s%d' % s, 'pq,pabc->qabc', A, B, X, 1.0, 0.0, [dict(S=s)], ext))
-            coef = np.zeros(6)
-            coef[TP_T3.index({1: (0, 2, 1), 2: (1, 0, 2), 3: (2, 0, 1), 4: (1, 2, 0), 5: (2, 1, 0)}[s])] = 1.0
-            jobs = [(q * v ** 3, q * v ** 3, coef) for q in range(n3)]
-            launches.append(S3Launch('W_oooo/F_oo add s%d' % s, self.R, self.Xp, v, 1.0, jobs))
+        # s = 1..5 for a chunk of targets q at a time into Xp[s-1][q - q0], then one gather pass
+        v3 = v ** 3
+        perm = {1: (0, 2, 1), 2: (1, 0, 2), 3: (2, 0, 1), 4: (1, 2, 0), 5: (2, 1, 0)}
+        qc = self.Xp.shape[1]
+        for q0 in range(0, n3, qc):
+            nq = min(qc, n3 - q0)
+            ext_c = dict(ext, q=nq)
+            Aq = Operand.strided(self.pfs, {'S': n3 * n3, 'p': n3, 'q': 1}, base=q0)
+            for s in range(1, 6):
+                X = Operand(self.Xp, 'qabc', (nq, v, v, v), base=(s - 1) * qc * v3)
+                launches.append(plan_term('W_oooo/F_oo s%d' % s, 'pq,pabc->qabc', Aq, B, X, 1.0, 0.0,
+                                          [dict(S=s)], ext_c))
+            jobs = []
+            for q in range(q0, q0 + nq):
+                srcs, coef = [0] * 6, np.zeros(6)
+                for s in range(1, 6):
+                    p = TP_T3.index(perm[s])
+                    srcs[p], coef[p] = ((s - 1) * qc + q - q0) * v3, 1.0
+                jobs.append((q * v3, srcs, coef))
+            launches.append(S3GatherLaunch('W_oooo/F_oo add', self.R, self.Xp, v, 1.0, jobs))
         return launches
 
     def _plan_t2t3(self):

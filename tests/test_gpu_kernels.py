@@ -151,7 +151,10 @@ def test_residual_and_update_vs_oracle(dev, o, v):
     t3_new = eng.get_t3()
     t3o, r3o, n2o, nao = tri.t3_update(o, Ro, t3, s.mo_energy, level_shift=0.05, nvir_diis=nd)
     assert abs(t3_new - t3o).max() <= 1e-13 * abs(t3o).max()
-    assert abs(n2 - n2o) <= 1e-12 * n2o + 1e-30 and abs(na - nao) <= 1e-12 * nao + 1e-30
+    # the projection cancels most of R when o is small (o = 1: n2 ~ 1e-10 of sum R^2), so the norms
+    # also carry an absolute round-off floor relative to sum R^2
+    floor = 1e-20 * float((Ro ** 2).sum()) + 1e-30
+    assert abs(n2 - n2o) <= 1e-12 * n2o + floor and abs(na - nao) <= 1e-12 * nao + floor
 
 
 @pytest.mark.parametrize("o,v", [(1, 3), (3, 7), (4, 12), (5, 19)])
