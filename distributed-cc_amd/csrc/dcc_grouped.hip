@@ -1,52 +1,35 @@
 // Grouped persistent gather-GEMM: one launch runs a list of independent contractions (for example
 // the same residual term of many small CC systems).  Each problem is a dcc_contract_f64 call in
 // descriptor form; the library tiles every problem, concatenates the tile ranges, and a fixed grid of
-// workgroups loops over the global tile list (tile t, t + gridDim.x, ...), locating the problem of
-// each tile by binary search over the per-problem tile offsets.  Every workgroup leaves the loop when
-// the list is exhausted, so the launch always drains.  Problems of one launch must write disjoint C
-// regions (no two problems accumulate into the same element).
+// workgroups (as many as are resident at once) walks the global tile list in windows of one grid:
+// in window w, workgroup b takes tile w * grid + xcd_remap(b), so the tiles one XCD runs together are
+// neighbours that share A and B panels in its L2.  The problem of a tile is found by binary search
+// over the per-problem tile offsets, and the tile itself is the same code as the one-launch-per-GEMM
+// kernel (dcc_gemm_tile.h: pair loads, one- or two-stage register prefetch, branch-free K maps).
+// Every workgroup leaves the loop when the list is exhausted, so the launch always drains.  Problems
+// of one launch must write disjoint C regions (no two problems accumulate into the same element).
 #include "dcc_common.h"
+#include "dcc_gemm_tile.h"
 
 #include <algorithm>
 #include <vector>
 
 namespace {
 
-typedef double dbl4 __attribute__((ext_vector_type(4)));
-constexpr int kThreads = 256;
-constexpr int kBK = 16;
-constexpr int kGroupM = 8;
+using namespace dcc_gemm;
 
-__device__ __forceinline__ int64_t gmap(const dcc_idxmap &mp, uint32_t x, int64_t batch) {
-    if (mp.table != nullptr) return mp.table[batch * mp.table_bstride + x];
-    const uint32_t e2 = (uint32_t)mp.e2;
-    const uint32_t e12 = (uint32_t)(mp.e1 * mp.e2);
-    const uint32_t q0 = x / e12;
-    const uint32_t r0 = x - q0 * e12;
-    const uint32_t q1 = r0 / e2;
-    const uint32_t q2 = r0 - q1 * e2;
-    return (int64_t)q0 * mp.s0 + (int64_t)q1 * mp.s1 + (int64_t)q2 * mp.s2;
-}
-
-template <int BM, int BN>
-__global__ __launch_bounds__(kThreads) void grouped_kernel(const dcc_gemm_problem *__restrict__ probs, int nprob,
-                                                           int64_t total_tiles) {
-    constexpr int WM = BM / 2, WN = BN / 2;
-    constexpr int FM = WM / 16, FN = WN / 16;
-    constexpr int LDA = BM + 1, LDB = BN + 1;
-    constexpr int A_PER_T = BM * kBK / kThreads;
-    constexpr int B_PER_T = BN * kBK / kThreads;
-    constexpr int MAIN = 2 * kBK * LDA + 2 * kBK * LDB;
-    constexpr int SMEM = MAIN > BN * (BM + 1) ? MAIN : BN * (BM + 1);
-    __shared__ double smem[SMEM];
-    double (*As)[kBK][LDA] = reinterpret_cast<double (*)[kBK][LDA]>(smem);
-    double (*Bs)[kBK][LDB] = reinterpret_cast<double (*)[kBK][LDB]>(smem + 2 * kBK * LDA);
-    const int tid = threadIdx.x;
-    const int wave = tid / 64, lane = tid % 64;
-    const int wm = wave / 2, wn = wave % 2;
-    const int lrow = lane % 16, lk = lane / 16;
-
-    for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
+// flags of a prepared problem: bits 0..2 the loader/epilogue choice of the launch (A K-contiguous,
+// B N-contiguous, C written along m; the same for every problem of a launch), bits 3..4 the pair
+// vector loads allowed for A and B of this problem under that choice.
+template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int STAGES>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MinWaves<BM, BN, STAGES>::v)))
+void grouped_kernel(const dcc_gemm_problem *__restrict__ probs, int nprob, int64_t total_tiles) {
+    __shared__ double smem[TileSmem<BM, BN, C_MFAST>::v];
+    const int64_t grid = gridDim.x;
+    for (int64_t base = 0; base < total_tiles; base += grid) {
+        const int64_t window = min(grid, total_tiles - base);
+        if ((int64_t)blockIdx.x >= window) break;
+        const int64_t t = base + xcd_remap((int64_t)blockIdx.x, window);
         // problem owning tile t: last p with tile_begin <= t
         int lo = 0, hi = nprob - 1;
         while (lo < hi) {
@@ -54,228 +37,147 @@ __global__ __launch_bounds__(kThreads) void grouped_kernel(const dcc_gemm_proble
             if (probs[mid].tile_begin <= t) lo = mid;
             else hi = mid - 1;
         }
-        const dcc_gemm_problem &p = probs[lo];
-        const bool a_kfast = (p.flags & 1) != 0, b_nfast = (p.flags & 2) != 0, c_mfast = (p.flags & 4) != 0;
-        const int64_t local = t - p.tile_begin;
-        const int64_t per_batch = p.tiles_m * p.tiles_n;
+        const dcc_gemm_problem &q = probs[lo];
+        ContractArgs<double> a;
+        a.M = 0;
+        a.N = (int)q.N;
+        a.K = (int)q.K;
+        a.alpha = q.alpha;
+        a.beta = q.beta;
+        a.A = static_cast<const double *>(q.A);
+        a.B = static_cast<const double *>(q.B);
+        a.C = static_cast<double *>(q.C);
+        a.am = q.am;
+        a.bn = q.bn;
+        a.cm = q.cm;
+        a.cn = q.cn;
+        a.ak = make_kmap(q.ak);
+        a.bk = make_kmap(q.bk);
+        a.batches = q.batches;
+        a.tiles_m = (int)q.tiles_m;
+        a.tiles_n = (int)q.tiles_n;
+        a.total_tiles = 0;
+        a.vec = (int)((q.flags >> 3) & 3);
+        const int64_t local = t - q.tile_begin;
+        const int64_t per_batch = q.tiles_m * q.tiles_n;
         const int64_t batch = local / per_batch;
-        const int64_t pid = local - batch * per_batch;
-        const int64_t group_span = (int64_t)kGroupM * p.tiles_n;
-        const int64_t group = pid / group_span;
-        const int first_m = (int)(group * kGroupM);
-        const int gsize = (int)min((int64_t)kGroupM, p.tiles_m - first_m);
-        const int64_t in_group = pid - group * group_span;
-        const int tm = first_m + (int)(in_group % gsize);
-        const int tn = (int)(in_group / gsize);
-        const dcc_batch bt = p.batches[batch];
-        const int64_t m0 = bt.m_begin + (int64_t)tm * BM;
-        if (m0 >= bt.m_end) continue;  // uniform across the workgroup
-        const int64_t m_end = bt.m_end;
-        const int n0 = tn * BN;
-        const int N = (int)p.N, K = (int)p.K;
-        const double *A = static_cast<const double *>(p.A);
-        const double *B = static_cast<const double *>(p.B);
-        double *C = static_cast<double *>(p.C);
-
-        const int a_kk = a_kfast ? (tid % kBK) : (tid / 16);
-        const int a_m_base = a_kfast ? (tid / kBK) : (tid % 16);
-        const int b_kk = b_nfast ? (tid / 16) : (tid % kBK);
-        const int b_n_base = b_nfast ? (tid % 16) : (tid / kBK);
-        int64_t a_row[A_PER_T];
-        uint32_t a_ok = 0;
-#pragma unroll
-        for (int r = 0; r < A_PER_T; ++r) {
-            const int64_t m = m0 + a_m_base + 16 * r;
-            const bool ok = m < m_end;
-            a_ok |= (ok ? 1u : 0u) << r;
-            a_row[r] = ok ? bt.a_off + gmap(p.am, (uint32_t)m, batch) : 0;
-        }
-        int64_t b_col[B_PER_T];
-        uint32_t b_ok = 0;
-#pragma unroll
-        for (int r = 0; r < B_PER_T; ++r) {
-            const int n = n0 + b_n_base + 16 * r;
-            const bool ok = n < N;
-            b_ok |= (ok ? 1u : 0u) << r;
-            b_col[r] = ok ? bt.b_off + gmap(p.bn, (uint32_t)n, batch) : 0;
-        }
-        const dcc::KMap akm = dcc::make_kmap(p.ak), bkm = dcc::make_kmap(p.bk);
-        double a_reg[A_PER_T], b_reg[B_PER_T];
-        // branch-free loads: out-of-range elements read offset 0 and are replaced by zero
-        auto load_tiles = [&](int k0) {
-            const uint32_t ka = (uint32_t)(k0 + a_kk), kb = (uint32_t)(k0 + b_kk);
-            const bool ak_ok = ka < (uint32_t)K;
-            const bool bk_ok = kb < (uint32_t)K;
-            const int64_t aoff = dcc::koff(akm, ak_ok ? ka : 0u);
-            const int64_t boff = dcc::koff(bkm, bk_ok ? kb : 0u);
-#pragma unroll
-            for (int r = 0; r < A_PER_T; ++r) {
-                const bool ok = ak_ok && ((a_ok >> r) & 1u);
-                const double x = A[ok ? a_row[r] + aoff : 0];
-                a_reg[r] = ok ? x : 0.0;
-            }
-#pragma unroll
-            for (int r = 0; r < B_PER_T; ++r) {
-                const bool ok = bk_ok && ((b_ok >> r) & 1u);
-                const double x = B[ok ? b_col[r] + boff : 0];
-                b_reg[r] = ok ? x : 0.0;
-            }
-        };
-        auto store_tiles = [&](int buf) {
-#pragma unroll
-            for (int r = 0; r < A_PER_T; ++r) As[buf][a_kk][a_m_base + 16 * r] = a_reg[r];
-#pragma unroll
-            for (int r = 0; r < B_PER_T; ++r) Bs[buf][b_kk][b_n_base + 16 * r] = b_reg[r];
-        };
-        dbl4 acc[FM][FN];
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j) acc[i][j] = dbl4{0, 0, 0, 0};
-        const int nk = (K + kBK - 1) / kBK;
-        __syncthreads();  // LDS of the previous tile is no longer read
-        load_tiles(0);
-        store_tiles(0);
-        __syncthreads();
-        for (int kt = 0; kt < nk; ++kt) {
-            const int buf = kt & 1;
-            load_tiles((kt + 1) * kBK);  // past the last tile: offset-0 reads into the idle buffer
-#pragma unroll
-            for (int ks = 0; ks < kBK / 4; ++ks) {
-                const int kr = ks * 4 + lk;
-                double af[FM], bf[FN];
-#pragma unroll
-                for (int i = 0; i < FM; ++i) af[i] = As[buf][kr][wm * WM + i * 16 + lrow];
-#pragma unroll
-                for (int j = 0; j < FN; ++j) bf[j] = Bs[buf][kr][wn * WN + j * 16 + lrow];
-#pragma unroll
-                for (int i = 0; i < FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < FN; ++j)
-                        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[i], bf[j], acc[i][j], 0, 0, 0);
-            }
-            store_tiles(buf ^ 1);
-            __syncthreads();
-        }
-        if (c_mfast) {
-            double *Cs = smem;
-#pragma unroll
-            for (int i = 0; i < FM; ++i)
-#pragma unroll
-                for (int j = 0; j < FN; ++j)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        Cs[(wn * WN + j * 16 + lrow) * (BM + 1) + wm * WM + i * 16 + lk + 4 * r] = acc[i][j][r];
-            __syncthreads();
-            constexpr int NSTEP = kThreads / BM;
-            const int ml = tid % BM;
-            const int64_t m = m0 + ml;
-            if (m < m_end) {
-                const int64_t crow = bt.c_off + gmap(p.cm, (uint32_t)m, batch);
-                for (int nl = tid / BM; nl < BN; nl += NSTEP) {
-                    const int n = n0 + nl;
-                    if (n >= N) break;
-                    double *c = C + crow + gmap(p.cn, (uint32_t)n, batch);
-                    double v = p.alpha * Cs[nl * (BM + 1) + ml];
-                    if (p.beta != 0.0) v += p.beta * *c;
-                    *c = v;
-                }
-            }
-            continue;  // the loop-top barrier protects Cs before the next tile's stores
-        }
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t m = m0 + wm * WM + i * 16 + lk + 4 * r;
-                if (m >= m_end) continue;
-                const int64_t crow = bt.c_off + gmap(p.cm, (uint32_t)m, batch);
-#pragma unroll
-                for (int j = 0; j < FN; ++j) {
-                    const int n = n0 + wn * WN + j * 16 + lrow;
-                    if (n >= N) continue;
-                    double *c = C + crow + gmap(p.cn, (uint32_t)n, batch);
-                    double v = p.alpha * acc[i][j][r];
-                    if (p.beta != 0.0) v += p.beta * *c;
-                    *c = v;
-                }
-            }
+        int tm, tn;
+        tile_coords(local - batch * per_batch, a.tiles_m, a.tiles_n, tm, tn);
+        __syncthreads();  // the previous tile's epilogue may still read LDS
+        contract_tile<double, BM, BN, A_KFAST, B_NFAST, C_MFAST, STAGES>(a, batch, tm, tn, smem);
     }
 }
 
-int64_t inner_stride(const dcc_idxmap &m) {
-    if (m.table != nullptr) return 0;
-    if (m.e2 > 1) return m.s2;
-    if (m.e1 > 1) return m.s1;
-    return m.s0;
-}
-
-bool tile_shape(int64_t shape, int *bm, int *bn) {
-    switch (shape) {
-        case 0: *bm = 64; *bn = 64; return true;
-        case 1: *bm = 128; *bn = 64; return true;
-        case 2: *bm = 64; *bn = 128; return true;
-        case 3: *bm = 128; *bn = 128; return true;
-        default: return false;
+template <int BM, int BN, int STAGES> struct GroupedLauncher {
+    template <bool AK, bool BN_, bool CM>
+    static int go(int64_t grid, const dcc_gemm_problem *probs, int nprob, int64_t total, hipStream_t s) {
+        auto kern = grouped_kernel<BM, BN, AK, BN_, CM, STAGES>;
+        if (grid == 0) {
+            // resident workgroups per CU of this kernel (registers and LDS), queried once
+            static int per_cu = 0;
+            int dev = 0, cus = 0;
+            DCC_HIP(hipGetDevice(&dev));
+            DCC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+            if (per_cu == 0) {
+                int n = 0;
+                DCC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kThreads, 0));
+                per_cu = std::max(n, 1);
+            }
+            grid = (int64_t)cus * per_cu;
+        }
+        if (grid > total) grid = total;
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(kThreads), 0, s, probs, nprob, total);
+        DCC_LAUNCH_CHECK();
+        return 0;
     }
+    static int run(int flags, int64_t grid, const dcc_gemm_problem *probs, int nprob, int64_t total, hipStream_t s) {
+        switch (flags & 7) {
+            case 0: return go<false, false, false>(grid, probs, nprob, total, s);
+            case 1: return go<true, false, false>(grid, probs, nprob, total, s);
+            case 2: return go<false, true, false>(grid, probs, nprob, total, s);
+            case 3: return go<true, true, false>(grid, probs, nprob, total, s);
+            case 4: return go<false, false, true>(grid, probs, nprob, total, s);
+            case 5: return go<true, false, true>(grid, probs, nprob, total, s);
+            case 6: return go<false, true, true>(grid, probs, nprob, total, s);
+            default: return go<true, true, true>(grid, probs, nprob, total, s);
+        }
+    }
+};
+
+// Loader/epilogue choice of one problem, as dcc_contract_f64 makes it.
+int problem_flags(const dcc_gemm_problem &p) {
+    const bool a_kfast = inner_stride(p.ak) == 1 || inner_stride(p.am) != 1;
+    const bool b_nfast = inner_stride(p.bn) == 1 || inner_stride(p.bk) != 1;
+    const bool c_mfast = inner_stride(p.cn) != 1;
+    return (a_kfast ? 1 : 0) | (b_nfast ? 2 : 0) | (c_mfast ? 4 : 0);
 }
 
 }  // namespace
 
 extern "C" {
 
-int dcc_grouped_prepare(dcc_gemm_problem *probs, int64_t nprob, int64_t tile_shape_id, int64_t *total_tiles) {
-    DCC_CHECK(nprob >= 0 && total_tiles != nullptr, "bad arguments");
+int dcc_grouped_prepare(dcc_gemm_problem *probs, int64_t nprob, int64_t tile_shape_id, int64_t *total_tiles,
+                        int64_t *variant) {
+    DCC_CHECK(nprob >= 0 && total_tiles != nullptr && variant != nullptr, "bad arguments");
     DCC_CHECK(nprob == 0 || probs != nullptr, "null problems");
-    int bm = 0, bn = 0;
-    DCC_CHECK(tile_shape(tile_shape_id, &bm, &bn), "tile shape must be 0..3");
+    int bm = 0, bn = 0, stages = 0;
+    DCC_CHECK(tile_shape_of(tile_shape_id, bm, bn, stages), "tile shape must be 0..7");
+    // The loaders and epilogue are correct for any strides; their choice only decides coalescing.
+    // One launch has one choice: the one of the problems with the most multiply-adds.
+    double weight[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = 0; i < nprob; ++i) {
+        const dcc_gemm_problem &p = probs[i];
+        DCC_CHECK(p.N >= 0 && p.K >= 0 && p.nbatch >= 0 && p.max_rows >= 0, "negative extent");
+        DCC_CHECK(p.N < ((int64_t)1 << 31) && p.K < ((int64_t)1 << 31), "extent exceeds 2^31");
+        DCC_CHECK(p.ak.table == nullptr && p.bk.table == nullptr, "K index maps must be mixed-radix maps");
+        for (const dcc_idxmap *m : {&p.am, &p.ak, &p.bk, &p.bn, &p.cm, &p.cn})
+            DCC_CHECK(m->table != nullptr ? m->table_bstride >= 0
+                                          : m->e1 >= 1 && m->e2 >= 1 && m->e1 * m->e2 < ((int64_t)1 << 32),
+                      "invalid index map");
+        weight[problem_flags(p)] += (double)p.N * (double)p.K * (double)p.max_rows * (double)p.nbatch;
+    }
+    const int flags = (int)(std::max_element(weight, weight + 8) - weight);
+    const bool a_kfast = flags & 1, b_nfast = (flags & 2) != 0;
     int64_t acc = 0;
     for (int64_t i = 0; i < nprob; ++i) {
         dcc_gemm_problem &p = probs[i];
-        DCC_CHECK(p.N >= 0 && p.K >= 0 && p.nbatch >= 0 && p.max_rows >= 0, "negative extent");
-        DCC_CHECK(p.ak.table == nullptr && p.bk.table == nullptr, "K index maps must be mixed-radix maps");
         p.tiles_m = (p.max_rows + bm - 1) / bm;
         p.tiles_n = (p.N + bn - 1) / bn;
         p.tile_begin = acc;
-        const bool a_kfast = inner_stride(p.ak) == 1 || inner_stride(p.am) != 1;
-        const bool b_nfast = inner_stride(p.bn) == 1 || inner_stride(p.bk) != 1;
-        const bool c_mfast = inner_stride(p.cn) != 1;
-        p.flags = (a_kfast ? 1 : 0) | (b_nfast ? 2 : 0) | (c_mfast ? 4 : 0);
+        const int vec = (pairs_contiguous(p.A, p.am, p.ak, a_kfast, 0, 16) ? 1 : 0) |
+                        (pairs_contiguous(p.B, p.bn, p.bk, !b_nfast, p.N, 16) ? 2 : 0);
+        p.flags = flags | (vec << 3);
         acc += p.nbatch * p.tiles_m * p.tiles_n;
     }
     DCC_CHECK(acc < ((int64_t)1 << 40), "too many tiles");
     *total_tiles = acc;
+    *variant = tile_shape_id | ((int64_t)flags << 3);
     return 0;
 }
 
-int dcc_contract_grouped_f64(const dcc_gemm_problem *probs, int64_t nprob, int64_t total_tiles,
-                             int64_t tile_shape_id, int64_t nworkgroups, dcc_stream_t stream) {
+int dcc_contract_grouped_f64(const dcc_gemm_problem *probs, int64_t nprob, int64_t total_tiles, int64_t variant,
+                             int64_t nworkgroups, dcc_stream_t stream) {
     DCC_CHECK(nprob >= 0 && total_tiles >= 0 && nworkgroups >= 0, "negative argument");
     DCC_CHECK(nprob < (1 << 30), "too many problems");
+    DCC_CHECK(variant >= 0 && variant < 64, "variant must come from dcc_grouped_prepare");
     if (nprob == 0 || total_tiles == 0) return 0;
     DCC_CHECK(probs != nullptr, "null problems");
-    int bm = 0, bn = 0;
-    DCC_CHECK(tile_shape(tile_shape_id, &bm, &bn), "tile shape must be 0..3");
-    int64_t grid = nworkgroups;
-    if (grid == 0) {
-        int dev = 0, cus = 0;
-        DCC_HIP(hipGetDevice(&dev));
-        DCC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        grid = (int64_t)cus * 4;  // persistent: a few resident workgroups per CU
-    }
-    if (grid > total_tiles) grid = total_tiles;
-    const dim3 g((unsigned)grid), b(kThreads);
+    int bm = 0, bn = 0, stages = 0;
+    tile_shape_of(variant & 7, bm, bn, stages);
+    const int flags = (int)(variant >> 3);
     hipStream_t s = dcc::as_stream(stream);
+    const int np = (int)nprob;
     if (bm == 64 && bn == 64)
-        hipLaunchKernelGGL((grouped_kernel<64, 64>), g, b, 0, s, probs, (int)nprob, total_tiles);
-    else if (bm == 128 && bn == 64)
-        hipLaunchKernelGGL((grouped_kernel<128, 64>), g, b, 0, s, probs, (int)nprob, total_tiles);
-    else if (bm == 64 && bn == 128)
-        hipLaunchKernelGGL((grouped_kernel<64, 128>), g, b, 0, s, probs, (int)nprob, total_tiles);
-    else
-        hipLaunchKernelGGL((grouped_kernel<128, 128>), g, b, 0, s, probs, (int)nprob, total_tiles);
-    DCC_LAUNCH_CHECK();
-    return 0;
+        return stages == 2 ? GroupedLauncher<64, 64, 2>::run(flags, nworkgroups, probs, np, total_tiles, s)
+                           : GroupedLauncher<64, 64, 1>::run(flags, nworkgroups, probs, np, total_tiles, s);
+    if (bm == 128 && bn == 64)
+        return stages == 2 ? GroupedLauncher<128, 64, 2>::run(flags, nworkgroups, probs, np, total_tiles, s)
+                           : GroupedLauncher<128, 64, 1>::run(flags, nworkgroups, probs, np, total_tiles, s);
+    if (bm == 64 && bn == 128)
+        return stages == 2 ? GroupedLauncher<64, 128, 2>::run(flags, nworkgroups, probs, np, total_tiles, s)
+                           : GroupedLauncher<64, 128, 1>::run(flags, nworkgroups, probs, np, total_tiles, s);
+    return stages == 2 ? GroupedLauncher<128, 128, 2>::run(flags, nworkgroups, probs, np, total_tiles, s)
+                       : GroupedLauncher<128, 128, 1>::run(flags, nworkgroups, probs, np, total_tiles, s);
 }
 
 int dcc_graph_capture_begin(dcc_stream_t stream) {

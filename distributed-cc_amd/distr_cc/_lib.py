@@ -114,7 +114,7 @@ _SIGS = {
     "dcc_tensor_axpby_f32": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f32_f64": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f64_f32": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
-    "dcc_grouped_prepare": ([c_vp, c_i64, c_i64, ctypes.POINTER(c_i64)], ctypes.c_int),
+    "dcc_grouped_prepare": ([c_vp, c_i64, c_i64, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64)], ctypes.c_int),
     "dcc_contract_grouped_f64": ([c_vp, c_i64, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_graph_capture_begin": ([c_vp], ctypes.c_int),
     "dcc_graph_capture_end": ([c_vp, ctypes.POINTER(c_vp)], ctypes.c_int),

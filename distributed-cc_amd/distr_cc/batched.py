@@ -6,7 +6,7 @@ contractions write disjoint outputs (each system's own R), so they run as ONE pe
 GEMM launch (dcc_contract_grouped_f64) per term; the bandwidth passes (Y gather, W_oooo/F_oo
 permuted adds, fused finalize) stay per system.  ``capture()`` records a whole step -- R reset, Y
 gather, all grouped GEMMs, all finalizes -- into one HIP graph, so a step is one graph launch.
-``autotune()`` times the four tile shapes of the grouped kernel for every term and keeps the fastest.
+``autotune()`` times the eight tile variants of the grouped kernel for every term and keeps the fastest.
 """
 import ctypes
 
@@ -16,7 +16,8 @@ from distr_cc import _lib as L
 from distr_cc.device import DeviceArray, Event
 from distr_cc.t3_engine import GemmLaunch, TriplesEngine
 
-TILE_SHAPES = {0: "64x64", 1: "128x64", 2: "64x128", 3: "128x128"}
+TILE_SHAPES = {0: "64x64", 1: "128x64", 2: "64x128", 3: "128x128",
+               4: "64x64s1", 5: "128x64s1", 6: "64x128s1", 7: "128x128s1"}
 
 
 class _GroupedPhase:
@@ -25,8 +26,10 @@ class _GroupedPhase:
         self.name = launches[0].name
         self.flops = sum(g.flops for g in launches)
         self.tile = 0
+        self.variant = 0
         self.dev = None
         self.total = 0
+        self.nworkgroups = 0   # persistent grid size; 0 = as many workgroups as are resident at once
 
     def problems(self):
         arr = (L.GemmProblem * len(self.launches))()
@@ -40,16 +43,18 @@ class _GroupedPhase:
 
     def prepare(self, tile, stream):
         arr = self.problems()
-        total = ctypes.c_int64()
-        L.call("dcc_grouped_prepare", ctypes.cast(arr, ctypes.c_void_p), len(arr), int(tile), ctypes.byref(total))
-        self.tile, self.total = int(tile), int(total.value)
+        total, variant = ctypes.c_int64(), ctypes.c_int64()
+        L.call("dcc_grouped_prepare", ctypes.cast(arr, ctypes.c_void_p), len(arr), int(tile), ctypes.byref(total),
+               ctypes.byref(variant))
+        self.tile, self.total, self.variant = int(tile), int(total.value), int(variant.value)
         host = np.frombuffer(bytes(arr), dtype=np.uint8).copy()
         if self.dev is not None:
             self.dev.free()
         self.dev = DeviceArray.from_host(host, stream)
 
     def run(self, stream):
-        L.call("dcc_contract_grouped_f64", self.dev.vp, len(self.launches), self.total, self.tile, 0,
+        L.call("dcc_contract_grouped_f64", self.dev.vp, len(self.launches), self.total, self.variant,
+               self.nworkgroups,
                stream.handle if stream is not None else None)
 
 

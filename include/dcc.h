@@ -353,14 +353,17 @@ typedef struct dcc_gemm_problem {
 
 typedef void *dcc_graph_t;
 
-/* Host side: tile every problem for tile shape 0 (64x64), 1 (128x64), 2 (64x128) or 3 (128x128),
- * fill the tiling fields and return the total tile count. */
-int dcc_grouped_prepare(dcc_gemm_problem *probs, int64_t nprob, int64_t tile_shape, int64_t *total_tiles);
+/* Host side: tile every problem for tile shape 0..7 (as dcc_contract_tiled_f64: 0..3 = 64x64, 128x64,
+ * 64x128, 128x128 with two-stage register prefetch, 4..7 the same with one stage), fill the tiling
+ * fields and flags, and return the total tile count and the kernel variant to launch with. */
+int dcc_grouped_prepare(dcc_gemm_problem *probs, int64_t nprob, int64_t tile_shape, int64_t *total_tiles,
+                        int64_t *variant);
 
 /* Persistent grouped FP64 gather-GEMM over a device array of prepared problems whose C regions are
- * disjoint; nworkgroups = 0 picks 4 per CU.  Every workgroup exits when the tile list is exhausted. */
+ * disjoint; variant from dcc_grouped_prepare; nworkgroups = 0 launches as many workgroups as are
+ * resident at once.  Every workgroup exits when the tile list is exhausted. */
 int dcc_contract_grouped_f64(const dcc_gemm_problem *probs, int64_t nprob, int64_t total_tiles,
-                             int64_t tile_shape, int64_t nworkgroups, dcc_stream_t stream);
+                             int64_t variant, int64_t nworkgroups, dcc_stream_t stream);
 
 /* HIP graph capture of everything issued on a (non-default) stream, and its replay. */
 int dcc_graph_capture_begin(dcc_stream_t stream);

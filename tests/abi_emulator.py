@@ -110,9 +110,11 @@ class Emulator:
     def dcc_graph_destroy(self, graph):
         return 0
 
-    def dcc_grouped_prepare(self, probs, nprob, tile, ptotal):
+    def dcc_grouped_prepare(self, probs, nprob, tile, ptotal, pvariant):
         from distr_cc import _lib as L
-        bm, bn = {0: (64, 64), 1: (128, 64), 2: (64, 128), 3: (128, 128)}[int(tile)]
+        if not 0 <= int(tile) <= 7:
+            raise AssertionError("tile shape must be 0..7")
+        bm, bn = {0: (64, 64), 1: (128, 64), 2: (64, 128), 3: (128, 128)}[int(tile) % 4]
         arr = (L.GemmProblem * int(nprob)).from_address(_ptr(probs))
         acc = 0
         for p in arr:
@@ -121,9 +123,12 @@ class Emulator:
             p.tile_begin = acc
             acc += p.nbatch * p.tiles_m * p.tiles_n
         ptotal._obj.value = acc
+        pvariant._obj.value = int(tile)
         return 0
 
-    def dcc_contract_grouped_f64(self, probs, nprob, total, tile, nwg, stream):
+    def dcc_contract_grouped_f64(self, probs, nprob, total, variant, nwg, stream):
+        if not 0 <= int(variant) < 64:
+            raise AssertionError("variant must come from dcc_grouped_prepare")
         from distr_cc import _lib as L
         raw = self.dev.view(_ptr(probs), np.uint8, int(nprob) * ctypes.sizeof(L.GemmProblem))
         arr = (L.GemmProblem * int(nprob)).from_buffer_copy(raw.tobytes())

@@ -386,6 +386,19 @@ def test_batched_ensemble_vs_engines(dev):
         assert abs(e.get_t3() - t3ref).max() <= 1e-12 * abs(t3ref).max()
         assert abs(e.read_norms()[0] - nrm[0]) <= 1e-11 * nrm[0]
     bt.close()
+    # every grouped kernel variant (four tile shapes x one/two stages), on a persistent grid of 7
+    # workgroups so each launch walks many windows, the last one partial
+    for tile in range(8):
+        _say("grouped variant %d" % tile)
+        for ph in bt.phases:
+            if hasattr(ph, "variant"):
+                ph.prepare(tile, st)
+                ph.nworkgroups = 7
+        for n, (o, v) in enumerate(systems):
+            bt.engines[n].set_t3(SyntheticSystem(o, v, naux=40, seed=90 + n).t3_tri())
+        bt.issue_step()
+        for e, (nrm, t3ref) in zip(bt.engines, refs):
+            assert abs(e.get_t3() - t3ref).max() <= 1e-12 * abs(t3ref).max(), tile
 
 
 @pytest.mark.parametrize("o,v", [(3, 20), (5, 17)])
