@@ -18,8 +18,9 @@ j=k blocks), permuted to ijk and divided by the number of distinct orderings of 
     dt2[i,k,d,c]     =  6 sum_{j,a,b} Lam_w[i,j,k,a,b,c] W_vvvo[a,b,d,j]
                       - 6 sum_{i',j',a} Lam_w[i',j',i,a,d,c] W_vooo[a,k,i',j']   (relabelled)
 
-Lam_w is rebuilt on the device from lam by S3 gather jobs for every backward call (lam is what
-autograd hands over; the dense form is never kept between calls).  ``T2T3Residual`` wraps
+Lam_w is rebuilt on the device from lam by S3 gather jobs for every backward call, one chunk of
+first occupied indices i at a time, in the same scratch buffer the forward uses for its base term X
+(lam is what autograd hands over; no dense T3-sized tensor is stored).  ``T2T3Residual`` wraps
 forward and backward in a torch.autograd.Function (PyTorch only as the autograd wrapper; every
 contraction is a libdcc gather-GEMM).
 """
@@ -30,7 +31,7 @@ import numpy as np
 from distr_cc import _lib as L
 from distr_cc.device import DeviceArray, init as device_init
 from distr_cc.distribute_t3 import TP_T3, canonical_case, ijk_to_linear, nocc3
-from distr_cc.t3_engine import Operand, TriplesEngine, plan_einsum, s3_jobs_array
+from distr_cc.t3_engine import Operand, plan_einsum, plan_t2t3, s3_jobs_array
 
 
 def _stabiliser(i, j, k):
@@ -47,77 +48,104 @@ def _stabiliser(i, j, k):
 class T2T3Adjoint:
     """Device buffers and launch plans of the forward T2->T3 residual and its VJP."""
 
-    def __init__(self, nocc, nvir, stream=None, device=0):
+    def __init__(self, nocc, nvir, stream=None, device=0, scratch_bytes=2 << 30):
+        """Holds t2, W_vvvo, W_vooo, the canonical r3 blocks, the adjoint lam and the gradients, plus
+        ONE scratch buffer of at most ``scratch_bytes`` for a chunk of first occupied indices i: the
+        forward's base term X and the backward's Lam_w use it in turn, so no T3-sized dense tensor
+        is ever stored -- both are rebuilt chunk by chunk on every call."""
         device_init(device)
         o, v = int(nocc), int(nvir)
         self.nocc, self.nvir, self.stream = o, v, stream
         self.n3 = nocc3(o)
-        self.eng = TriplesEngine(o, v, stream=stream, device=device)
-        self.fwd = [g for g in self.eng.launches if g.name.startswith("T2T3:")]
-        assert self.fwd
         f64 = np.float64
+        self.R = DeviceArray((self.n3, v, v, v), f64)
+        self.t2 = DeviceArray((o, o, v, v), f64)
+        self.W_ovvv = DeviceArray((o, v, v, v), f64)     # W_vvvo[a,b,d,j] as [j,a,b,d]
+        self.W_oovo = DeviceArray((o, o, v, o), f64)     # W_vooo[a,l,i,j] as [i,j,a,l]
+        self.scratch, self.fwd = plan_t2t3(o, v, self.R, self.t2, self.W_ovvv, self.W_oovo, scratch_bytes)
+        assert self.fwd
         self.lam = DeviceArray((self.n3, v, v, v), f64)
-        self.lam_w = DeviceArray((o, o, o, v, v, v), f64)
         self.dt2 = DeviceArray((o, o, v, v), f64)
         self.dW_ovvv = DeviceArray((o, v, v, v), f64)    # dW_vvvo[a,b,d,j] as [j,a,b,d]
         self.dW_oovo = DeviceArray((o, o, v, o), f64)    # dW_vooo[a,l,i,j] as [i,j,a,l]
         v3 = v ** 3
-        jobs = []
-        for i in range(o):
-            for j in range(o):
-                for k in range(o):
-                    case, (ci, cj, ck) = canonical_case(i, j, k)
-                    # lam of a degenerate block only enters L through its part symmetric under the
-                    # block's stabiliser; symmetrising it makes Lam_w simultaneously symmetric:
-                    # Lam_w[ijk] = (1/6) sum_{g in stab} lam[c].transpose(g).transpose(TP_T3[case])
-                    coef = np.zeros(6)
-                    t = TP_T3[case]
-                    for g in _stabiliser(ci, cj, ck):
-                        coef[TP_T3.index((g[t[0]], g[t[1]], g[t[2]]))] += 1.0 / 6.0
-                    jobs.append((((i * o + j) * o + k) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
-        self.jobs = DeviceArray.from_host(s3_jobs_array(jobs))
-        self.njobs = len(jobs)
+        ic = self.scratch.shape[0]
+        # split-K partials of the two gradients whose M x N is one chunk of (i, occupied, virtual) rows
+        # against one index, with K = o v^2: the peeled occupied index becomes a batch entry writing
+        # its own slice, and the slices are summed in a fixed order (dcc_sum_slices)
+        self.P_vooo = DeviceArray((o, ic, o, v, o), f64)
+        self.P_t2 = DeviceArray((o, ic, o, v, v), f64)
         ext = {x: v for x in "abcdef"}
         ext.update({x: o for x in "ijklmn"})
-        Lw = lambda letters: Operand(self.lam_w, letters, (o, o, o, v, v, v))
-        t2 = lambda letters: Operand(self.eng.t2, letters, (o, o, v, v))
         one = [({}, {}, {})]
-        self.bwd = [
-            plan_einsum("vjp dW_vvvo", "ijkabc,ikdc->jabd", Lw("ijkabc"), t2("ikdc"),
-                        Operand(self.dW_ovvv, "jabd", (o, v, v, v)), 6.0, 0.0, ext, one),
-            plan_einsum("vjp dW_vooo", "ijkabc,lkbc->ijal", Lw("ijkabc"), t2("lkbc"),
-                        Operand(self.dW_oovo, "ijal", (o, o, v, o)), -6.0, 0.0, ext, one),
-            plan_einsum("vjp dt2 vvvo", "ijkabc,jabd->ikdc", Lw("ijkabc"),
-                        Operand(self.eng.W_ovvv, "jabd", (o, v, v, v)),
-                        Operand(self.dt2, "ikdc", (o, o, v, v)), 6.0, 0.0, ext, one),
-            plan_einsum("vjp dt2 vooo", "ijkabc,ijal->lkbc", Lw("ijkabc"),
-                        Operand(self.eng.W_oovo, "ijal", (o, o, v, o)),
-                        Operand(self.dt2, "lkbc", (o, o, v, v)), -6.0, 1.0, ext, one),
-        ]
+        self.chunks = []
+        for i0 in range(0, o, ic):
+            n = min(ic, o - i0)
+            jobs = []
+            for i in range(i0, i0 + n):
+                for j in range(o):
+                    for k in range(o):
+                        case, (ci, cj, ck) = canonical_case(i, j, k)
+                        # lam of a degenerate block only enters L through its part symmetric under
+                        # the block's stabiliser; symmetrising it makes Lam_w simultaneously symmetric:
+                        # Lam_w[ijk] = (1/6) sum_{g in stab} lam[c].transpose(g).transpose(TP_T3[case])
+                        coef = np.zeros(6)
+                        t = TP_T3[case]
+                        for g in _stabiliser(ci, cj, ck):
+                            coef[TP_T3.index((g[t[0]], g[t[1]], g[t[2]]))] += 1.0 / 6.0
+                        jobs.append(((((i - i0) * o + j) * o + k) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
+            ext["i"] = n
+            Lw = Operand(self.scratch, "ijkabc", (n, o, o, v, v, v))
+            t2_i = Operand(self.t2, "ikdc", (n, o, v, v), base=i0 * o * v * v)
+            # every gradient is zeroed first and accumulated over chunks; split-K partials are
+            # written with beta 0 and summed into their gradient rows
+            split_k = [({"k": k}, {"k": k}, {"S": k}) for k in range(o)]
+            split_j = [({"j": j}, {"j": j}, {"S": j}) for j in range(o)]
+            launches = [
+                plan_einsum("vjp dW_vvvo", "ijkabc,ikdc->jabd", Lw, t2_i,
+                            Operand(self.dW_ovvv, "jabd", (o, v, v, v)), 6.0, 1.0, ext, one),
+                plan_einsum("vjp dW_vooo", "ijabc,lbc->ijal", Lw, Operand(self.t2, "lkbc", (o, o, v, v)),
+                            Operand(self.P_vooo, "Sijal", (o, n, o, v, o)), -6.0, 0.0, ext, split_k),
+                ("sum", self.dW_oovo, i0 * o * v * o, self.P_vooo, o, n * o * v * o),
+                plan_einsum("vjp dt2 vvvo", "ikabc,abd->ikdc", Lw, Operand(self.W_ovvv, "jabd", (o, v, v, v)),
+                            Operand(self.P_t2, "Sikdc", (o, n, o, v, v)), 6.0, 0.0, ext, split_j),
+                ("sum", self.dt2, i0 * o * v * v, self.P_t2, o, n * o * v * v),
+                plan_einsum("vjp dt2 vooo", "ijkabc,ijal->lkbc", Lw,
+                            Operand(self.W_oovo, "ijal", (n, o, v, o), base=i0 * o * v * o),
+                            Operand(self.dt2, "lkbc", (o, o, v, v)), -6.0, 1.0, ext, one),
+            ]
+            self.chunks.append((DeviceArray.from_host(s3_jobs_array(jobs)), len(jobs), launches))
 
     def _h(self):
         return self.stream.handle if self.stream is not None else None
 
     def set_inputs(self, t2, W_vvvo, W_vooo):
         """Host arrays: t2 (o,o,v,v), W_vvvo (v,v,v,o), W_vooo (v,o,o,o)."""
-        self.eng.t2.upload(np.ascontiguousarray(t2, dtype=np.float64), self.stream)
-        self.eng.W_ovvv.upload(np.ascontiguousarray(np.asarray(W_vvvo).transpose(3, 0, 1, 2)), self.stream)
-        self.eng.W_oovo.upload(np.ascontiguousarray(np.asarray(W_vooo).transpose(2, 3, 0, 1)), self.stream)
+        self.t2.upload(np.ascontiguousarray(t2, dtype=np.float64), self.stream)
+        self.W_ovvv.upload(np.ascontiguousarray(np.asarray(W_vvvo).transpose(3, 0, 1, 2)), self.stream)
+        self.W_oovo.upload(np.ascontiguousarray(np.asarray(W_vooo).transpose(2, 3, 0, 1)), self.stream)
 
     def forward(self):
-        """R <- T2->T3 part of r3 (canonical blocks) for the current inputs (the engine's T2->T3
-        launches: the base-term GEMMs and the permutation passes)."""
-        self.eng.R.fill(0.0, self.stream)
+        """R <- T2->T3 part of r3 (canonical blocks) for the current inputs (the base-term GEMMs and
+        the permutation passes of the triples engine, chunk by chunk)."""
+        self.R.fill(0.0, self.stream)
         for g in self.fwd:
             g.run(self.stream)
-        return self.eng.R
+        return self.R
 
     def backward(self):
         """dt2, dW_ovvv ([j,a,b,d] layout), dW_oovo ([i,j,a,l] layout) for the adjoint in ``lam``."""
-        L.call("dcc_s3_transform_f64", self.lam_w.vp, self.lam.vp, self.nvir, 0.0, self.jobs.vp, self.njobs,
-               self._h())
-        for g in self.bwd:
-            g.run(self.stream)
+        for grad in (self.dt2, self.dW_ovvv, self.dW_oovo):
+            grad.fill(0.0, self.stream)
+        for jobs, njobs, launches in self.chunks:
+            L.call("dcc_s3_transform_f64", self.scratch.vp, self.lam.vp, self.nvir, 0.0, jobs.vp, njobs,
+                   self._h())
+            for g in launches:
+                if isinstance(g, tuple):     # ('sum', gradient, row offset, partials, slices, count)
+                    _, dst, off, part, ns, cnt = g
+                    L.call("dcc_sum_slices_f64", dst.offset_ptr(off), part.vp, ns, cnt, 1.0, 1.0, self._h())
+                else:
+                    g.run(self.stream)
         return self.dt2, self.dW_ovvv, self.dW_oovo
 
     def vjp_host(self, lam):

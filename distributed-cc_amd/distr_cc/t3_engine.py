@@ -7,13 +7,15 @@ Data in HBM (FP64, C order):
   Z              (npairs, o, v, v, v)  Y[l][b,d,c] + Y[l][c,b,d] stored as [d,b,c], right after Y
   WR1, WR3[2]    (o, v, o, v)      ring weights, rows (r,x), cols (l,d)
   Gs             (pairs per chunk, o, v, v, v)  G[r;pq][a,b,c] before its placement
+  Xt             (i per chunk, o, o, v, v, v)   T2->T3 base term X over ordered (i,j,k)
   pfs            (6, n3, n3)       W_oooo/F_oo prefactors
+  Xp             (5, targets per chunk, v, v, v)  W_oooo/F_oo terms s = 1..5 before their permutation
   intermediates  shapes of distr_cc/rccsdt.py:380,445,625
 
 The residual follows distr_cc/rccsdt.py:368-433 term by term, grouped into
 batched gather-GEMMs (dcc_contract_f64):
   * T2->T3 (rccsdt.py:388-399): the base term X[ijk] for every ordered (i,j,k) (2 GEMMs), then one
-    S3 pass per simultaneous (ijk, abc) permutation into the canonical blocks;
+    S3 gather pass: each canonical block adds its six simultaneously (ijk, abc)-permuted sources;
   * F_vv and W_vvvv x T3 (rccsdt.py:400-402,424-426): 6 launches, N = (block, ...);
   * W_ovov/W_ovvo x T3 (rccsdt.py:540-596): with G the part-3 operator
         G[r;pq][a,b,c] = sum_{l,d} ( -W_ovov[l,b,r,d] Y[l][a,d,c] - W_ovov[l,c,r,d] Y[l][a,b,d]
@@ -21,10 +23,12 @@ batched gather-GEMMs (dcc_contract_f64):
                         - 1/2 (W_ovov[l,a,r,d] + W_ovvo[l,a,d,r]) (Y[l][b,d,c] + Y[l][c,b,d]) )
     part 3 of target (i,j,k) is G[i;jk], part 4 is G[j;ik] with (a,b) swapped and
     part 5 is G[k;ij] read as [c,a,b]; G is computed once for all r (GEMMs batched over
-    the pairs p<=q, rows (r, x)) into a scratch buffer and placed by one S3 pass per
-    part; with Z the last three terms are one GEMM over K = (term, l, d), B = [Y; Z];
+    the pairs p<=q, rows (r, x)) into a scratch buffer and placed by one S3 gather pass
+    (all three parts of a target at once); with Z the W_ovvo term and the two
+    (W_ovov + W_ovvo) terms are one GEMM over K = (term, l, d), B = [Y; Z];
   * W_oooo/F_oo x T3 (rccsdt.py:616-732): 6 GEMMs M = n3 targets, K = n3 sources,
-    N = v^3, with the prefactors built on the device.
+    N = v^3, with the prefactors built on the device; term 0 goes straight into R, terms
+    1..5 into Xp per chunk of targets and are added, permuted, by one S3 gather pass.
 Then dcc_r3_finalize_f64 fuses rccsdt.py:947-971.
 
 T3 -> r1/r2 (rccsdt.py:253-321) and T3 -> W_vooo/W_vvvo (rccsdt.py:323-366)
@@ -258,6 +262,54 @@ class S3GatherLaunch:
                stream.handle if stream is not None else None)
 
 
+def plan_t2t3(o, v, R, t2, W_ovvv, W_oovo, scratch_bytes):
+    """T2 -> T3 terms (rccsdt.py:388-399) added into the canonical blocks R; returns (Xt, launches).
+
+    The twelve terms are the six simultaneous (ijk, abc) permutations of one base term
+        X[i,j,k][a,b,c] = sum_d W_vvvo[a,b,d,j] t2[i,k,d,c] - sum_l W_vooo[a,l,i,j] t2[l,k,b,c],
+    r3[ijk][abc] += sum_pi X[pi(ijk)][pi(abc)].  X is built for every ordered (i,j,k) by two GEMMs
+    (M = (j,a,b), N = (i,k,c), K = d and M = (i,j,a), N = (k,b,c), K = l; no batch tables), and
+    one multi-source S3 gather pass adds, for every canonical target, its (up to six) permuted
+    source blocks in one read-modify-write.  Ordered triples are processed in chunks of the first
+    index whose X fits ``scratch_bytes``; Xt is that chunk buffer (ic, o, o, v, v, v).
+    W_ovvv holds W_vvvo[a,b,d,j] as [j,a,b,d] and W_oovo holds W_vooo[a,l,i,j] as [i,j,a,l]."""
+    v3 = v ** 3
+    dt = R.dtype
+    ic = int(max(1, min(o, scratch_bytes // max(1, o * o * v3 * dt.itemsize))))
+    Xt = DeviceArray((ic, o, o, v, v, v), dt)
+    ext = {x: v for x in 'abcd'}
+    ext.update({x: o for x in 'ijkl'})
+    one = [({}, {}, {})]
+    # source-position permutations of (i,j,k); the abc permutation of the S3 job is its inverse
+    pis = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    triples = canonical_triples(o)
+    launches = []
+    for i0 in range(0, o, ic):
+        n = min(ic, o - i0)
+        ext['i'] = n
+        X = Operand(Xt, 'ijkabc', (n, o, o, v, v, v))
+        A1 = Operand(W_ovvv, 'jabd', (o, v, v, v))
+        B1 = Operand(t2, 'ikdc', (n, o, v, v), base=i0 * o * v * v)
+        launches.append(plan_einsum('T2T3: X W_vvvo', 'jabd,ikdc->ijkabc', A1, B1, X, 1.0, 0.0, ext, one))
+        A2 = Operand(W_oovo, 'ijal', (n, o, v, o), base=i0 * o * v * o)
+        B2 = Operand(t2, 'lkbc', (o, o, v, v))
+        launches.append(plan_einsum('T2T3: X W_vooo', 'ijal,lkbc->ijkabc', A2, B2, X, -1.0, 1.0, ext, one))
+        # one gather pass: target q takes sum_pi X[pi(ijk)]^{pi^-1} over the sources in this chunk
+        jobs = []
+        for q, t in enumerate(triples):
+            srcs, coef = [0] * 6, np.zeros(6)
+            for pos in pis:
+                src = (t[pos[0]], t[pos[1]], t[pos[2]])
+                if i0 <= src[0] < i0 + n:
+                    p = TP_T3.index(tuple(pos.index(u) for u in range(3)))
+                    srcs[p] = (((src[0] - i0) * o + src[1]) * o + src[2]) * v3
+                    coef[p] = 1.0
+            if coef.any():
+                jobs.append((q * v3, srcs, coef))
+        launches.append(S3GatherLaunch('T2T3: symmetrise', R, Xt, v, 1.0, jobs))
+    return Xt, launches
+
+
 def s3_jobs_array(jobs):
     arr = np.zeros(len(jobs), dtype=L.S3JOB_DTYPE)
     for n, (dst, src, coef) in enumerate(jobs):
@@ -462,49 +514,9 @@ class TriplesEngine:
         return launches
 
     def _plan_t2t3(self):
-        """T2 -> T3 terms (rccsdt.py:388-399).
-
-        The twelve terms are the six simultaneous (ijk, abc) permutations of one base term
-            X[i,j,k][a,b,c] = sum_d W_vvvo[a,b,d,j] t2[i,k,d,c] - sum_l W_vooo[a,l,i,j] t2[l,k,b,c],
-        r3[ijk][abc] += sum_pi X[pi(ijk)][pi(abc)].  X is built for every ordered (i,j,k) by two GEMMs
-        (M = (j,a,b), N = (i,k,c), K = d and M = (i,j,a), N = (k,b,c), K = l; no batch tables), and
-        each permutation pi is one S3 pass adding the permuted source blocks into the canonical
-        targets.  Ordered triples are processed in chunks of the first index that fit
-        ``ring_g_bytes``."""
-        o, v = self.nocc, self.nvir
-        v3 = v ** 3
-        dt = self.dtype
-        ic = int(max(1, min(o, self.ring_g_bytes // max(1, o * o * v3 * dt.itemsize))))
-        self.Xt = DeviceArray((ic, o, o, v, v, v), dt)
-        ext = {x: v for x in 'abcd'}
-        ext.update({x: o for x in 'ijkl'})
-        one = [({}, {}, {})]
-        # source-position permutations of (i,j,k); the abc permutation of the S3 job is its inverse
-        pis = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
-        launches = []
-        for i0 in range(0, o, ic):
-            n = min(ic, o - i0)
-            ext['i'] = n
-            X = Operand(self.Xt, 'ijkabc', (n, o, o, v, v, v))
-            A1 = Operand(self.W_ovvv, 'jabd', (o, v, v, v))
-            B1 = Operand(self.t2, 'ikdc', (n, o, v, v), base=i0 * o * v * v)
-            launches.append(plan_einsum('T2T3: X W_vvvo', 'jabd,ikdc->ijkabc', A1, B1, X, 1.0, 0.0, ext, one))
-            A2 = Operand(self.W_oovo, 'ijal', (n, o, v, o), base=i0 * o * v * o)
-            B2 = Operand(self.t2, 'lkbc', (o, o, v, v))
-            launches.append(plan_einsum('T2T3: X W_vooo', 'ijal,lkbc->ijkabc', A2, B2, X, -1.0, 1.0, ext, one))
-            # one gather pass: target q takes sum_pi X[pi(ijk)]^{pi^-1} over the sources in this chunk
-            jobs = []
-            for q, t in enumerate(self.triples):
-                srcs, coef = [0] * 6, np.zeros(6)
-                for pos in pis:
-                    src = (t[pos[0]], t[pos[1]], t[pos[2]])
-                    if i0 <= src[0] < i0 + n:
-                        p = TP_T3.index(tuple(pos.index(u) for u in range(3)))
-                        srcs[p] = (((src[0] - i0) * o + src[1]) * o + src[2]) * v3
-                        coef[p] = 1.0
-                if coef.any():
-                    jobs.append((q * v3, srcs, coef))
-            launches.append(S3GatherLaunch('T2T3: symmetrise', self.R, self.Xt, v, 1.0, jobs))
+        """T2 -> T3 terms (plan_t2t3), with the base-term scratch bounded by ``ring_g_bytes``."""
+        self.Xt, launches = plan_t2t3(self.nocc, self.nvir, self.R, self.t2, self.W_ovvv, self.W_oovo,
+                                      self.ring_g_bytes)
         return launches
 
     def _plan_ring(self):

@@ -68,11 +68,20 @@ def _check(adj, o, v, seed):
         assert abs(fd - np.sum(grad * d)) < 1e-8 * max(1.0, abs(fd))
 
 
-@pytest.mark.parametrize("o,v", [(1, 4), (3, 5), (4, 3)])
-def test_adjoint_plan(monkeypatch, o, v):
-    abi_emulator.install(monkeypatch)
+def _adjoint(o, v, ic):
+    """T2T3Adjoint whose scratch holds ic first occupied indices (None: the default bound)."""
     from distr_cc.adjoint import T2T3Adjoint
-    _check(T2T3Adjoint(o, v), o, v, 10 + o)
+    if ic is None:
+        return T2T3Adjoint(o, v)
+    adj = T2T3Adjoint(o, v, scratch_bytes=ic * o * o * v ** 3 * 8)
+    assert adj.scratch.shape[0] == ic and len(adj.chunks) == -(-o // ic)
+    return adj
+
+
+@pytest.mark.parametrize("o,v,ic", [(1, 4, None), (3, 5, None), (4, 3, None), (4, 3, 1), (5, 3, 2)])
+def test_adjoint_plan(monkeypatch, o, v, ic):
+    abi_emulator.install(monkeypatch)
+    _check(_adjoint(o, v, ic), o, v, 10 + o)
 
 
 def test_autograd_function(monkeypatch):
@@ -94,10 +103,9 @@ def test_autograd_function(monkeypatch):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("o,v", [(1, 4), (3, 7), (6, 13)])
-def test_adjoint_gpu(o, v):
+@pytest.mark.parametrize("o,v,ic", [(1, 4, None), (3, 7, None), (6, 13, None), (6, 13, 4)])
+def test_adjoint_gpu(o, v, ic):
     from distr_cc import device
-    from distr_cc.adjoint import T2T3Adjoint
     device.init(0)
-    print("adjoint o=%d v=%d" % (o, v), flush=True)
-    _check(T2T3Adjoint(o, v), o, v, 20 + o)
+    print("adjoint o=%d v=%d chunk=%s" % (o, v, ic), flush=True)
+    _check(_adjoint(o, v, ic), o, v, 20 + o)
