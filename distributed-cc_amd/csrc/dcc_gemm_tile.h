@@ -105,11 +105,21 @@ template <int a, int b> struct CMax { static constexpr int v = a > b ? a : b; };
 // K-contiguous (KFAST): pair (2 kp, 2 kp + 1) with kp = tid % 8, rows tid / 8 + 32 r (NR = R/32, NK = 2);
 // row-contiguous: rows (2 i, 2 i + 1) with i = tid % (R/2), k = tid / (R/2) + (512 / R) j (NR = 2,
 // NK = R/32).  reg[r][j] holds row row(tid, r) at k kat(tid, r, j).
+// NIT: K-offset iterators per thread (one per distinct k of the thread in a tile; the second k of a
+// K-contiguous pair is the first plus one whenever pairs are loaded as vectors).  With R = 128 rows a
+// row-contiguous operand's k is the wave index plus a constant, the same in every lane, so its offsets
+// are computed once per wave in scalar registers.
 template <int R, bool KFAST> struct StageP {
     static constexpr int NR = KFAST ? R / 32 : 2;
     static constexpr int NK = KFAST ? 2 : R / 32;
+    static constexpr int NIT = KFAST ? 1 : NK;
     __device__ static int row(int tid, int r) { return KFAST ? tid / 8 + 32 * r : 2 * (tid % (R / 2)) + r; }
-    __device__ static int kidx(int tid, int j) { return KFAST ? 2 * (tid % 8) + j : tid / (R / 2) + (512 / R) * j; }
+    __device__ static int kbase(int tid) {
+        if constexpr (KFAST) return 2 * (tid % 8);
+        else if constexpr (R / 2 == 64) return __builtin_amdgcn_readfirstlane(tid / 64);
+        else return tid / (R / 2);
+    }
+    __device__ static int kidx(int tid, int j) { return KFAST ? kbase(tid) + j : kbase(tid) + (512 / R) * j; }
     __device__ static int kat(int tid, int r, int j) { return kidx(tid, j); }
     __device__ static int rat(int tid, int r, int j) { return row(tid, r); }
 };
@@ -118,13 +128,17 @@ template <int R, bool KFAST> struct StageP {
 // four (the LDS of four workgroups fits a CU); the two-stage one three, since its second register set
 // does not fit 128 VGPRs without spilling; larger tiles are limited by LDS anyway.
 template <int BM, int BN, int STAGES> struct MinWaves {
-    static constexpr int v = (BM * BN <= 64 * 64) ? (STAGES == 1 ? 4 : 3) : 1;
+    static constexpr int v = (BM * BN <= 64 * 64) ? (STAGES == 1 ? 4 : 3) : (BM * BN <= 128 * 64) ? 3 : 2;
 };
 
 // LDS elements of one tile: two K buffers of A and B, or the C staging tile of the C_MFAST epilogue.
+// The C_MFAST epilogue stages the tile in EPI_PASSES passes of BN / EPI_PASSES columns (one wave column
+// per pass when the whole tile would not fit the main-loop buffers), so it never needs more LDS than
+// the main loop and does not lower the number of resident workgroups.
 template <int BM, int BN, bool C_MFAST> struct TileSmem {
     static constexpr int MAIN = 2 * kBK * (BM + 1) + 2 * kBK * (BN + 1);
-    static constexpr int v = C_MFAST ? CMax<MAIN, BN * (BM + 1)>::v : MAIN;
+    static constexpr int EPI_PASSES = BN * (BM + 1) > MAIN ? 2 : 1;
+    static constexpr int v = C_MFAST ? CMax<MAIN, BN / EPI_PASSES * (BM + 1)>::v : MAIN;
 };
 
 // Output tile (tm, tn) of batch `batch`.  C_MFAST selects the epilogue: false writes C straight from
@@ -176,14 +190,21 @@ __device__ __forceinline__ void contract_tile(const ContractArgs<T> &p, int64_t 
     // outputs that are never stored), and k beyond K reads offset 0 and is replaced by zero.
     typedef T T2 __attribute__((ext_vector_type(2)));
     T a_reg[STAGES][SA::NR][SA::NK], b_reg[STAGES][SB::NR][SB::NK];
-    // one operand: X with row offsets rows[], K map km, into reg[NR][NK]
-    auto load_op = [&](const T *X, const int64_t *rows, const KMap &km, int k0, bool vfull, auto &reg,
-                       auto kfast_tag, auto geom) {
+    // K offsets of this thread's k values, advanced by one K tile after every load (tiles are loaded
+    // in increasing k0), so the main loop needs no index decomposition
+    dcc::KIter a_it[SA::NIT], b_it[SB::NIT];
+#pragma unroll
+    for (int j = 0; j < SA::NIT; ++j) a_it[j] = dcc::kiter(p.ak, (uint32_t)SA::kidx(tid, j));
+#pragma unroll
+    for (int j = 0; j < SB::NIT; ++j) b_it[j] = dcc::kiter(p.bk, (uint32_t)SB::kidx(tid, j));
+    // one operand: X with row offsets rows[], K map km and iterators it[], into reg[NR][NK]
+    auto load_op = [&](const T *X, const int64_t *rows, const KMap &km, dcc::KIter *it, int k0, bool vfull,
+                       auto &reg, auto kfast_tag, auto geom) {
         typedef decltype(geom) G;
         if constexpr (decltype(kfast_tag)::value) {
             const uint32_t k = (uint32_t)(k0 + G::kidx(tid, 0));
             if (vfull) {
-                const int64_t o = koff(km, k);
+                const int64_t o = it[0].off;
 #pragma unroll
                 for (int r = 0; r < G::NR; ++r) {
                     const T2 x = *reinterpret_cast<const T2 *>(X + rows[r] + o);
@@ -191,11 +212,12 @@ __device__ __forceinline__ void contract_tile(const ContractArgs<T> &p, int64_t 
                     reg[r][1] = x[1];
                 }
             } else {
+                // element by element: k from the iterator, k + 1 decomposed afresh
 #pragma unroll
                 for (int jj = 0; jj < 2; ++jj) {
                     const uint32_t kj = k + jj;
                     const bool ok = kj < (uint32_t)p.K;
-                    const int64_t o = koff(km, ok ? kj : 0u);
+                    const int64_t o = !ok ? 0 : jj == 0 ? it[0].off : koff(km, kj);
 #pragma unroll
                     for (int r = 0; r < G::NR; ++r) {
                         const T x = X[rows[r] + o];
@@ -203,12 +225,14 @@ __device__ __forceinline__ void contract_tile(const ContractArgs<T> &p, int64_t 
                     }
                 }
             }
+            dcc::kstep(km, it[0]);
         } else {
 #pragma unroll
             for (int j = 0; j < G::NK; ++j) {
                 const uint32_t k = (uint32_t)(k0 + G::kidx(tid, j));
                 const bool ok = k < (uint32_t)p.K;
-                const int64_t o = koff(km, ok ? k : 0u);
+                const int64_t o = ok ? it[j].off : 0;
+                dcc::kstep(km, it[j]);
                 if (vfull) {
                     const T2 x = *reinterpret_cast<const T2 *>(X + rows[0] + o);
                     reg[0][j] = x[0];
@@ -225,8 +249,8 @@ __device__ __forceinline__ void contract_tile(const ContractArgs<T> &p, int64_t 
     };
     auto load_tiles = [&](int s, int k0) {
         const bool vfull = vec && k0 + kBK <= p.K;   // the last, partial K tile loads per element
-        load_op(p.A, a_row, p.ak, k0, vfull, a_reg[s], std::integral_constant<bool, A_KFAST>{}, SA{});
-        load_op(p.B, b_col, p.bk, k0, vfull, b_reg[s], std::integral_constant<bool, !B_NFAST>{}, SB{});
+        load_op(p.A, a_row, p.ak, a_it, k0, vfull, a_reg[s], std::integral_constant<bool, A_KFAST>{}, SA{});
+        load_op(p.B, b_col, p.bk, b_it, k0, vfull, b_reg[s], std::integral_constant<bool, !B_NFAST>{}, SB{});
     };
     auto store_tiles = [&](int s, int buf) {
 #pragma unroll
@@ -272,7 +296,9 @@ __device__ __forceinline__ void contract_tile(const ContractArgs<T> &p, int64_t 
     if constexpr (STAGES == 2) {
         // Step kt multiplies LDS buffer kt&1 while register set kt&1 receives tile kt+2 and set
         // (kt+1)&1 (tile kt+1) is written to the other buffer; one barrier per step.  The loop is
-        // unrolled by two so register sets are indexed statically.
+        // unrolled by two so register sets are indexed statically, and always runs whole pairs (an
+        // odd last step multiplies a zero tile): a second exit made the register allocator copy every
+        // accumulator once per iteration.
         load_tiles(0, 0);
         load_tiles(1, kBK);
         store_tiles(0, 0);
@@ -282,7 +308,6 @@ __device__ __forceinline__ void contract_tile(const ContractArgs<T> &p, int64_t 
             compute(0);
             store_tiles(1, 1);
             __syncthreads();
-            if (kt + 1 >= nk) break;
             load_tiles(1, (kt + 3) * kBK);
             compute(1);
             store_tiles(0, 0);
@@ -301,28 +326,40 @@ __device__ __forceinline__ void contract_tile(const ContractArgs<T> &p, int64_t 
     }
 
     if constexpr (C_MFAST) {
-        // ---- LDS-staged epilogue: tile stored n-major, then written with lanes along m ----
-        T *Cs = smem;  // the main loop ended with a barrier, so the staging buffers are free
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    Cs[(wn * WN + j * 16 + lrow) * (BM + 1) + wm * WM + i * 16 + MF::row(lk, r)] = acc[i][j][r];
-        __syncthreads();
+        // ---- LDS-staged epilogue: tile stored n-major, then written with lanes along m; in two
+        // passes of one wave column each when the whole tile exceeds the main-loop LDS ----
+        constexpr int NP = TileSmem<BM, BN, true>::EPI_PASSES;
+        constexpr int NC = BN / NP;                 // columns per pass (NP = 2: NC = WN)
         constexpr int NSTEP = kThreads / BM;
+        T *Cs = smem;  // the main loop ended with a barrier, so the staging buffers are free
         const int ml = tid % BM;
         const int64_t m = m0 + ml;
-        if (m >= m_end) return;
-        const int64_t crow = bt.c_off + map_off(p.cm, (uint32_t)m, batch);
-        for (int nl = tid / BM; nl < BN; nl += NSTEP) {
-            const int n = n0 + nl;
-            if (n >= p.N) break;
-            T *c = p.C + crow + map_off(p.cn, (uint32_t)n, batch);
-            double v = p.alpha * (double)Cs[nl * (BM + 1) + ml];
-            if (p.beta != 0.0) v += p.beta * (double)*c;
-            *c = (T)v;
+        const bool mok = m < m_end;
+        const int64_t crow = mok ? bt.c_off + map_off(p.cm, (uint32_t)m, batch) : 0;
+#pragma unroll
+        for (int h = 0; h < NP; ++h) {
+            if (h > 0) __syncthreads();  // the previous pass's columns have been written out
+            if (NP == 1 || wn == h) {
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r)
+                            Cs[(wn * WN + j * 16 + lrow - h * NC) * (BM + 1) + wm * WM + i * 16 + MF::row(lk, r)] =
+                                acc[i][j][r];
+            }
+            __syncthreads();
+            if (mok) {
+                for (int nl = tid / BM; nl < NC; nl += NSTEP) {
+                    const int n = n0 + h * NC + nl;
+                    if (n >= p.N) break;
+                    T *c = p.C + crow + map_off(p.cn, (uint32_t)n, batch);
+                    double v = p.alpha * (double)Cs[nl * (BM + 1) + ml];
+                    if (p.beta != 0.0) v += p.beta * (double)*c;
+                    *c = (T)v;
+                }
+            }
         }
         return;
     }
