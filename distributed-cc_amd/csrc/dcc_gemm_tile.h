@@ -11,9 +11,9 @@
 //
 // Tile: BM x BN per workgroup of 256 threads (4 wave64), each wave owning a (BM/2) x (BN/2) sub-tile
 // as (BM/32) x (BN/32) MFMA 16x16 blocks; BK = 16 with a double-buffered LDS pipeline (one barrier
-// per K step).  Every thread loads its A and B elements at fixed k rows of the tile, so the K index
-// map is evaluated once per operand and tile (by reciprocal multiplication), and the loop body is
-// free of branches.
+// per K step).  Every thread loads its A and B elements at fixed k rows of the tile, so the K offsets
+// are decomposed once (by reciprocal multiplication) and then advanced from tile to tile without
+// division; the only branches in the loop are uniform (vector or per-element loads).
 #pragma once
 
 #include "dcc_common.h"
