@@ -35,11 +35,11 @@ void contract_kernel(const ContractArgs<T> p) {
 //   accumulator: lane holds D[4*(lane/16) + r][lane%16] (as v_mfma_f32_16x16x4f32).
 // Each operand is staged in LDS as hi and lo planes in one of two layouts, chosen by which of its
 // indices is contiguous in memory:
-//   K-contiguous ([row][k], 16-byte chunks XOR-swizzled per row pair): every thread loads a pair
-//     (k, k+1) of R/16 rows and writes each pair as one 32-bit word per plane; a fragment is one
-//     16-byte ds_read_b128.
-//   row-contiguous ([k][row], stride R + 16): every thread loads a pair of adjacent rows at R/16
-//     values of k and writes each pair as one word per plane; a fragment is two ds_read_b64_tr_b16,
+//   K-contiguous ([row][k], 16-byte chunks XOR-swizzled per row pair): every thread loads vectors of
+//     Q consecutive k (Q = 2, or 4 for packed operands whose layout allows 16-byte loads) and writes
+//     each as Q 16-bit values per plane; a fragment is one 16-byte ds_read_b128.
+//   row-contiguous ([k][row], stride R + 16): every thread loads vectors of Q adjacent rows at one k
+//     and writes each as Q 16-bit values per plane; a fragment is two ds_read_b64_tr_b16,
 //     which on gfx950 gives lane 16g + i the four values at column i of the 4 x 16 block whose row j
 //     the lanes 16g + 4j .. 16g + 4j + 3 address (four 16-bit values each).
 // The K offsets of the next tile's 32 k values are computed once per step by wave 0 (reciprocal
@@ -69,18 +69,23 @@ __device__ __forceinline__ void split_pair(uint32_t w0, uint32_t w1, uint32_t &h
     lo = __builtin_amdgcn_perm(w1, w0, 0x07060302u);
 }
 
-// Staging geometry of one operand with R rows (m for A, n for B) per tile.
-template <int R, bool KFAST> struct Stage3 {
+// Staging geometry of one operand with R rows (m for A, n for B) per tile, loaded in vectors of Q
+// elements (Q = 2: pairs, Q = 4: quads of packed words, one 16-byte load each): along k for a
+// K-contiguous operand (Q consecutive k of one row), along rows otherwise (Q adjacent rows at one k).
+template <int R, bool KFAST, int Q> struct Stage3 {
     // K-contiguous planes are [row][32 k] with the four 16-byte chunks of a row XOR-swizzled by
-    // (row >> 1) & 3: the 16-byte fragment reads of 8 consecutive rows and the 32-bit pair writes
-    // of a half-wave then touch distinct banks, without padding.
+    // (row >> 1) & 3: the 16-byte fragment reads of 8 consecutive rows and the vector writes of a
+    // half-wave then touch distinct banks, without padding.
     static constexpr int PLANE = KFAST ? R * kBK3 : kBK3 * (R + 16);
-    static constexpr int NR = KFAST ? R / 16 : 2;               // rows per thread
-    static constexpr int NK = KFAST ? 2 : R / 16;               // k values per thread
-    static constexpr int RP = R / 2;                            // row pairs (row-contiguous layout)
-    __device__ static int row(int tid, int r) { return KFAST ? tid / 16 + 16 * r : 2 * (tid % RP) + r; }
+    static constexpr int VPR = kBK3 / Q;                        // vectors per row (K-contiguous)
+    static constexpr int RQ = R / Q;                            // row groups (row-contiguous)
+    static constexpr int NR = KFAST ? R * VPR / kThreads : Q;   // rows per thread
+    static constexpr int NK = KFAST ? Q : R * kBK3 / (kThreads * Q);  // k values per thread
+    __device__ static int row(int tid, int r) {
+        return KFAST ? tid / VPR + (kThreads / VPR) * r : Q * (tid % RQ) + r;
+    }
     __device__ static int kidx(int tid, int j) {
-        return KFAST ? 2 * (tid % 16) + j : tid / RP + (kThreads / RP) * j;
+        return KFAST ? Q * (tid % VPR) + j : tid / RQ + (kThreads / RQ) * j;
     }
     // 16-bit element index of (row, k) in a plane
     __device__ static int at(int row, int k) {
@@ -105,12 +110,15 @@ template <int R, bool KFAST> struct Stage3 {
     }
 };
 
-// PACKED: A and B hold packed hi/lo words instead of FP32 values (the pointers of p are reinterpreted).
-template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, bool PACKED>
+// MODE 0: FP32 operands loaded in pairs; 1: packed hi/lo words instead of FP32 values (the pointers
+// of p are reinterpreted) loaded in pairs; 2: packed words loaded in quads (16-byte loads).
+template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int MODE>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
 void contract_bf16x3_kernel(const ContractArgs<float> p) {
-    typedef Stage3<BM, A_KFAST> SA;
-    typedef Stage3<BN, !B_NFAST> SB;
+    constexpr bool PACKED = MODE >= 1;
+    constexpr int Q = MODE == 2 ? 4 : 2;
+    typedef Stage3<BM, A_KFAST, Q> SA;
+    typedef Stage3<BN, !B_NFAST, Q> SB;
     constexpr int WM = BM / 2, WN = BN / 2;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int BUF = 2 * (SA::PLANE + SB::PLANE);            // one buffer: hi + lo planes, 16-bit units
@@ -168,41 +176,41 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
     const E *Ap = reinterpret_cast<const E *>(p.A);
     const E *Bp = reinterpret_cast<const E *>(p.B);
     E a_reg[SA::NR][SA::NK], b_reg[SB::NR][SB::NK];
-    typedef E E2 __attribute__((ext_vector_type(2)));
+    typedef E EQ __attribute__((ext_vector_type(Q)));
     auto load_tiles = [&](int t, auto tail, auto vec) {
         constexpr bool TAIL = decltype(tail)::value;
         if constexpr (decltype(vec)::value && !TAIL) {
-            // each thread's pairs are adjacent and 8-byte aligned: one 8-byte load per pair
+            // each thread's vectors are adjacent and aligned: one 4Q-byte load per vector
             if constexpr (A_KFAST) {
                 const int64_t o = ktab[t][0][SA::kidx(tid, 0)];
 #pragma unroll
                 for (int r = 0; r < SA::NR; ++r) {
-                    const E2 x = *reinterpret_cast<const E2 *>(Ap + a_row[r] + o);
-                    a_reg[r][0] = x[0];
-                    a_reg[r][1] = x[1];
+                    const EQ x = *reinterpret_cast<const EQ *>(Ap + a_row[r] + o);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) a_reg[r][q] = x[q];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < SA::NK; ++j) {
-                    const E2 x = *reinterpret_cast<const E2 *>(Ap + a_row[0] + ktab[t][0][SA::kidx(tid, j)]);
-                    a_reg[0][j] = x[0];
-                    a_reg[1][j] = x[1];
+                    const EQ x = *reinterpret_cast<const EQ *>(Ap + a_row[0] + ktab[t][0][SA::kidx(tid, j)]);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) a_reg[q][j] = x[q];
                 }
             }
             if constexpr (!B_NFAST) {
                 const int64_t o = ktab[t][1][SB::kidx(tid, 0)];
 #pragma unroll
                 for (int r = 0; r < SB::NR; ++r) {
-                    const E2 x = *reinterpret_cast<const E2 *>(Bp + b_col[r] + o);
-                    b_reg[r][0] = x[0];
-                    b_reg[r][1] = x[1];
+                    const EQ x = *reinterpret_cast<const EQ *>(Bp + b_col[r] + o);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) b_reg[r][q] = x[q];
                 }
             } else {
 #pragma unroll
                 for (int j = 0; j < SB::NK; ++j) {
-                    const E2 x = *reinterpret_cast<const E2 *>(Bp + b_col[0] + ktab[t][1][SB::kidx(tid, j)]);
-                    b_reg[0][j] = x[0];
-                    b_reg[1][j] = x[1];
+                    const EQ x = *reinterpret_cast<const EQ *>(Bp + b_col[0] + ktab[t][1][SB::kidx(tid, j)]);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) b_reg[q][j] = x[q];
                 }
             }
             return;
@@ -234,43 +242,53 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
             }
         }
     };
-    // pairs along k (K-contiguous layout) or along rows (row-contiguous layout), one word per plane
+    // vectors along k (K-contiguous layout) or along rows (row-contiguous layout): Q/2 split pairs,
+    // Q consecutive 16-bit values per plane, written as one 2Q-byte store
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+    auto put = [&](unsigned short *P, int e, const uint32_t *w) {
+        if constexpr (Q == 2) *reinterpret_cast<uint32_t *>(&P[e]) = w[0];
+        else *reinterpret_cast<u32x2 *>(&P[e]) = u32x2{w[0], w[1]};
+    };
     auto store_tiles = [&](int buf) {
         unsigned short *As = lds + buf * BUF;
         unsigned short *Bs = As + 2 * SA::PLANE;
-        uint32_t hi, lo;
+        uint32_t hi[Q / 2], lo[Q / 2];
         if constexpr (A_KFAST) {
 #pragma unroll
             for (int r = 0; r < SA::NR; ++r) {
-                split_pair(a_reg[r][0], a_reg[r][1], hi, lo);
+#pragma unroll
+                for (int h = 0; h < Q / 2; ++h) split_pair(a_reg[r][2 * h], a_reg[r][2 * h + 1], hi[h], lo[h]);
                 const int e = SA::at(SA::row(tid, r), SA::kidx(tid, 0));
-                *reinterpret_cast<uint32_t *>(&As[e]) = hi;
-                *reinterpret_cast<uint32_t *>(&As[SA::PLANE + e]) = lo;
+                put(As, e, hi);
+                put(As + SA::PLANE, e, lo);
             }
         } else {
 #pragma unroll
             for (int j = 0; j < SA::NK; ++j) {
-                split_pair(a_reg[0][j], a_reg[1][j], hi, lo);
+#pragma unroll
+                for (int h = 0; h < Q / 2; ++h) split_pair(a_reg[2 * h][j], a_reg[2 * h + 1][j], hi[h], lo[h]);
                 const int e = SA::at(SA::row(tid, 0), SA::kidx(tid, j));
-                *reinterpret_cast<uint32_t *>(&As[e]) = hi;
-                *reinterpret_cast<uint32_t *>(&As[SA::PLANE + e]) = lo;
+                put(As, e, hi);
+                put(As + SA::PLANE, e, lo);
             }
         }
         if constexpr (!B_NFAST) {
 #pragma unroll
             for (int r = 0; r < SB::NR; ++r) {
-                split_pair(b_reg[r][0], b_reg[r][1], hi, lo);
+#pragma unroll
+                for (int h = 0; h < Q / 2; ++h) split_pair(b_reg[r][2 * h], b_reg[r][2 * h + 1], hi[h], lo[h]);
                 const int e = SB::at(SB::row(tid, r), SB::kidx(tid, 0));
-                *reinterpret_cast<uint32_t *>(&Bs[e]) = hi;
-                *reinterpret_cast<uint32_t *>(&Bs[SB::PLANE + e]) = lo;
+                put(Bs, e, hi);
+                put(Bs + SB::PLANE, e, lo);
             }
         } else {
 #pragma unroll
             for (int j = 0; j < SB::NK; ++j) {
-                split_pair(b_reg[0][j], b_reg[1][j], hi, lo);
+#pragma unroll
+                for (int h = 0; h < Q / 2; ++h) split_pair(b_reg[2 * h][j], b_reg[2 * h + 1][j], hi[h], lo[h]);
                 const int e = SB::at(SB::row(tid, 0), SB::kidx(tid, j));
-                *reinterpret_cast<uint32_t *>(&Bs[e]) = hi;
-                *reinterpret_cast<uint32_t *>(&Bs[SB::PLANE + e]) = lo;
+                put(Bs, e, hi);
+                put(Bs + SB::PLANE, e, lo);
             }
         }
     };
@@ -289,10 +307,10 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
     // for tile kt+2 during step kt (its last reader, the load of tile kt, ran in step kt-1).
     const int nk = (p.K + kBK3 - 1) / kBK3;
     const int nfull = p.K / kBK3;  // tiles with every k < K
-    // 8-byte pair loads: the host checked strides, extents and pointer alignment; the batch offsets
-    // (and, for row pairs of A, the row range) are checked here, uniformly per workgroup
-    const bool vec = (p.vec & 3) == 3 && ((bt.a_off | bt.b_off) & 1) == 0 &&
-                     (A_KFAST || ((bt.m_begin | bt.m_end) & 1) == 0);
+    // vector loads: the host checked strides, extents and pointer alignment; the batch offsets
+    // (and, for row vectors of A, the row range) are checked here, uniformly per workgroup
+    const bool vec = (p.vec & 3) == 3 && ((bt.a_off | bt.b_off) & (Q - 1)) == 0 &&
+                     (A_KFAST || ((bt.m_begin | bt.m_end) & (Q - 1)) == 0);
     auto mainloop = [&](auto vtag) {
     fill_table(0, 0);
     fill_table(1, kBK3);
@@ -501,39 +519,40 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
 }
 
-template <int BM, int BN, bool C_MFAST, bool PACKED>
+template <int BM, int BN, bool C_MFAST, int MODE>
 void launch_bf16x3(ContractArgs<float> &args, bool a_kfast, bool b_nfast, dim3 grid, dim3 block,
                    hipStream_t stream) {
     if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, PACKED>), grid, block, 0, stream,
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, MODE>), grid, block, 0, stream,
                            args);
     else if (a_kfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, PACKED>), grid, block, 0, stream,
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, MODE>), grid, block, 0, stream,
                            args);
     else if (b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, PACKED>), grid, block, 0, stream,
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, MODE>), grid, block, 0, stream,
                            args);
     else
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, PACKED>), grid, block, 0, stream,
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, MODE>), grid, block, 0, stream,
                            args);
 }
 
 template <int BM, int BN>
 int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool a_kfast, bool b_nfast,
-               bool c_mfast, bool packed, hipStream_t s) {
+               bool c_mfast, int mode, hipStream_t s) {
     args.tiles_m = (int)((max_rows + BM - 1) / BM);
     args.tiles_n = (int)((args.N + BN - 1) / BN);
     args.total_tiles = nbatch * args.tiles_m * args.tiles_n;
     DCC_CHECK(args.total_tiles < ((int64_t)1 << 31), "too many tiles");
     const dim3 grid((unsigned)args.total_tiles), block(kThreads);
-    if (c_mfast && packed)
-        launch_bf16x3<BM, BN, true, true>(args, a_kfast, b_nfast, grid, block, s);
-    else if (c_mfast)
-        launch_bf16x3<BM, BN, true, false>(args, a_kfast, b_nfast, grid, block, s);
-    else if (packed)
-        launch_bf16x3<BM, BN, false, true>(args, a_kfast, b_nfast, grid, block, s);
-    else
-        launch_bf16x3<BM, BN, false, false>(args, a_kfast, b_nfast, grid, block, s);
+    if (c_mfast) {
+        if (mode == 2) launch_bf16x3<BM, BN, true, 2>(args, a_kfast, b_nfast, grid, block, s);
+        else if (mode == 1) launch_bf16x3<BM, BN, true, 1>(args, a_kfast, b_nfast, grid, block, s);
+        else launch_bf16x3<BM, BN, true, 0>(args, a_kfast, b_nfast, grid, block, s);
+    } else {
+        if (mode == 2) launch_bf16x3<BM, BN, false, 2>(args, a_kfast, b_nfast, grid, block, s);
+        else if (mode == 1) launch_bf16x3<BM, BN, false, 1>(args, a_kfast, b_nfast, grid, block, s);
+        else launch_bf16x3<BM, BN, false, 0>(args, a_kfast, b_nfast, grid, block, s);
+    }
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -571,17 +590,22 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
     const bool a_kfast = inner_stride(ak) == 1 || inner_stride(am) != 1;
     const bool b_nfast = inner_stride(bn) == 1 || inner_stride(bk) != 1;
     const bool c_mfast = inner_stride(cn) != 1;
-    args.vec = (pairs_contiguous(A, am, ak, a_kfast, 0, 8) ? 1 : 0) |
-               (pairs_contiguous(B, bn, bk, !b_nfast, N, 8) ? 2 : 0);
+    // packed operands whose layout allows it are loaded in quads (16 bytes), the rest in pairs
+    const bool quads = packed && vectors_contiguous(A, am, ak, a_kfast, 0, 4, 16) &&
+                       vectors_contiguous(B, bn, bk, !b_nfast, N, 4, 16);
+    const int mode = quads ? 2 : packed ? 1 : 0;
+    args.vec = quads ? 3
+                     : (pairs_contiguous(A, am, ak, a_kfast, 0, 8) ? 1 : 0) |
+                           (pairs_contiguous(B, bn, bk, !b_nfast, N, 8) ? 2 : 0);
     hipStream_t s = dcc::as_stream(stream);
     // default: 128x128 unless one side of the problem is small
     int shape = (int)tile_shape;
     if (shape < 0) shape = (max_rows <= 64 ? 0 : 1) + (N <= 64 ? 0 : 2);
     switch (shape) {
-        case 0: return run_bf16x3<64, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, packed, s);
-        case 1: return run_bf16x3<128, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, packed, s);
-        case 2: return run_bf16x3<64, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, packed, s);
-        default: return run_bf16x3<128, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, packed, s);
+        case 0: return run_bf16x3<64, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
+        case 1: return run_bf16x3<128, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
+        case 2: return run_bf16x3<64, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
+        default: return run_bf16x3<128, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
     }
 }
 

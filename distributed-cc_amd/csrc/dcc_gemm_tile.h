@@ -414,25 +414,31 @@ inline int64_t inner_stride(const dcc_idxmap &m) {
     return m.s0;
 }
 
-// Whether the element pairs a loader thread holds -- (k, k+1) of one row when the operand is staged
-// K-contiguous, rows (2i, 2i+1) at one k otherwise -- are adjacent in memory and aligned to the pair
-// size for every row and k (batch offsets and, for A, the row range parity are checked in the
-// kernel).  nrows: the row extent (N for B) must be even so no pair straddles its end (0 = checked
-// per batch in the kernel).
+// Whether the vectors of q elements a loader thread holds -- (k .. k+q-1) of one row when the operand
+// is staged K-contiguous, rows (q i .. q i + q-1) at one k otherwise -- are adjacent in memory and
+// aligned to the vector size (vec_bytes) for every row and k (batch offsets and, for A, the row range
+// are checked in the kernel).  nrows: the row extent (N for B) must be a multiple of q so no vector
+// straddles its end (0 = checked per batch in the kernel).
+inline bool vectors_contiguous(const void *base, const dcc_idxmap &rows, const dcc_idxmap &k, bool kfast,
+                               int64_t nrows, int64_t q, size_t vec_bytes) {
+    if (reinterpret_cast<uintptr_t>(base) % vec_bytes != 0 || rows.table != nullptr || k.table != nullptr)
+        return false;
+    const dcc_idxmap &vec = kfast ? k : rows;    // the map whose inner index the vector runs along
+    const dcc_idxmap &other = kfast ? rows : k;
+    if (inner_stride(vec) != 1) return false;
+    // vectors start at multiples of q: the inner level must have an extent divisible by q, or be the
+    // only level
+    if (vec.e2 > 1 ? vec.e2 % q != 0 : false) return false;
+    if (vec.e2 > 1 && (vec.s0 % q != 0 || vec.s1 % q != 0)) return false;
+    if (vec.e2 <= 1 && vec.e1 > 1 && (vec.e1 % q != 0 || vec.s0 % q != 0)) return false;
+    if (other.s0 % q != 0 || other.s1 % q != 0 || other.s2 % q != 0) return false;
+    if (!kfast && nrows % q != 0) return false;
+    return true;
+}
+
 inline bool pairs_contiguous(const void *base, const dcc_idxmap &rows, const dcc_idxmap &k, bool kfast,
                              int64_t nrows, size_t pair_bytes) {
-    if (reinterpret_cast<uintptr_t>(base) % pair_bytes != 0 || rows.table != nullptr || k.table != nullptr)
-        return false;
-    const dcc_idxmap &pair = kfast ? k : rows;   // the map whose inner index the pair runs along
-    const dcc_idxmap &other = kfast ? rows : k;
-    if (inner_stride(pair) != 1) return false;
-    // pairs start at even indices: the inner level must have an even extent, or be the only level
-    if (pair.e2 > 1 ? pair.e2 % 2 != 0 : false) return false;
-    if (pair.e2 > 1 && ((pair.s0 | pair.s1) & 1)) return false;
-    if (pair.e2 <= 1 && pair.e1 > 1 && (pair.e1 % 2 != 0 || (pair.s0 & 1))) return false;
-    if ((other.s0 | other.s1 | other.s2) & 1) return false;
-    if (!kfast && nrows % 2 != 0) return false;
-    return true;
+    return vectors_contiguous(base, rows, k, kfast, nrows, 2, pair_bytes);
 }
 
 // Tile shape id 0..7: 0..3 = 64x64, 128x64, 64x128, 128x128 with two-stage register prefetch,

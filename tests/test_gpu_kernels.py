@@ -284,21 +284,23 @@ def test_contract_bf16x3_plain_gemm(dev, M, N, K):
     assert abs(got - exact).max() < 1e-4 * scale * np.sqrt(K)         # split error ~2^-16 per product
 
 
-@pytest.mark.parametrize("even", [False, True])
+@pytest.mark.parametrize("vec", ["none", "pairs", "quads"])
 @pytest.mark.parametrize("packed", [False, True])
 @pytest.mark.parametrize("tile", [0, 1, 2, 3])
 @pytest.mark.parametrize("layout", ["kfast-nfast", "mfast-nfast", "kfast-kfast", "mfast-kfast-ct"])
-def test_contract_bf16x3_layouts(dev, layout, tile, packed, even):
+def test_contract_bf16x3_layouts(dev, layout, tile, packed, vec):
     """Every staging layout of the split-BF16x3 kernel (A contiguous along k or m, B along n or k),
     the LDS-staged epilogue (C contiguous along m), partial M/N tiles, a K tail and a batch offset,
     for each workgroup tile, with FP32 operands or operands packed by dcc_pack_bf16x3; checked
-    against the same split evaluated exactly on the host.  ``even`` sizes and offsets let the
-    kernel load element pairs as 8-byte vectors; odd ones force the per-element loads."""
+    against the same split evaluated exactly on the host.  ``vec`` picks sizes and offsets that force
+    per-element loads, allow 8-byte pair loads (extents 2 mod 4), or allow 16-byte quad loads of
+    packed operands (extents and offsets divisible by 4)."""
     from abi_emulator import bf16_split
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
-    M, N, K = (150, 84, 74) if even else (150, 83, 75)
-    _say("contract bf16x3 layout %s tile %d packed %d even %d" % (layout, tile, packed, even))
+    M, N, K = {"none": (150, 83, 75), "pairs": (150, 84, 74), "quads": (152, 84, 76)}[vec]
+    even = vec != "none"
+    _say("contract bf16x3 layout %s tile %d packed %d vec %s" % (layout, tile, packed, vec))
     rng = np.random.default_rng(11 + tile)
     A = rng.normal(size=(M, K)).astype(np.float32)
     B = rng.normal(size=(K, N)).astype(np.float32)
