@@ -279,6 +279,50 @@ def batched_ensemble(nsys, steps=3, seed=20261202):
     return out
 
 
+def probe_residual(o, v, precision, tiles, seed=20261202):
+    """One triples engine of the given precision on the synthetic system (nocc=o, nvir=v): tiles loaded
+    from ``tiles`` if it exists, else autotuned (and saved there when a path is given); the residual is
+    run twice and the second one is timed, per launch.  Used to profile one precision's kernels."""
+    from distr_cc.device import Event, Stream
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    dt, contract = {"fp64": (np.float64, "native"), "fp32": (np.float32, "native"),
+                    "bf16x3": (np.float32, "bf16x3")}[precision]
+    sysm = SyntheticSystem(o, v, naux=300, seed=seed)
+    st = Stream()
+    eng = TriplesEngine(o, v, stream=st, dtype=dt, contract=contract)
+    eng.set_t3(sysm.t3_tri().astype(dt))
+    eng.set_intermediates(sysm.bare_intermediates(), sysm.t2())
+    if tiles and os.path.exists(tiles):
+        eng.set_tile_config(json.load(open(tiles)))
+    else:
+        eng.autotune()
+        if tiles:
+            json.dump(eng.tile_config(), open(tiles, "w"))
+    eng.residual()
+    ev = {}
+
+    def timer(g, what):
+        e = Event()
+        e.record(st)
+        ev.setdefault(id(g), []).append((what, e))
+
+    a, b = Event(), Event()
+    a.record(st)
+    eng.residual(timer=timer)
+    b.record(st)
+    st.synchronize()
+    launches = []
+    for g in eng.launches:
+        lst = ev.get(id(g), [])
+        starts = [e for w, e in lst if w == "start"]
+        ends = [e for w, e in lst if w == "end"]
+        if starts and ends:
+            launches.append({"name": g.name, "ms": starts[0].elapsed_ms(ends[0]), "flops": getattr(g, "flops", 0.0)})
+    print(json.dumps({"probe": "triples residual", "precision": precision, "nocc": o, "nvir": v,
+                      "residual_ms": a.elapsed_ms(b), "launches": launches}))
+
+
 def calibrate_copy(mib):
     """dst = src over ``mib`` MiB of FP64 with dcc_tensor_axpby_f64 (reads and writes mib MiB each)."""
     import ctypes
@@ -319,6 +363,8 @@ def main():
     ap.add_argument("--batched", type=int, default=96, help="systems in the batched ensemble (0 = skip)")
     ap.add_argument("--q-nocc", type=int, default=5)
     ap.add_argument("--q-nvir", type=int, default=19, help="(Q) timing size (0 = skip)")
+    ap.add_argument("--probe-residual", choices=("fp64", "fp32", "bf16x3"), default=None,
+                    help="only time one engine's residual at --nocc/--nvir (tiles via --tiles) and exit")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -336,6 +382,9 @@ def main():
     init(0)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.probe_residual:
+        probe_residual(o, v, args.probe_residual, args.tiles)
         return
     print("bench: building synthetic system o=%d v=%d" % (o, v), file=sys.stderr, flush=True)
     sysm = SyntheticSystem(o, v, naux=300, seed=20261202)
