@@ -31,16 +31,70 @@ VIR = "abcdefgh"
 GEN = "PQRS"
 
 
+# split-K thresholds: contractions with K >= SPLIT_K_MIN whose output fills fewer than 64 tiles, peeled
+# K letter of at least SPLIT_MIN_SLICES values
+SPLIT_K_MIN = 4096
+SPLIT_MIN_SLICES = 8
+
+
+def _split_k_letter(sub, A, B, C, ext, tile=64):
+    """For a contraction whose M x N would occupy a few workgroups against a long K (an r1, F_oo or
+    Foo1-type term: one occupied or virtual index per side, K = o v^2), the K letter to peel into
+    batch entries that write partial slices: the largest one that is not the contiguous index of A
+    or B (so every slice still loads coalesced).  None when the contraction fills the chip anyway or
+    C is not a contiguous array in its letter order."""
+    sa, rest = sub.split(',')
+    sb, sc = rest.split('->')
+    kl = [x for x in sa if x in sb and x not in sc]
+    m = int(np.prod([ext[x] for x in sc if x in sa]))
+    n = int(np.prod([ext[x] for x in sc if x in sb]))
+    k = int(np.prod([ext[x] for x in kl]))
+    if -(-m // tile) * -(-n // tile) >= 64 or k < SPLIT_K_MIN or C.shape is None or tuple(C.letters) != tuple(sc):
+        return None
+    shape = [ext[x] for x in sc]
+    acc, want = 1, []
+    for e in reversed(shape):
+        want.append(acc)
+        acc *= e
+    if [C.strides[x] for x in sc] != list(reversed(want)):
+        return None
+    cand = [x for x in kl if A.strides[x] != 1 and B.strides[x] != 1 and SPLIT_MIN_SLICES <= ext[x] <= 1024]
+    return max(cand, key=lambda x: ext[x]) if cand else None
+
+
 class _Program:
     """An ordered list of device steps built once and replayed every iteration."""
 
     def __init__(self, engine):
         self.eng = engine
         self.steps = []
+        self.keep = []
 
     def ein(self, name, sub, A, B, C, alpha, beta):
-        g = plan_einsum(name, sub, A, B, C, alpha, beta, self.eng.ext, [({}, {}, {})])
+        ext = self.eng.ext
+        x = _split_k_letter(sub, A, B, C, ext)
+        if x is None:
+            g = plan_einsum(name, sub, A, B, C, alpha, beta, ext, [({}, {}, {})])
+            self.steps.append(g.run)
+            return
+        # split-K: slice s fixes K letter x = s and writes its own partial of C; the slices are then
+        # summed in a fixed order into C (deterministic, no atomics)
+        sa, rest = sub.split(',')
+        sb, sc = rest.split('->')
+        S = ext[x]
+        sl = next(c for c in "TUVWYZ" if c not in ext and c not in sub)
+        P = DeviceArray((S,) + tuple(ext[c] for c in sc), C.buf.dtype)
+        self.keep.append(P)
+        g = plan_einsum(name + " split-K", "%s,%s->%s" % (sa.replace(x, ""), sb.replace(x, ""), sc),
+                        A, B, Operand(P, sl + sc, P.shape), alpha, 0.0, dict(ext, **{sl: S}),
+                        [({x: s}, {x: s}, {sl: s}) for s in range(S)])
+        count = int(np.prod(P.shape[1:]))
+        fn = "dcc_sum_slices_" + ("f64" if C.buf.dtype == np.float64 else "f32")
+        dst = C.buf.offset_ptr(C.base)
+        stream = self.eng.stream
         self.steps.append(g.run)
+        self.steps.append(lambda _s: L.call(fn, dst, P.vp, S, count, 1.0, float(beta),
+                                            stream.handle if stream is not None else None))
 
     def axpby(self, dst, src, letters, alpha, beta):
         ext = self.eng.ext

@@ -128,13 +128,19 @@ def test_engine_fp32_plan(monkeypatch, o, v):
     assert abs(eng.get_t3() - t3o).max() <= 1e-5 * abs(t3o).max()
 
 
+@pytest.mark.parametrize("split", [False, True])
 @pytest.mark.parametrize("o,v,seed", [(2, 3, 1), (3, 4, 2), (1, 4, 3)])
-def test_full_iteration_plan(monkeypatch, o, v, seed):
-    """RCCSDTEngine.update_amps (dressing, lower order, T3, update) == oracle update_amps."""
+def test_full_iteration_plan(monkeypatch, o, v, seed, split):
+    """RCCSDTEngine.update_amps (dressing, lower order, T3, update) == oracle update_amps; ``split``
+    lowers the split-K thresholds so every long-K lower-order contraction takes the partial-slice path."""
     abi_emulator.install(monkeypatch)
     from ccsdt_cases import molecule_like, random_amplitudes
+    from distr_cc import rccsdt_engine as RE
     from distr_cc.rccsdt_engine import RCCSDTEngine
     from oracle import rccsdt_lower as lo
+    if split:
+        monkeypatch.setattr(RE, "SPLIT_K_MIN", 1)
+        monkeypatch.setattr(RE, "SPLIT_MIN_SLICES", 2)
     F, g, e, _ = molecule_like(o, v, seed)
     t1, t2, t3 = random_amplitudes(o, v, seed + 100)
     eng = RCCSDTEngine(o, v, F, g, e, level_shift=0.05, nvir_diis=max(1, v - 1))

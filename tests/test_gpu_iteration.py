@@ -23,11 +23,18 @@ def dev():
     return device
 
 
-@pytest.mark.parametrize("o,v,seed", [(1, 4, 3), (2, 3, 1), (3, 4, 2), (4, 7, 5), (5, 9, 6)])
-def test_full_iteration_vs_oracle(dev, o, v, seed):
+@pytest.mark.parametrize("o,v,seed,split", [(1, 4, 3, False), (2, 3, 1, False), (3, 4, 2, False), (4, 7, 5, False),
+                                          (5, 9, 6, False), (4, 7, 5, True), (5, 9, 6, True)])
+def test_full_iteration_vs_oracle(dev, monkeypatch, o, v, seed, split):
+    """``split`` lowers the split-K thresholds so the long-K lower-order contractions run as partial
+    slices plus dcc_sum_slices (at production sizes they do so by themselves)."""
     from ccsdt_cases import molecule_like, random_amplitudes
+    from distr_cc import rccsdt_engine as RE
     from distr_cc.rccsdt_engine import RCCSDTEngine
-    _say("full iteration o=%d v=%d" % (o, v))
+    if split:
+        monkeypatch.setattr(RE, "SPLIT_K_MIN", 1)
+        monkeypatch.setattr(RE, "SPLIT_MIN_SLICES", 2)
+    _say("full iteration o=%d v=%d split-K %d" % (o, v, split))
     F, g, e, _ = molecule_like(o, v, seed)
     t1, t2, t3 = random_amplitudes(o, v, seed + 100)
     eng = RCCSDTEngine(o, v, F, g, e, level_shift=0.05, nvir_diis=max(1, v - 1))
