@@ -621,6 +621,10 @@ class TriplesEngine:
         lo['P2'] = DeviceArray((o, qc, o, v, v), dt)     # split-K partials of r2 (E_vovv term)
         lo['P1'] = DeviceArray((qc * o, o, v, 1), dt)    # split-K partials of r1
         lo['P3'] = DeviceArray((o, qc, o, v, o), dt)     # split-K partials of W_vooo
+        # T3 terms of W_vooo / W_vvvo, accumulated while Z is built for T3 -> r1/r2 and added to the
+        # intermediates by t3_to_w, so each chunk of Z is built once per iteration
+        lo['dWoovo'] = DeviceArray((o, o, v, o), dt)     # as W_oovo [i,j,a,l]
+        lo['dWovvv'] = DeviceArray((o, v, v, v), dt)     # as W_ovvv [j,a,b,d]
         ext = {x: v for x in 'abcdexyz'}
         ext.update({'p': o, 'r': o, 'l': o, 'm': o, 'i': o, 'k': o, 'Q': qc, 'X': 1})
         Zs = (qc, o, o, v, v, v)
@@ -662,15 +666,15 @@ class TriplesEngine:
             a3.append(plan_einsum('t3->W_vooo q%d+%d' % (q0, nq), 'Qread,med->Qram', Z('Qpread'),
                                   Operand(lo['Eoovv_t'], 'mped', (o, o, v, v)),
                                   Operand(lo['P3'], 'SQram', (o, nq, o, v, o)), 1.0, 0.0, ext_c, e_split))
-            a3.append(('sum', self.W_oovo, q0 * o * v * o, lo['P3'], o, nq * o * v * o, 1.0))
+            a3.append(('sum', lo['dWoovo'], q0 * o * v * o, lo['P3'], o, nq * o * v * o, 1.0))
             a3.append(plan_einsum('t3->W_vvvo q%d+%d' % (q0, nq), 'Qpreba,rpde->Qabd', Z('Qpreba'),
                                   Operand(lo['Eoovv'], 'rpde', (o, o, v, v)),
-                                  Operand(self.W_ovvv, 'Qabd', (o, v, v, v)), -1.0, 1.0, ext_c,
+                                  Operand(lo['dWovvv'], 'Qabd', (o, v, v, v)), -1.0, 1.0, ext_c,
                                   [({}, {}, {'Q': q0})]))
             self._retarget([g for g in a2 + a3 if isinstance(g, GemmLaunch)])
             chunks.append((DeviceArray.from_host(s3_jobs_array(jobs)), len(jobs), a2, a3))
         lo['chunks'] = chunks
-        lo['z_valid'] = False
+        lo['dw_valid'] = False
         self._lower = lo
 
     def set_t1_dressed(self, t1_fock_ov, E_oovv, E_ooov, E_vovv):
@@ -748,31 +752,44 @@ class TriplesEngine:
     def t3_to_r1r2(self, download=True):
         """T3 contributions to r1 and the un-symmetrized r2 (rccsdt.py:253-321).
 
-        Returns host arrays, or with ``download=False`` the device buffers
+        The T3 terms of W_vooo / W_vvvo (rccsdt.py:323-366) are computed from the same chunks of Z
+        and kept for t3_to_w.  Returns host arrays, or with ``download=False`` the device buffers
         r1 (o, v, 1) and r2 (o, o, v, v)."""
         lo = self._lower
-        lo['r1'].fill(0.0, self.stream)
-        lo['r2'].fill(0.0, self.stream)
+        for buf in ('r1', 'r2', 'dWoovo', 'dWovvv'):
+            lo[buf].fill(0.0, self.stream)
         for c, chunk in enumerate(lo['chunks']):
             self._build_z(c)
             self._run_lower(chunk[2])
-        lo['z_valid'] = len(lo['chunks']) == 1
+            self._run_lower(chunk[3])
+        lo['dw_valid'] = True
         if not download:
             return lo['r1'], lo['r2']
         return (lo['r1'].download(stream=self.stream).reshape(self.nocc, self.nvir).astype(np.float64),
                 lo['r2'].download(stream=self.stream).astype(np.float64))
 
+    def _add_dw(self):
+        s = self.stream.handle if self.stream is not None else None
+        for dst, src in ((self.W_oovo, self._lower['dWoovo']), (self.W_ovvv, self._lower['dWovvv'])):
+            lay = L.Strided()
+            lay.ndim = 1
+            lay.shape[0], lay.dst_stride[0], lay.src_stride[0] = src.size, 1, 1
+            L.call("dcc_tensor_axpby_" + self.sfx, dst.vp, src.vp, ctypes.byref(lay), 1.0, 1.0, s)
+
     def t3_to_w(self, reuse_z=False):
         """Add the T3 terms of rccsdt.py:323-366 to the device W_vooo / W_vvvo (after set_intermediates).
 
-        ``reuse_z`` keeps the Z built by the preceding t3_to_r1r2 when T3 and the dressed integrals
-        have not changed since (one chunk only)."""
+        ``reuse_z`` adds the terms computed by the preceding t3_to_r1r2 when T3 and the dressed
+        integrals have not changed since; otherwise they are computed here, chunk by chunk."""
         lo = self._lower
-        for c, chunk in enumerate(lo['chunks']):
-            if not (reuse_z and lo['z_valid']):
+        if not (reuse_z and lo['dw_valid']):
+            for buf in ('dWoovo', 'dWovvv'):
+                lo[buf].fill(0.0, self.stream)
+            for c, chunk in enumerate(lo['chunks']):
                 self._build_z(c)
-            self._run_lower(chunk[3])
-        lo['z_valid'] = False
+                self._run_lower(chunk[3])
+        self._add_dw()
+        lo['dw_valid'] = False
 
     def get_w_t3_adjusted(self):
         """Device W_vooo, W_vvvo (incl. T3 terms) back in the reference layouts."""
