@@ -78,7 +78,11 @@ def _pad(x, axes_v, vp, o):
 
 
 class QEngine:
-    """Device state of one (Q)/[Q] evaluation."""
+    """Device state of (Q)/[Q] evaluations for one (nocc, nvir, blksize).
+
+    The device buffers and the launch plans (index maps and batch tables of every GEMM) are built on
+    the first kernel() call and reused by later calls of the same engine, which only upload the new
+    integrals and amplitudes and replay the launches."""
 
     def __init__(self, nocc, nvir, blksize=4, stream=None, device=0, max_chunk_bytes=1 << 30):
         device_init(device)
@@ -90,82 +94,104 @@ class QEngine:
         o, vp, bs = self.o, self.vp, self.bs
         per_task = bs ** 4 * o ** 4 * 8
         self.chunk = max(1, min(int(max_chunk_bytes // per_task), 4096))
+        self.dev = None
+        self.wplans = None
+        self.task_plans = {}
 
     def _s(self):
         return self.stream.handle if self.stream is not None else None
 
-    def kernel(self, pppp, t2_oovv, t3_dev, mo_energy, job_idx=0, n_jobs=1):
-        """(e_[Q], e_(Q)) for the canonical-block T3 already on the device (``t3_dev``)."""
-        o, v, vp, bs = self.o, self.v, self.vp, self.bs
-        e = np.asarray(mo_energy, dtype=np.float64)
-        pppp = np.asarray(pppp)
-        t2q = np.asarray(t2_oovv).transpose(2, 3, 0, 1)
-        dev = {}
-        dev['vvvo'] = DeviceArray.from_host(_pad(pppp[o:, o:, o:, :o], (0, 1, 2), vp, o))
-        dev['vooo'] = DeviceArray.from_host(_pad(pppp[o:, :o, :o, :o], (0,), vp, o))
-        dev['vvoo'] = DeviceArray.from_host(_pad(pppp[o:, o:, :o, :o], (0, 1), vp, o))
-        dev['ovvo'] = DeviceArray.from_host(_pad(pppp[:o, o:, o:, :o], (1, 2), vp, o))
-        dev['ovov'] = DeviceArray.from_host(_pad(pppp[:o, o:, :o, o:], (1, 3), vp, o))
-        dev['vvvv'] = DeviceArray.from_host(_pad(pppp[o:, o:, o:, o:], (0, 1, 2, 3), vp, o))
-        dev['oooo'] = DeviceArray.from_host(np.ascontiguousarray(pppp[:o, :o, :o, :o]))
-        dev['t2'] = DeviceArray.from_host(_pad(t2q, (0, 1), vp, o))
-        dev['e_occ'] = DeviceArray.from_host(np.ascontiguousarray(e[:o]))
-        dev['e_vir'] = DeviceArray.from_host(np.concatenate([e[o:], np.zeros(vp - v)]))
-        dev['T'] = DeviceArray((vp, vp, vp, o, o, o))
-        L.call("dcc_t3_to_abc_f64", t3_dev.vp, o, v, vp, dev['T'].vp, self._s())
-        dev['Wv'] = DeviceArray((vp, vp, vp, vp, o, o))
-        dev['Wo'] = DeviceArray((vp, vp, o, o, o, o))
+    def _op(self, name, letters):
         shapes = {'vvvo': 'vvvo', 'vooo': 'vooo', 'vvoo': 'vvoo', 'ovvo': 'ovvo', 'ovov': 'ovov',
                   'vvvv': 'vvvv', 'oooo': 'oooo', 't2': 'vvoo', 'T': 'vvvooo', 'Wv': 'vvvvoo', 'Wo': 'vvoooo'}
+        shp = tuple(self.vp if ch == 'v' else self.o for ch in shapes[name])
+        return Operand(self.dev[name], letters, shp)
 
-        def op(name, letters):
-            shp = tuple(vp if ch == 'v' else o for ch in shapes[name])
-            return Operand(dev[name], letters, shp)
-
+    def _setup(self):
+        o, vp = self.o, self.vp
+        shp = {'vvvo': (vp, vp, vp, o), 'vooo': (vp, o, o, o), 'vvoo': (vp, vp, o, o), 'ovvo': (o, vp, vp, o),
+               'ovov': (o, vp, o, vp), 'vvvv': (vp, vp, vp, vp), 'oooo': (o, o, o, o), 't2': (vp, vp, o, o),
+               'e_occ': (o,), 'e_vir': (vp,), 'T': (vp, vp, vp, o, o, o), 'Wv': (vp, vp, vp, vp, o, o),
+               'Wo': (vp, vp, o, o, o, o)}
+        self.dev = {k: DeviceArray(s) for k, s in shp.items()}
         full = {x: vp for x in 'abcdef'}
         full.update({x: o for x in 'ijklmn'})
         # dense W_vvvvoo / W_vvoooo (beta = 0 on the first term of each)
-        for n, (sub, an, bn, alpha) in enumerate(WV_TERMS):
-            sa, rest = sub.split(',')
-            sb, sc = rest.split('->')
-            plan_einsum('Q W_vvvvoo', sub, op(an, sa), op(bn, sb), op('Wv', sc), alpha, 0.0 if n == 0 else 1.0,
-                        full, [({}, {}, {})]).run(self.stream)
-        for n, (sub, an, bn, alpha) in enumerate(WO_TERMS):
-            sa, rest = sub.split(',')
-            sb, sc = rest.split('->')
-            plan_einsum('Q W_vvoooo', sub, op(an, sa), op(bn, sb), op('Wo', sc), alpha, 0.0 if n == 0 else 1.0,
-                        full, [({}, {}, {})]).run(self.stream)
+        self.wplans = []
+        for terms, out in ((WV_TERMS, 'Wv'), (WO_TERMS, 'Wo')):
+            for n, (sub, an, bn, alpha) in enumerate(terms):
+                sa, rest = sub.split(',')
+                sb, sc = rest.split('->')
+                self.wplans.append(plan_einsum('Q W', sub, self._op(an, sa), self._op(bn, sb), self._op(out, sc),
+                                               alpha, 0.0 if n == 0 else 1.0, full, [({}, {}, {})]))
 
-        tasks = create_full_task_list(v, bs)
-        per = (len(tasks) + n_jobs - 1) // n_jobs
-        tasks = tasks[job_idx * per: min(job_idx * per + per, len(tasks))]
+    def _plan_tasks(self, tasks):
+        """Launches of every chunk of ``tasks``; the chunks share one pair of z4/t4 buffers."""
+        o, vp, bs = self.o, self.vp, self.bs
         ext = {x: bs for x in 'abcd'}
         ext.update({'e': vp, 'f': vp})
         ext.update({x: o for x in 'ijklmn'})
-        paren = bracket = 0.0
+        nmax = min(self.chunk, len(tasks))
+        z4 = DeviceArray((nmax, bs, bs, bs, bs, o, o, o, o))
+        t4 = DeviceArray((nmax, bs, bs, bs, bs, o, o, o, o))
+        part = DeviceArray((nmax * bs ** 4, 2))
+        tot = DeviceArray((2,))
+        chunks = []
         for c0 in range(0, len(tasks), self.chunk):
             chunk = tasks[c0:c0 + self.chunk]
             nt = len(chunk)
             ext['H'] = nt
-            z4 = DeviceArray((nt, bs, bs, bs, bs, o, o, o, o))
-            t4 = DeviceArray((nt, bs, bs, bs, bs, o, o, o, o))
             blocks = [dict(a=a0, b=b0, c=cc, d=d0) for (a0, b0, cc, d0) in chunk]
+            launches = []
             for out, terms in ((z4, Z4_TERMS), (t4, T4_TERMS)):
-                C = Operand(out, 'Habcdijkl', (nt, bs, bs, bs, bs, o, o, o, o))
+                C = Operand(out, 'Habcdijkl', (nmax, bs, bs, bs, bs, o, o, o, o))
                 for n, (sub, an, bn, alpha) in enumerate(terms):
                     sa, rest = sub.split(',')
                     sb = rest.split('->')[0]
                     batches = [(blk, blk, {'H': h}) for h, blk in enumerate(blocks)]
-                    plan_einsum('Q ' + sub, sub, op(an, sa), op(bn, sb), C, alpha, 0.0 if n == 0 else 1.0,
-                                ext, batches).run(self.stream)
+                    launches.append(plan_einsum('Q ' + sub, sub, self._op(an, sa), self._op(bn, sb), C, alpha,
+                                                0.0 if n == 0 else 1.0, ext, batches))
             tk = DeviceArray.from_host(np.array(chunk, dtype=np.int32).reshape(-1, 4))
+            chunks.append((nt, launches, tk))
+        return z4, t4, part, tot, chunks
+
+    def kernel(self, pppp, t2_oovv, t3_dev, mo_energy, job_idx=0, n_jobs=1):
+        """(e_[Q], e_(Q)) for the canonical-block T3 already on the device (``t3_dev``)."""
+        o, v, vp, bs = self.o, self.v, self.vp, self.bs
+        if self.dev is None:
+            self._setup()
+        dev, st = self.dev, self.stream
+        e = np.asarray(mo_energy, dtype=np.float64)
+        pppp = np.asarray(pppp)
+        t2q = np.asarray(t2_oovv).transpose(2, 3, 0, 1)
+        dev['vvvo'].upload(_pad(pppp[o:, o:, o:, :o], (0, 1, 2), vp, o), st)
+        dev['vooo'].upload(_pad(pppp[o:, :o, :o, :o], (0,), vp, o), st)
+        dev['vvoo'].upload(_pad(pppp[o:, o:, :o, :o], (0, 1), vp, o), st)
+        dev['ovvo'].upload(_pad(pppp[:o, o:, o:, :o], (1, 2), vp, o), st)
+        dev['ovov'].upload(_pad(pppp[:o, o:, :o, o:], (1, 3), vp, o), st)
+        dev['vvvv'].upload(_pad(pppp[o:, o:, o:, o:], (0, 1, 2, 3), vp, o), st)
+        dev['oooo'].upload(np.ascontiguousarray(pppp[:o, :o, :o, :o], dtype=np.float64), st)
+        dev['t2'].upload(_pad(t2q, (0, 1), vp, o), st)
+        dev['e_occ'].upload(np.ascontiguousarray(e[:o]), st)
+        dev['e_vir'].upload(np.concatenate([e[o:], np.zeros(vp - v)]), st)
+        L.call("dcc_t3_to_abc_f64", t3_dev.vp, o, v, vp, dev['T'].vp, self._s())
+        for g in self.wplans:
+            g.run(st)
+        key = (int(job_idx), int(n_jobs))
+        if key not in self.task_plans:
+            tasks = create_full_task_list(v, bs)
+            per = (len(tasks) + n_jobs - 1) // n_jobs
+            self.task_plans[key] = self._plan_tasks(tasks[job_idx * per: min(job_idx * per + per, len(tasks))])
+        z4, t4, part, tot, chunks = self.task_plans[key]
+        paren = bracket = 0.0
+        for nt, launches, tk in chunks:
+            for g in launches:
+                g.run(st)
             nwg = nt * bs ** 4
-            part = DeviceArray((nwg, 2))
             L.call("dcc_q_finalize_f64", t4.vp, z4.vp, tk.vp, nt, bs, o, v, vp, dev['e_occ'].vp, dev['e_vir'].vp,
                    dev['vvoo'].vp, dev['t2'].vp, part.vp, self._s())
-            tot = DeviceArray((2,))
             L.call("dcc_reduce_pairs_f64", part.vp, nwg, tot.vp, self._s())
-            p, b = tot.download(stream=self.stream)
+            p, b = tot.download(stream=st)
             paren += p
             bracket += b
         return bracket / 12.0, (paren + bracket) / 12.0
