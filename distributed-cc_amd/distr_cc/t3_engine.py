@@ -113,9 +113,10 @@ class GemmLaunch:
         self.name = name
         self.args = (int(M), int(N), int(K), float(alpha), float(beta), ctypes.c_void_p(A.ptr), am, ak,
                      ctypes.c_void_p(B.ptr), bk, bn, ctypes.c_void_p(C.ptr), cm, cn)
-        bt = np.zeros(len(batches), dtype=L.BATCH_DTYPE)
-        for n, (ao, bo, co, mb, me) in enumerate(batches):
-            bt[n] = (ao, bo, co, mb, me)
+        rows = np.asarray(batches, dtype=np.int64).reshape(-1, 5)
+        bt = np.zeros(rows.shape[0], dtype=L.BATCH_DTYPE)
+        for n, field in enumerate(L.BATCH_DTYPE.names):
+            bt[field] = rows[:, n]
         self.batches = DeviceArray.from_host(bt)
         self.nbatch = len(batches)
         self.max_rows = int(max_rows)
@@ -188,20 +189,22 @@ def plan_einsum(name, sub, A, B, C, alpha, beta, ext, batches, max_levels=3):
         peel += gm.pop(0)
     while len(gn) > max_levels:
         peel += gn.pop(0)
-    entries = list(batches)
+    # base offsets of the given entries, then every value of each peeled letter (entry-major order)
+    offs = np.array([(A.offset(fa), B.offset(fb), C.offset(fc)) for fa, fb, fc in batches],
+                    dtype=np.int64).reshape(-1, 3)
     for x in peel:
-        grown = []
-        for fa, fb, fc in entries:
-            for val in range(ext[x]):
-                grown.append(tuple(dict(f, **{x: f.get(x, 0) + val}) for f in (fa, fb, fc)))
-        entries = grown
+        sx = np.array([op.strides.get(x, 0) for op in (A, B, C)], dtype=np.int64)
+        vals = np.arange(ext[x], dtype=np.int64)
+        offs = (offs[:, None, :] + vals[None, :, None] * sx[None, None, :]).reshape(-1, 3)
     am, M = _group_map(gm, ext, A)
     ak, K = _group_map(gk, ext, A)
     bk, _ = _group_map(gk, ext, B)
     bn, N = _group_map(gn, ext, B)
     cm, _ = _group_map(gm, ext, C)
     cn, _ = _group_map(gn, ext, C)
-    bt = [(A.offset(fa), B.offset(fb), C.offset(fc), 0, M) for fa, fb, fc in entries]
+    bt = np.zeros((offs.shape[0], 5), dtype=np.int64)
+    bt[:, :3] = offs
+    bt[:, 4] = M
     return GemmLaunch(name, M, N, K, alpha, beta, A.buf, am, ak, B.buf, bk, bn, C.buf, cm, cn, bt, M,
                       2.0 * M * N * K * len(bt))
 
