@@ -488,7 +488,8 @@ def main():
         print("  sum of GEMM launches %.3f ms of %.3f ms per step" % (tot, ms), file=sys.stderr)
         a2 = [b.elapsed_ms(c) for (w1, b), (w2, c) in zip(phase_ev[0::3], phase_ev[1::3])]
         a3 = [b.elapsed_ms(c) for (w1, b), (w2, c) in zip(phase_ev[1::3], phase_ev[2::3])]
-        print("  T3->r1/r2 (A2) %.3f ms, T3->W (A3) %.3f ms" % (np.mean(a2), np.mean(a3)), file=sys.stderr)
+        print("  T3->r1/r2 with the T3->W terms (A2, A3 contractions) %.3f ms, adding the T3->W terms %.3f ms"
+              % (np.mean(a2), np.mean(a3)), file=sys.stderr)
 
     F = algorithmic_flops(o, v)
     out = {
