@@ -412,9 +412,12 @@ def main():
     phase_ev = []
 
     def step(timer=None):
-        """A2 + A3 + A4-A6 + A7-A9: T3 -> r1/r2, T3 -> W (on W rebuilt from T2), residual, update."""
+        """A2 + A3 + A4-A6 + A7-A9: T3 -> r1/r2, T3 -> W (on W rebuilt from T2), the residual's operands
+        from the intermediates (ring weights, W_oooo/F_oo prefactors, F_vv folded into W_vvvv),
+        residual, update."""
         for src, dst in zip(w_saved, (eng.W_oovo, eng.W_ovvv)):
             L.call("dcc_memcpy_dtod", dst.vp, src.vp, src.nbytes, stream.handle)
+        eng.prepare_residual()
         if timer is not None:
             e = Event()
             e.record(stream)

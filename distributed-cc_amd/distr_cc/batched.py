@@ -4,8 +4,9 @@ Every system keeps its own device buffers (a TriplesEngine each); the launch lis
 have the same structure, so launch p of every system is one residual term of that system.  Those
 contractions write disjoint outputs (each system's own R), so they run as ONE persistent grouped
 GEMM launch (dcc_contract_grouped_f64) per term; the bandwidth passes (Y gather, W_oooo/F_oo
-permuted adds, fused finalize) stay per system.  ``capture()`` records a whole step -- R reset, Y
-gather, all grouped GEMMs, all finalizes -- into one HIP graph, so a step is one graph launch.
+permuted adds, fused finalize) stay per system.  ``capture()`` records a whole step -- the residual
+operands from the intermediates, R reset, Y gather, all grouped GEMMs, all finalizes -- into one HIP
+graph, so a step is one graph launch.
 ``autotune()`` times the eight tile variants of the grouped kernel for every term and keeps the fastest.
 """
 import ctypes
@@ -91,6 +92,7 @@ class BatchedTriples:
         """Issue one triples update of every system on the stream (no host synchronisation)."""
         st = self.stream
         for e in self.engines:
+            e.prepare_residual()
             e.R.fill(0.0, st)
             e.materialize_y()
         for ph in self.phases:

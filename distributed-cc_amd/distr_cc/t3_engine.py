@@ -374,6 +374,11 @@ class TriplesEngine:
         self.W_ovvv = DeviceArray((o, v, v, v), dt)   # W_vvvo[a,b,d,j] stored as [j,a,b,d]
         self.W_oovo = DeviceArray((o, o, v, o), dt)   # W_vooo[a,l,i,j] stored as [i,j,a,l]
         self.W_vvvv = DeviceArray((v, v, v, v), dt)
+        # W_vvvv with the F_vv terms folded in (built by prepare_residual): the first two W_vvvv terms
+        # read W1[a,b,d,e] = W_vvvv[a,b,d,e] + F_vv[a,d] d_be + F_vv[b,e] d_ad, the second
+        # W2[a,c,d,e] = W_vvvv[a,c,d,e] + F_vv[c,e] d_ad, so rccsdt.py:400-402 need no launches of their own
+        self.W1 = DeviceArray((v, v, v, v), dt)
+        self.W2 = DeviceArray((v, v, v, v), dt)
         self.WR1 = DeviceArray((o, v, o, v), dt)
         self.WR3 = DeviceArray((2, o, v, o, v), dt)   # [W_ovvo term, (W_ovov + W_ovvo) term]
         w3 = (o * v) ** 2 * dt.itemsize
@@ -424,7 +429,7 @@ class TriplesEngine:
         tensor, so each element is split once per residual instead of once per tile that reads it.
         Y and Z are packed in place (they are rebuilt from T3 for every residual and read by nothing
         else); the others get packed copies.  Returns the (source, packed, count) passes to run."""
-        sources = [self.t3, self.Yz, self.W_vvvv, self.F_vv, self.W_ovvv, self.W_oovo, self.t2, self.WR1,
+        sources = [self.t3, self.Yz, self.W_vvvv, self.W1, self.W2, self.F_vv, self.W_ovvv, self.W_oovo, self.t2, self.WR1,
                    self.WR3, self.pfs]
         packed = {}
 
@@ -473,14 +478,12 @@ class TriplesEngine:
         # T2 -> T3 (rccsdt.py:388-399)
         Rb = Operand(self.R, 'Babc', (n3, v, v, v))
         launches += self._plan_t2t3()
-        # F_vv and W_vvvv x T3 with the block index in N (rccsdt.py:400-402, 424-426)
+        # W_vvvv x T3 with the block index in N (rccsdt.py:424-426); the F_vv terms (rccsdt.py:400-402)
+        # ride along in W1 and W2 (see __init__)
         one = [dict()]
-        for sub, tl in [('ad,Bdbc->Babc', 'Bdbc'), ('bd,Badc->Babc', 'Badc'), ('cd,Babd->Babc', 'Babd')]:
-            A = Operand(self.F_vv, sub[:2], (v, v))
-            B = Operand(self.t3, tl, (n3, v, v, v))
-            launches.append(plan_term('F_vv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
-        for sub, tl in [('abde,Bdec->Babc', 'Bdec'), ('acde,Bdbe->Babc', 'Bdbe'), ('bcde,Bade->Babc', 'Bade')]:
-            A = Operand(self.W_vvvv, sub[:4], (v, v, v, v))
+        for sub, tl, W in [('abde,Bdec->Babc', 'Bdec', self.W1), ('acde,Bdbe->Babc', 'Bdbe', self.W2),
+                           ('bcde,Bade->Babc', 'Bade', self.W_vvvv)]:
+            A = Operand(W, sub[:4], (v, v, v, v))
             B = Operand(self.t3, tl, (n3, v, v, v))
             launches.append(plan_term('W_vvvv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
         launches += self._plan_ring()
@@ -730,9 +733,19 @@ class TriplesEngine:
             (o, o, v, v), -0.5, 1.0)
 
     def prepare_residual(self):
-        """Ring weights and W_oooo/F_oo prefactors from the device intermediates."""
+        """Ring weights, W_oooo/F_oo prefactors and the F_vv-folded W1, W2 from the device intermediates."""
         o, v = self.nocc, self.nvir
         s = self.stream.handle if self.stream is not None else None
+        v2, v3 = v * v, v ** 3
+        for W in (self.W1, self.W2):
+            L.call("dcc_memcpy_dtod", W.vp, self.W_vvvv.vp, W.nbytes, s)
+        F = self.F_vv
+        # W1[a,b,d,b] += F[a,d];  W1[a,b,a,e] += F[b,e];  W2[a,c,a,e] += F[c,e]
+        tensor_axpby(Operand.strided(self.W1, {'a': v3, 'b': v2 + 1, 'd': v}),
+                     Operand.strided(F, {'a': v, 'd': 1}), 'abd', (v, v, v), 1.0, 1.0, self.stream)
+        for W in (self.W1, self.W2):
+            tensor_axpby(Operand.strided(W, {'a': v3 + v, 'b': v2, 'e': 1}),
+                         Operand.strided(F, {'b': v, 'e': 1}), 'abe', (v, v, v), 1.0, 1.0, self.stream)
         L.call("dcc_ring_weights_" + self.sfx, self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
                self.WR3b.vp, s)
         L.call("dcc_oooo_prefactors_" + self.sfx, self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
