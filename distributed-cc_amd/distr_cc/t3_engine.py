@@ -9,6 +9,7 @@ Data in HBM (FP64, C order):
   Gs             (pairs per chunk, o, v, v, v)  G[r;pq][a,b,c] before its placement
   Xt             (i per chunk, o, o, v, v, v)   T2->T3 base term X over ordered (i,j,k)
   pfs            (6, n3, n3)       W_oooo/F_oo prefactors
+  W1, W2         (v, v, v, v)      W_vvvv with the F_vv terms folded in
   Xp             (5, targets per chunk, v, v, v)  W_oooo/F_oo terms s = 1..5 before their permutation
   intermediates  shapes of distr_cc/rccsdt.py:380,445,625
 
@@ -16,7 +17,8 @@ The residual follows distr_cc/rccsdt.py:368-433 term by term, grouped into
 batched gather-GEMMs (dcc_contract_f64):
   * T2->T3 (rccsdt.py:388-399): the base term X[ijk] for every ordered (i,j,k) (2 GEMMs), then one
     S3 gather pass: each canonical block adds its six simultaneously (ijk, abc)-permuted sources;
-  * F_vv and W_vvvv x T3 (rccsdt.py:400-402,424-426): 6 launches, N = (block, ...);
+  * F_vv and W_vvvv x T3 (rccsdt.py:400-402,424-426): 3 launches, N = (block, ...); the F_vv terms
+    are folded into two copies of W_vvvv with F_vv added on their diagonals (prepare_residual);
   * W_ovov/W_ovvo x T3 (rccsdt.py:540-596): with G the part-3 operator
         G[r;pq][a,b,c] = sum_{l,d} ( -W_ovov[l,b,r,d] Y[l][a,d,c] - W_ovov[l,c,r,d] Y[l][a,b,d]
                         + W_ovvo[l,a,d,r] Y[l][d,b,c]
