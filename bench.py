@@ -121,14 +121,16 @@ def mixed_vs_fp64(o, v, iters, seed=20261202, sigma_b=0.01, autotune="direct"):
     for name, dt, contract in (("fp64", np.float64, "native"), ("fp32", np.float32, "native"),
                                ("bf16x3", np.float32, "bf16x3")):
         print("bench: mixed %s engine nocc=%d nvir=%d" % (name, o, v), file=sys.stderr, flush=True)
+        # tiles tuned on the small system first, so that the two engines never share HBM
+        tiles = _tiles_from_small(dt, contract, seed, sigma_b) if autotune == "transfer" else None
         st = Stream()
         eng = RCCSDTEngine(o, v, sysm.fock, pppp, sysm.mo_energy, stream=st, t3_dtype=dt,
                            triples_contract=contract)
         eng.set_amplitudes(t1, t2, t3)
         if autotune == "direct":
             eng.tr.autotune()
-        elif autotune == "transfer":
-            eng.tr.set_tiles_by_name(_tiles_from_small(dt, contract, seed, sigma_b))
+        elif tiles is not None:
+            eng.tr.set_tiles_by_name(tiles)
         st.synchronize()
         eng.update_amps()       # warm-up iteration (first-launch costs), then restart
         print("bench: mixed %s warm-up done" % name, file=sys.stderr, flush=True)

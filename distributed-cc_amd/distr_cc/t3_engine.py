@@ -9,7 +9,7 @@ Data in HBM (FP64, C order):
   Gs             (pairs per chunk, o, v, v, v)  G[r;pq][a,b,c] before its placement
   Xt             (i per chunk, o, o, v, v, v)   T2->T3 base term X over ordered (i,j,k)
   pfs            (6, n3, n3)       W_oooo/F_oo prefactors
-  W1, W2         (v, v, v, v)      W_vvvv with the F_vv terms folded in
+  W1             (v, v, v, v)      W_vvvv with the F_vv terms folded in
   Xp             (5, targets per chunk, v, v, v)  W_oooo/F_oo terms s = 1..5 before their permutation
   intermediates  shapes of distr_cc/rccsdt.py:380,445,625
 
@@ -18,7 +18,7 @@ batched gather-GEMMs (dcc_contract_f64):
   * T2->T3 (rccsdt.py:388-399): the base term X[ijk] for every ordered (i,j,k) (2 GEMMs), then one
     S3 gather pass: each canonical block adds its six simultaneously (ijk, abc)-permuted sources;
   * F_vv and W_vvvv x T3 (rccsdt.py:400-402,424-426): 3 launches, N = (block, ...); the F_vv terms
-    are folded into two copies of W_vvvv with F_vv added on their diagonals (prepare_residual);
+    are folded into one copy of W_vvvv with F_vv/2 added on two of its diagonals (prepare_residual);
   * W_ovov/W_ovvo x T3 (rccsdt.py:540-596): with G the part-3 operator
         G[r;pq][a,b,c] = sum_{l,d} ( -W_ovov[l,b,r,d] Y[l][a,d,c] - W_ovov[l,c,r,d] Y[l][a,b,d]
                         + W_ovvo[l,a,d,r] Y[l][d,b,c]
@@ -358,8 +358,9 @@ class TriplesEngine:
         if ring_fused is None:
             from distr_cc.device import mem_get_info
             free = mem_get_info()[0]
-            # room for Z plus the rest of the engine (W_oooo scratch, lower-order chunks) and a margin
-            ring_fused = free > (ysize * 2 + n3 * v3 * 2) * dt.itemsize + (24 << 30)
+            # room for Z plus the rest of the engine (W_vvvv and its F_vv-folded copy, W_oooo scratch,
+            # lower-order chunks) and a margin
+            ring_fused = free > (ysize * 2 + n3 * v3 * 2 + v3 * v) * dt.itemsize + (24 << 30)
         self.ring_fused = bool(ring_fused)
         self.Yz = DeviceArray((2 if self.ring_fused else 1, self.npairs, o, v, v, v), dt)
         self.Y = DeviceArray((self.npairs, o, v, v, v), dt, ptr=self.Yz.ptr, owner=False)
@@ -376,11 +377,11 @@ class TriplesEngine:
         self.W_ovvv = DeviceArray((o, v, v, v), dt)   # W_vvvo[a,b,d,j] stored as [j,a,b,d]
         self.W_oovo = DeviceArray((o, o, v, o), dt)   # W_vooo[a,l,i,j] stored as [i,j,a,l]
         self.W_vvvv = DeviceArray((v, v, v, v), dt)
-        # W_vvvv with the F_vv terms folded in (built by prepare_residual): the first two W_vvvv terms
-        # read W1[a,b,d,e] = W_vvvv[a,b,d,e] + F_vv[a,d] d_be + F_vv[b,e] d_ad, the second
-        # W2[a,c,d,e] = W_vvvv[a,c,d,e] + F_vv[c,e] d_ad, so rccsdt.py:400-402 need no launches of their own
+        # W_vvvv with the F_vv terms folded in (built by prepare_residual), read by all three W_vvvv
+        # terms: W1[p,q,d,e] = W_vvvv[p,q,d,e] + 1/2 F_vv[p,d] d_qe + 1/2 F_vv[q,e] d_pd.  The three
+        # terms then add 1/2 (F1 + F2) + 1/2 (F1 + F3) + 1/2 (F2 + F3) = F1 + F2 + F3, the F_vv terms of
+        # rccsdt.py:400-402, which need no launches of their own
         self.W1 = DeviceArray((v, v, v, v), dt)
-        self.W2 = DeviceArray((v, v, v, v), dt)
         self.WR1 = DeviceArray((o, v, o, v), dt)
         self.WR3 = DeviceArray((2, o, v, o, v), dt)   # [W_ovvo term, (W_ovov + W_ovvo) term]
         w3 = (o * v) ** 2 * dt.itemsize
@@ -431,7 +432,7 @@ class TriplesEngine:
         tensor, so each element is split once per residual instead of once per tile that reads it.
         Y and Z are packed in place (they are rebuilt from T3 for every residual and read by nothing
         else); the others get packed copies.  Returns the (source, packed, count) passes to run."""
-        sources = [self.t3, self.Yz, self.W_vvvv, self.W1, self.W2, self.F_vv, self.W_ovvv, self.W_oovo, self.t2, self.WR1,
+        sources = [self.t3, self.Yz, self.W_vvvv, self.W1, self.F_vv, self.W_ovvv, self.W_oovo, self.t2, self.WR1,
                    self.WR3, self.pfs]
         packed = {}
 
@@ -481,11 +482,10 @@ class TriplesEngine:
         Rb = Operand(self.R, 'Babc', (n3, v, v, v))
         launches += self._plan_t2t3()
         # W_vvvv x T3 with the block index in N (rccsdt.py:424-426); the F_vv terms (rccsdt.py:400-402)
-        # ride along in W1 and W2 (see __init__)
+        # ride along in W1 (see __init__)
         one = [dict()]
-        for sub, tl, W in [('abde,Bdec->Babc', 'Bdec', self.W1), ('acde,Bdbe->Babc', 'Bdbe', self.W2),
-                           ('bcde,Bade->Babc', 'Bade', self.W_vvvv)]:
-            A = Operand(W, sub[:4], (v, v, v, v))
+        for sub, tl in [('abde,Bdec->Babc', 'Bdec'), ('acde,Bdbe->Babc', 'Bdbe'), ('bcde,Bade->Babc', 'Bade')]:
+            A = Operand(self.W1, sub[:4], (v, v, v, v))
             B = Operand(self.t3, tl, (n3, v, v, v))
             launches.append(plan_term('W_vvvv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
         launches += self._plan_ring()
@@ -735,19 +735,17 @@ class TriplesEngine:
             (o, o, v, v), -0.5, 1.0)
 
     def prepare_residual(self):
-        """Ring weights, W_oooo/F_oo prefactors and the F_vv-folded W1, W2 from the device intermediates."""
+        """Ring weights, W_oooo/F_oo prefactors and the F_vv-folded W1 from the device intermediates."""
         o, v = self.nocc, self.nvir
         s = self.stream.handle if self.stream is not None else None
         v2, v3 = v * v, v ** 3
-        for W in (self.W1, self.W2):
-            L.call("dcc_memcpy_dtod", W.vp, self.W_vvvv.vp, W.nbytes, s)
+        L.call("dcc_memcpy_dtod", self.W1.vp, self.W_vvvv.vp, self.W1.nbytes, s)
         F = self.F_vv
-        # W1[a,b,d,b] += F[a,d];  W1[a,b,a,e] += F[b,e];  W2[a,c,a,e] += F[c,e]
+        # W1[p,q,d,q] += 1/2 F[p,d];  W1[p,q,p,e] += 1/2 F[q,e]
         tensor_axpby(Operand.strided(self.W1, {'a': v3, 'b': v2 + 1, 'd': v}),
-                     Operand.strided(F, {'a': v, 'd': 1}), 'abd', (v, v, v), 1.0, 1.0, self.stream)
-        for W in (self.W1, self.W2):
-            tensor_axpby(Operand.strided(W, {'a': v3 + v, 'b': v2, 'e': 1}),
-                         Operand.strided(F, {'b': v, 'e': 1}), 'abe', (v, v, v), 1.0, 1.0, self.stream)
+                     Operand.strided(F, {'a': v, 'd': 1}), 'abd', (v, v, v), 0.5, 1.0, self.stream)
+        tensor_axpby(Operand.strided(self.W1, {'a': v3 + v, 'b': v2, 'e': 1}),
+                     Operand.strided(F, {'b': v, 'e': 1}), 'abe', (v, v, v), 0.5, 1.0, self.stream)
         L.call("dcc_ring_weights_" + self.sfx, self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
                self.WR3b.vp, s)
         L.call("dcc_oooo_prefactors_" + self.sfx, self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
