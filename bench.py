@@ -289,7 +289,8 @@ def batched_ensemble(nsys, steps=3, seed=20261202):
 
 def probe_residual(o, v, precision, tiles, seed=20261202):
     """One triples engine of the given precision on the synthetic system (nocc=o, nvir=v): tiles loaded
-    from ``tiles`` if it exists, else autotuned (and saved there when a path is given); the residual is
+    from ``tiles`` if it exists (a tile list of this size or a {launch name: tile} map), else autotuned
+    (and saved there by launch name when a path is given); the residual is
     run twice and the second one is timed, per launch.  Used to profile one precision's kernels."""
     from distr_cc.device import Event, Stream
     from distr_cc.synthetic import SyntheticSystem
@@ -302,11 +303,13 @@ def probe_residual(o, v, precision, tiles, seed=20261202):
     eng.set_t3(sysm.t3_tri().astype(dt))
     eng.set_intermediates(sysm.bare_intermediates(), sysm.t2())
     if tiles and os.path.exists(tiles):
-        eng.set_tile_config(json.load(open(tiles)))
+        cfg = json.load(open(tiles))
+        # a tile_config() list of this size, or a tiles_by_name() map (from any size)
+        eng.set_tiles_by_name(cfg) if isinstance(cfg, dict) else eng.set_tile_config(cfg)
     else:
         eng.autotune()
         if tiles:
-            json.dump(eng.tile_config(), open(tiles, "w"))
+            json.dump(eng.tiles_by_name(), open(tiles, "w"))
     eng.residual()
     ev = {}
 

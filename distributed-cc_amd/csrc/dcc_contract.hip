@@ -69,23 +69,25 @@ __device__ __forceinline__ void split_pair(uint32_t w0, uint32_t w1, uint32_t &h
     lo = __builtin_amdgcn_perm(w1, w0, 0x07060302u);
 }
 
-// Staging geometry of one operand with R rows (m for A, n for B) per tile, loaded in vectors of Q
-// elements (Q = 2: pairs, Q = 4: quads of packed words, one 16-byte load each): along k for a
-// K-contiguous operand (Q consecutive k of one row), along rows otherwise (Q adjacent rows at one k).
-template <int R, bool KFAST, int Q> struct Stage3 {
+// Staging geometry of one operand with R rows (m for A, n for B) per tile and NT threads, loaded in
+// vectors of Q elements (Q = 2: pairs, Q = 4: quads of packed words, one 16-byte load each): along k
+// for a K-contiguous operand (Q consecutive k of one row), along rows otherwise (Q adjacent rows at
+// one k).
+template <int R, bool KFAST, int Q, int NT> struct Stage3 {
     // K-contiguous planes are [row][32 k] with the four 16-byte chunks of a row XOR-swizzled by
     // (row >> 1) & 3: the 16-byte fragment reads of 8 consecutive rows and the vector writes of a
     // half-wave then touch distinct banks, without padding.
     static constexpr int PLANE = KFAST ? R * kBK3 : kBK3 * (R + 16);
     static constexpr int VPR = kBK3 / Q;                        // vectors per row (K-contiguous)
     static constexpr int RQ = R / Q;                            // row groups (row-contiguous)
-    static constexpr int NR = KFAST ? R * VPR / kThreads : Q;   // rows per thread
-    static constexpr int NK = KFAST ? Q : R * kBK3 / (kThreads * Q);  // k values per thread
+    static constexpr int NR = KFAST ? R * VPR / NT : Q;         // rows per thread
+    static constexpr int NK = KFAST ? Q : R * kBK3 / (NT * Q);  // k values per thread
+    static_assert(NR * NK * NT == R * kBK3, "the threads cover the tile exactly once");
     __device__ static int row(int tid, int r) {
-        return KFAST ? tid / VPR + (kThreads / VPR) * r : Q * (tid % RQ) + r;
+        return KFAST ? tid / VPR + (NT / VPR) * r : Q * (tid % RQ) + r;
     }
     __device__ static int kidx(int tid, int j) {
-        return KFAST ? Q * (tid % VPR) + j : tid / RQ + (kThreads / RQ) * j;
+        return KFAST ? Q * (tid % VPR) + j : tid / RQ + (NT / RQ) * j;
     }
     // 16-bit element index of (row, k) in a plane
     __device__ static int at(int row, int k) {
@@ -112,14 +114,18 @@ template <int R, bool KFAST, int Q> struct Stage3 {
 
 // MODE 0: FP32 operands loaded in pairs; 1: packed hi/lo words instead of FP32 values (the pointers
 // of p are reinterpreted) loaded in pairs; 2: packed words loaded in quads (16-byte loads).
-template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int MODE>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2)))
+// NW waves per workgroup: 4 as a 2 x 2 grid, or 8 (256 x 128 and 128 x 256 tiles, 64 x 64 per wave,
+// one workgroup per CU), which loads a quarter fewer operand bytes per flop than two 128 x 128 tiles.
+template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int MODE, int NW>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 void contract_bf16x3_kernel(const ContractArgs<float> p) {
     constexpr bool PACKED = MODE >= 1;
     constexpr int Q = MODE == 2 ? 4 : 2;
-    typedef Stage3<BM, A_KFAST, Q> SA;
-    typedef Stage3<BN, !B_NFAST, Q> SB;
-    constexpr int WM = BM / 2, WN = BN / 2;
+    constexpr int NT = NW * 64;
+    typedef Stage3<BM, A_KFAST, Q, NT> SA;
+    typedef Stage3<BN, !B_NFAST, Q, NT> SB;
+    constexpr int WGM = NW == 4 ? 2 : (BM >= BN ? 4 : 2), WGN = NW / WGM;   // wave grid
+    constexpr int WM = BM / WGM, WN = BN / WGN;
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int BUF = 2 * (SA::PLANE + SB::PLANE);            // one buffer: hi + lo planes, 16-bit units
     constexpr int MAIN_BYTES = 2 * BUF * 2;                      // two buffers
@@ -164,7 +170,7 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
     // K offsets of tile k0 into table slot t (-1 beyond K): lanes 0..15 of waves 0, 1 fill the A
     // entries, of waves 2, 3 the B entries
     auto fill_table = [&](int t, int k0) {
-        if (lane < 16) {
+        if (wave < 4 && lane < 16) {
             const int e = (wave & 1) * 16 + lane;
             const uint32_t k = (uint32_t)(k0 + e);
             const int64_t o = koff(wave < 2 ? p.ak : p.bk, k);
@@ -293,7 +299,7 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
         }
     };
 
-    const int wm = wave / 2, wn = wave % 2;
+    const int wm = wave / WGN, wn = wave % WGN;
     const int lrow = lane % 16, lk = lane / 16;
     flt4 acc[FM][FN];
 #pragma unroll
@@ -360,7 +366,7 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
                 for (int r = 0; r < 4; ++r)
                     Cs[(wn * WN + j * 16 + lrow) * (BM + 1) + wm * WM + i * 16 + 4 * lk + r] = acc[i][j][r];
         __syncthreads();
-        constexpr int NSTEP = kThreads / BM;
+        constexpr int NSTEP = NT / BM;
         const int ml = tid % BM;
         const int64_t m = m0 + ml;
         if (m >= m_end) return;
@@ -519,39 +525,39 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
 }
 
-template <int BM, int BN, bool C_MFAST, int MODE>
+template <int BM, int BN, bool C_MFAST, int MODE, int NW>
 void launch_bf16x3(ContractArgs<float> &args, bool a_kfast, bool b_nfast, dim3 grid, dim3 block,
                    hipStream_t stream) {
     if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, MODE>), grid, block, 0, stream,
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, MODE, NW>), grid, block, 0, stream,
                            args);
     else if (a_kfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, MODE>), grid, block, 0, stream,
-                           args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, MODE, NW>), grid, block, 0,
+                           stream, args);
     else if (b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, MODE>), grid, block, 0, stream,
-                           args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, MODE, NW>), grid, block, 0,
+                           stream, args);
     else
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, MODE>), grid, block, 0, stream,
-                           args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, MODE, NW>), grid, block, 0,
+                           stream, args);
 }
 
-template <int BM, int BN>
+template <int BM, int BN, int NW = 4>
 int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool a_kfast, bool b_nfast,
                bool c_mfast, int mode, hipStream_t s) {
     args.tiles_m = (int)((max_rows + BM - 1) / BM);
     args.tiles_n = (int)((args.N + BN - 1) / BN);
     args.total_tiles = nbatch * args.tiles_m * args.tiles_n;
     DCC_CHECK(args.total_tiles < ((int64_t)1 << 31), "too many tiles");
-    const dim3 grid((unsigned)args.total_tiles), block(kThreads);
+    const dim3 grid((unsigned)args.total_tiles), block(NW * 64);
     if (c_mfast) {
-        if (mode == 2) launch_bf16x3<BM, BN, true, 2>(args, a_kfast, b_nfast, grid, block, s);
-        else if (mode == 1) launch_bf16x3<BM, BN, true, 1>(args, a_kfast, b_nfast, grid, block, s);
-        else launch_bf16x3<BM, BN, true, 0>(args, a_kfast, b_nfast, grid, block, s);
+        if (mode == 2) launch_bf16x3<BM, BN, true, 2, NW>(args, a_kfast, b_nfast, grid, block, s);
+        else if (mode == 1) launch_bf16x3<BM, BN, true, 1, NW>(args, a_kfast, b_nfast, grid, block, s);
+        else launch_bf16x3<BM, BN, true, 0, NW>(args, a_kfast, b_nfast, grid, block, s);
     } else {
-        if (mode == 2) launch_bf16x3<BM, BN, false, 2>(args, a_kfast, b_nfast, grid, block, s);
-        else if (mode == 1) launch_bf16x3<BM, BN, false, 1>(args, a_kfast, b_nfast, grid, block, s);
-        else launch_bf16x3<BM, BN, false, 0>(args, a_kfast, b_nfast, grid, block, s);
+        if (mode == 2) launch_bf16x3<BM, BN, false, 2, NW>(args, a_kfast, b_nfast, grid, block, s);
+        else if (mode == 1) launch_bf16x3<BM, BN, false, 1, NW>(args, a_kfast, b_nfast, grid, block, s);
+        else launch_bf16x3<BM, BN, false, 0, NW>(args, a_kfast, b_nfast, grid, block, s);
     }
     DCC_LAUNCH_CHECK();
     return 0;
@@ -568,7 +574,7 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
                   valid_map(cn),
               "invalid index map");
     DCC_CHECK(ak.table == nullptr && bk.table == nullptr, "K index maps must be mixed-radix maps");
-    DCC_CHECK(tile_shape >= -1 && tile_shape <= 3, "tile shape must be -1..3");
+    DCC_CHECK(tile_shape >= -1 && tile_shape <= 5, "tile shape must be -1..5");
     if (M == 0 || N == 0 || nbatch == 0 || max_rows == 0) return 0;
     DCC_CHECK(A != nullptr && B != nullptr && C != nullptr && batches != nullptr, "null pointer");
     ContractArgs<float> args;
@@ -605,6 +611,8 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
         case 0: return run_bf16x3<64, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         case 1: return run_bf16x3<128, 64>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         case 2: return run_bf16x3<64, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
+        case 4: return run_bf16x3<256, 128, 8>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
+        case 5: return run_bf16x3<128, 256, 8>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         default: return run_bf16x3<128, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
     }
 }

@@ -432,21 +432,24 @@ class TriplesEngine:
         tensor, so each element is split once per residual instead of once per tile that reads it.
         Y and Z are packed in place (they are rebuilt from T3 for every residual and read by nothing
         else); the others get packed copies.  Returns the (source, packed, count) passes to run."""
-        sources = [self.t3, self.Yz, self.W_vvvv, self.W1, self.F_vv, self.W_ovvv, self.W_oovo, self.t2, self.WR1,
-                   self.WR3, self.pfs]
+        sources = [self.t3, self.Yz, self.W1, self.W_ovvv, self.W_oovo, self.t2, self.WR1, self.WR3, self.pfs]
         packed = {}
+        # Y/Z are packed in place unless a GEMM kept on FP32 (K <= 16, tiny systems) reads them
+        yz = self.Yz.ptr, self.Yz.ptr + self.Yz.nbytes
+        yz_in_place = not any(yz[0] <= g.args[i].value < yz[1] for g in self.launches
+                              if isinstance(g, GemmLaunch) and g.fn != "dcc_contract_f32_bf16x3" for i in (5, 8))
 
         def find(ptr):
             for src in sources:
                 if src.ptr <= ptr < src.ptr + src.nbytes:
                     if id(src) not in packed:
-                        dst = src if src is self.Yz else DeviceArray(src.shape, np.uint32)
+                        dst = src if (src is self.Yz and yz_in_place) else DeviceArray(src.shape, np.uint32)
                         packed[id(src)] = (src, dst)
                     return packed[id(src)]
             return None
 
         for g in self.launches:
-            if not isinstance(g, GemmLaunch):
+            if not isinstance(g, GemmLaunch) or g.fn != "dcc_contract_f32_bf16x3":
                 continue
             a, b = find(g.args[5].value), find(g.args[8].value)
             if a is None or b is None:
@@ -465,9 +468,12 @@ class TriplesEngine:
             L.call("dcc_pack_bf16x3", src.vp, dst.vp, n, s)
 
     def _retarget(self, launches):
+        """Split-BF16x3 engine: GEMMs with K > 16 run as split-BF16x3 products; those with K <= 16 (the
+        occupied-index T2->T3 term: an HBM-bound rank-o update, whose k-step of 32 would be half
+        padding) stay on FP32 MFMA, which is also the more accurate of the two."""
         if self.contract == "bf16x3":
             for g in launches:
-                if isinstance(g, GemmLaunch):
+                if isinstance(g, GemmLaunch) and g.args[2] > 16:
                     g.fn = "dcc_contract_f32_bf16x3"
 
     # ------------------------------------------------------------------ plan
@@ -907,19 +913,27 @@ class TriplesEngine:
         return {g.name: g.tile for g in self._tunable() if isinstance(g, GemmLaunch)}
 
     def set_tiles_by_name(self, tiles):
-        """Apply a tiles_by_name() map; launches without an entry keep their tile."""
+        """Apply a tiles_by_name() map; launches without an entry keep their tile.  A split-BF16x3
+        tile with a 256-wide side (4: 256x128, 5: 128x256) becomes 128x128 on a launch whose extent on
+        that side is at most 128 here (a chunked launch of a larger system), where it would be mostly
+        padding."""
         for g in self._tunable():
             if isinstance(g, GemmLaunch) and g.name in tiles:
-                g.tile = int(tiles[g.name])
+                t = int(tiles[g.name])
+                if g.fn.endswith("bf16x3") and ((t == 4 and g.max_rows <= 128) or (t == 5 and g.args[1] <= 128)):
+                    t = 3
+                g.tile = t
 
     def autotune(self, reps=2, include_lower=True):
         """Time the workgroup tiles of every contraction launch and keep the fastest: eight for the
-        FP64/FP32 kernels (four shapes x one- or two-stage prefetch), four shapes for split-BF16x3.
+        FP64/FP32 kernels (four shapes x one- or two-stage prefetch), six shapes for split-BF16x3
+        (the four of 256 threads and 256x128, 128x256 of 512 threads).
 
         Run it before a residual (it accumulates into R and the scratch buffers); returns
         {launch name: (tile, ms)}."""
         from distr_cc.device import Event
         shapes = ("64x64", "128x64", "64x128", "128x128", "64x64s1", "128x64s1", "64x128s1", "128x128s1")
+        shapes3 = ("64x64", "128x64", "64x128", "128x128", "256x128", "128x256")
         launches = self._tunable(include_lower)
         out = {}
         st = self.stream
@@ -927,7 +941,8 @@ class TriplesEngine:
             if not isinstance(g, GemmLaunch):
                 continue
             best = None
-            for tile in range(4 if g.fn.endswith("bf16x3") else len(shapes)):
+            names = shapes3 if g.fn.endswith("bf16x3") else shapes
+            for tile in range(len(names)):
                 g.tile = tile
                 g.run(st)
                 a, b = Event(), Event()
@@ -939,7 +954,7 @@ class TriplesEngine:
                 if best is None or ms < best[1]:
                     best = (tile, ms)
             g.tile = best[0]
-            out["%d %s" % (n, g.name)] = (shapes[best[0]], best[1])
+            out["%d %s" % (n, g.name)] = (names[best[0]], best[1])
         return out
 
     def residual_flops(self):

@@ -237,13 +237,28 @@ def test_bf16x3_residual_plan(monkeypatch):
     s = SyntheticSystem(o, v, naux=20, seed=31)
     im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
     eng = TriplesEngine(o, v, dtype=np.float32, contract="bf16x3")
-    # every residual GEMM reads packed (pre-split) operands
-    assert all(g.fn == "dcc_contract_packed_bf16x3" for g in eng.launches if hasattr(g, "args"))
+    # every residual GEMM with K > 16 reads packed (pre-split) operands, the others stay FP32
+    gemms = [g for g in eng.launches if hasattr(g, "args")]
+    assert all(g.fn == ("dcc_contract_packed_bf16x3" if g.args[2] > 16 else "dcc_contract_f32") for g in gemms)
+    assert any(g.fn == "dcc_contract_f32" for g in gemms) and any(g.args[2] > 16 for g in gemms)
     eng.set_t3(t3)
     eng.set_intermediates(im, t2)
     eng.residual()
     Ro = tri.compute_r3_tri(o, v, im, t2, t3)
     assert abs(eng.R.download().astype(np.float64) - Ro).max() <= 3e-4 * abs(Ro).max()
+
+
+def test_bf16x3_tile_transfer(monkeypatch):
+    """Tiles transferred by launch name: a 256-wide split-BF16x3 tile falls back to 128x128 on a
+    launch whose extent on that side is at most 128; other tiles are applied as given."""
+    abi_emulator.install(monkeypatch)
+    from distr_cc.t3_engine import GemmLaunch, TriplesEngine
+    eng = TriplesEngine(3, 5, dtype=np.float32, contract="bf16x3")
+    gemms = [g for g in eng.launches if isinstance(g, GemmLaunch) and g.fn.endswith("bf16x3")]
+    assert gemms and all(g.max_rows <= 128 and g.args[1] <= 128 for g in gemms)
+    for t, want in ((4, 3), (5, 3), (2, 2)):
+        eng.set_tiles_by_name({g.name: t for g in gemms})
+        assert all(g.tile == want for g in gemms)
 
 
 def test_batched_ensemble_plan(monkeypatch):
