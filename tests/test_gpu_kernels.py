@@ -76,40 +76,47 @@ def test_contract_transposed_operands(dev, M, N, K):
         assert abs(dC.download() - ref).max() < 1e-12 * abs(ref).max(), tile
 
 
-@pytest.mark.parametrize("v", [1, 5, 8, 13, 20])
-def test_s3_transform_all_perms(dev, v):
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("v", [1, 5, 8, 13, 20, 33])
+def test_s3_transform_all_perms(dev, v, dt):
+    """dcc_s3_transform_*: every single permutation and a mixed combination, FP64 and FP32 storage
+    (partial 8^3 tiles and several tile orbits)."""
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
     from distr_cc.t3_engine import s3_jobs_array
-    _say("s3 transform v=%d" % v)
+    _say("s3 transform v=%d %s" % (v, dt))
     rng = np.random.default_rng(v)
-    X = rng.normal(size=(3, v, v, v))
+    typ = np.float64 if dt == "f64" else np.float32
+    X = rng.normal(size=(3, v, v, v)).astype(typ)
     perms = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
     coefs = [np.eye(6)[p] for p in range(6)] + [np.array([2.0, -1.0, -1.0, 0.5, 0.25, -3.0])]
     jobs = []
     for n, c in enumerate(coefs):
         jobs.append((n * v ** 3, (n % 3) * v ** 3, c))
     dX = DeviceArray.from_host(X)
-    dO = DeviceArray.zeros((len(coefs), v, v, v))
+    dO = DeviceArray.zeros((len(coefs), v, v, v), typ)
     dj = DeviceArray.from_host(s3_jobs_array(jobs))
-    L.call("dcc_s3_transform_f64", dO.vp, dX.vp, v, 0.0, dj.vp, len(jobs), None)
-    out = dO.download()
+    L.call("dcc_s3_transform_" + dt, dO.vp, dX.vp, v, 0.0, dj.vp, len(jobs), None)
+    out = dO.download().astype(np.float64)
     for n, c in enumerate(coefs):
-        src = X[n % 3]
+        src = X[n % 3].astype(np.float64)
         ref = sum(ci * src.transpose(p) for ci, p in zip(c, perms))
-        assert abs(out[n] - ref).max() < 1e-13, (n, v)
+        # FP64 exact up to the sum's rounding; FP32: one rounding of the FP64 sum to float
+        assert abs(out[n] - ref).max() < (1e-13 if dt == "f64" else 1e-6 * max(1.0, abs(ref).max())), (n, v)
 
 
-@pytest.mark.parametrize("v", [1, 5, 8, 13, 20])
-def test_s3_gather_multi_source(dev, v):
-    """dcc_s3_gather_f64: a different source block per permutation, repeated sources, zero
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+@pytest.mark.parametrize("v", [1, 5, 8, 13, 20, 33])
+def test_s3_gather_multi_source(dev, v, dt):
+    """dcc_s3_gather_*: a different source block per permutation, repeated sources, zero
     coefficients and beta accumulation, against NumPy."""
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
-    _say("s3 gather v=%d" % v)
+    _say("s3 gather v=%d %s" % (v, dt))
     rng = np.random.default_rng(40 + v)
-    X = rng.normal(size=(5, v, v, v))
-    O0 = rng.normal(size=(3, v, v, v))
+    typ = np.float64 if dt == "f64" else np.float32
+    X = rng.normal(size=(5, v, v, v)).astype(typ)
+    O0 = rng.normal(size=(3, v, v, v)).astype(typ)
     perms = ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0))
     v3 = v ** 3
     jobs = [(0, [0, 1, 2, 3, 4, 0], np.array([1.0, -1.0, 0.5, 2.0, 0.0, 3.0])),
@@ -119,12 +126,14 @@ def test_s3_gather_multi_source(dev, v):
     for n, (dst, srcs, coef) in enumerate(jobs):
         arr[n]['dst'], arr[n]['src'], arr[n]['coef'] = dst, [x * v3 for x in srcs], coef
     dX, dO, dj = DeviceArray.from_host(X), DeviceArray.from_host(O0), DeviceArray.from_host(arr)
-    L.call("dcc_s3_gather_f64", dO.vp, dX.vp, v, 0.5, dj.vp, len(jobs), None)
-    out = dO.download()
+    L.call("dcc_s3_gather_" + dt, dO.vp, dX.vp, v, 0.5, dj.vp, len(jobs), None)
+    out = dO.download().astype(np.float64)
+    tol = 1e-13 if dt == "f64" else 1e-6
     for dst, srcs, coef in jobs:
         q = dst // v3
-        ref = 0.5 * O0[q] + sum(c * X[sx].transpose(p) for c, sx, p in zip(coef, srcs, perms) if c != 0.0)
-        assert abs(out[q] - ref).max() < 1e-13 * max(1.0, abs(ref).max()), (q, v)
+        ref = 0.5 * O0[q].astype(np.float64) + sum(c * X[sx].astype(np.float64).transpose(p)
+                                                     for c, sx, p in zip(coef, srcs, perms) if c != 0.0)
+        assert abs(out[q] - ref).max() < tol * max(1.0, abs(ref).max()), (q, v)
 
 
 @pytest.mark.parametrize("o,v", [(1, 3), (3, 4), (4, 9), (5, 17), (3, 24)])
