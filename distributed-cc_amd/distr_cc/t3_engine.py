@@ -388,8 +388,14 @@ class TriplesEngine:
         self.WR3a = DeviceArray((o, v, o, v), dt, ptr=self.WR3.ptr, owner=False)
         self.WR3b = DeviceArray((o, v, o, v), dt, ptr=self.WR3.ptr + w3, owner=False)
         self.pfs = DeviceArray((6, n3, n3), dt)
-        # W_oooo/F_oo terms s = 1..5 before their permutation, for a chunk of target blocks
-        qc = int(max(1, min(n3, (ring_g_bytes // max(1, 5 * v3 * dt.itemsize)))))
+        # W_oooo/F_oo terms s = 1..5 before their permutation, for a chunk of target blocks (the GEMMs'
+        # M); a chunk is rounded up to a multiple of 64 rows when that stays within 5/4 of the bound,
+        # so the GEMM tiles are not partly empty (e.g. 52 -> 64 at nocc=16, nvir=160 in FP64)
+        per_target = max(1, 5 * v3 * dt.itemsize)
+        qc = ring_g_bytes // per_target
+        if qc >= 32 and -(-qc // 64) * 64 * per_target * 4 <= 5 * ring_g_bytes:
+            qc = -(-qc // 64) * 64
+        qc = int(max(1, min(n3, qc)))
         self.Xp = DeviceArray((5, qc, v, v, v), dt)
         self.eia = DeviceArray((o, v))
         self.norb = int(L._lib.dcc_r3_finalize_tiles(v))

@@ -50,6 +50,26 @@ def test_engine_ring_chunked(monkeypatch, o, v, pairs, fused):
     assert abs(eng.R.download() - Ro).max() <= 1e-12 * abs(Ro).max()
 
 
+def test_engine_oooo_chunk_rounding(monkeypatch):
+    """W_oooo/F_oo target chunks: a bound of 52 targets is rounded up to one 64-row chunk (within
+    5/4 of the bound), leaving a partial second chunk of 20 of the 84 targets."""
+    abi_emulator.install(monkeypatch)
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    o, v = 7, 3
+    s = SyntheticSystem(o, v, naux=20, seed=23)
+    rng = np.random.default_rng(5)
+    im = {k: x + 1e-3 * rng.normal(size=x.shape) for k, x in s.bare_intermediates().items()}
+    t2, t3 = s.t2(), s.t3_tri()
+    eng = TriplesEngine(o, v, ring_g_bytes=52 * 5 * v ** 3 * 8)
+    assert eng.n3 == 84 and eng.Xp.shape[1] == 64
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.residual()
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3, batch_size=11)
+    assert abs(eng.R.download() - Ro).max() <= 1e-12 * abs(Ro).max()
+
+
 @pytest.mark.parametrize("o,v", [(1, 3), (3, 4), (4, 3), (2, 5)])
 def test_engine_t3_to_r1r2_and_w_plan(monkeypatch, o, v):
     _t3_to_r1r2_and_w(monkeypatch, o, v, None)
