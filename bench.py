@@ -275,6 +275,19 @@ def batched_ensemble(nsys, steps=3, seed=20261202):
     b.record(st)
     ms = a.elapsed_ms(b) / steps
     norms = bt.norms()
+    # parity at this scale: one replay from the inputs against every system's own launches (one GEMM
+    # per term and system), compared through the update norm ||r3||^2 of each system
+    for s, inp in enumerate(inputs):
+        bt.engines[s].set_t3(inp[0])
+    bt.step()
+    graph_norms = bt.norms()
+    rel = []
+    for s, inp in enumerate(inputs):
+        e = bt.engines[s]
+        e.set_t3(inp[0])
+        e.residual()
+        own = e.finalize(inp[3])[0]
+        rel.append(abs(graph_norms[s][0] - own) / abs(own))
     shapes = {}
     for k, (shape, _) in tuned.items():
         shapes[shape] = shapes.get(shape, 0) + 1
@@ -282,7 +295,8 @@ def batched_ensemble(nsys, steps=3, seed=20261202):
                        "per step, grouped persistent GEMMs in one HIP graph" % nsys,
            "ms_per_step": ms, "system_updates_per_s": nsys / (ms * 1e-3),
            "contraction_tflops": bt.flops() / (ms * 1e-3) / 1e12,
-           "autotuned_tile_shapes": shapes, "finite_norms": bool(np.all(np.isfinite(norms)))}
+           "autotuned_tile_shapes": shapes, "finite_norms": bool(np.all(np.isfinite(norms))),
+           "max_rel_norm_diff_vs_per_system_launches": float(max(rel))}
     bt.close()
     return out
 

@@ -103,8 +103,10 @@ def test_autograd_function(monkeypatch):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("o,v,ic", [(1, 4, None), (3, 7, None), (6, 13, None), (6, 13, 4)])
+@pytest.mark.parametrize("o,v,ic", [(1, 4, None), (3, 7, None), (6, 13, None), (6, 13, 4), (6, 40, None), (6, 40, 4)])
 def test_adjoint_gpu(o, v, ic):
+    """Forward against the oracle, VJP against the FP64 torch reference, central differences through the
+    oracle; nocc=6, nvir=40 is the adjoint's reference-scale case (one and two scratch chunks)."""
     from distr_cc import device
     device.init(0)
     print("adjoint o=%d v=%d chunk=%s" % (o, v, ic), flush=True)
