@@ -105,14 +105,16 @@ def test_s3_transform_all_perms(dev, v, dt):
         assert abs(out[n] - ref).max() < (1e-13 if dt == "f64" else 1e-6 * max(1.0, abs(ref).max())), (n, v)
 
 
+@pytest.mark.parametrize("shift", [0, 1])
 @pytest.mark.parametrize("dt", ["f64", "f32"])
 @pytest.mark.parametrize("v", [1, 5, 8, 13, 20, 33])
-def test_s3_gather_multi_source(dev, v, dt):
+def test_s3_gather_multi_source(dev, v, dt, shift):
     """dcc_s3_gather_*: a different source block per permutation, repeated sources, zero
-    coefficients and beta accumulation, against NumPy."""
+    coefficients and beta accumulation, against NumPy; ``shift`` = 1 moves every block one element
+    off alignment."""
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
-    _say("s3 gather v=%d %s" % (v, dt))
+    _say("s3 gather v=%d %s shift %d" % (v, dt, shift))
     rng = np.random.default_rng(40 + v)
     typ = np.float64 if dt == "f64" else np.float32
     X = rng.normal(size=(5, v, v, v)).astype(typ)
@@ -124,10 +126,15 @@ def test_s3_gather_multi_source(dev, v, dt):
             (1 * v3, [2, 0, 0, 0, 0, 1], np.array([1.5, 0.0, 0.0, 0.0, 0.0, -2.0]))]
     arr = np.zeros(len(jobs), dtype=L.S3GATHER_DTYPE)
     for n, (dst, srcs, coef) in enumerate(jobs):
-        arr[n]['dst'], arr[n]['src'], arr[n]['coef'] = dst, [x * v3 for x in srcs], coef
-    dX, dO, dj = DeviceArray.from_host(X), DeviceArray.from_host(O0), DeviceArray.from_host(arr)
+        arr[n]['dst'], arr[n]['src'], arr[n]['coef'] = dst + shift, [x * v3 + shift for x in srcs], coef
+    pad = np.zeros(shift, typ)
+    dX = DeviceArray.from_host(np.concatenate([pad, X.ravel()]))
+    dO = DeviceArray.from_host(np.concatenate([pad, O0.ravel()]))
+    dj = DeviceArray.from_host(arr)
     L.call("dcc_s3_gather_" + dt, dO.vp, dX.vp, v, 0.5, dj.vp, len(jobs), None)
-    out = dO.download().astype(np.float64)
+    outf = dO.download()
+    assert np.all(outf[:shift] == 0)
+    out = outf[shift:].reshape(O0.shape).astype(np.float64)
     tol = 1e-13 if dt == "f64" else 1e-6
     for dst, srcs, coef in jobs:
         q = dst // v3
