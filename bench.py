@@ -513,6 +513,18 @@ def main():
         print("  T3->r1/r2 with the T3->W terms (A2, A3 contractions) %.3f ms, adding the T3->W terms %.3f ms"
               % (np.mean(a2), np.mean(a3)), file=sys.stderr)
 
+    # the HBM-bound fused projection/denominator/update (A7-A9): algorithmic bytes 3 S (read R, read and
+    # write t3), S = n3 v^3 8 bytes; timed on its own after the steps (it keeps updating t3)
+    fin = []
+    for _ in range(5):
+        a, b = Event(), Event()
+        a.record(stream)
+        eng.finalize_launch()
+        b.record(stream)
+        fin.append((a, b))
+    stream.synchronize()
+    fin_ms = float(np.mean([a.elapsed_ms(b) for a, b in fin[1:]]))
+    fin_bytes = 3 * eng.n3 * v ** 3 * 8
     F = algorithmic_flops(o, v)
     out = {
         "metric": METRIC,
@@ -534,6 +546,9 @@ def main():
                    "residual_tflops": F / (ms * 1e-3) / 1e12,
                    "residual_frac_fp64_peak": F / (ms * 1e-3) / 1e12 / PEAK_FP64_MFMA_TFLOPS,
                    "r3_norm_sq": norms[0], "wall_s": wall,
+                   "r3_finalize": {"ms": fin_ms, "algorithmic_bytes": fin_bytes,
+                                   "GB_per_s": fin_bytes / (fin_ms * 1e-3) / 1e9,
+                                   "frac_hbm_peak": fin_bytes / (fin_ms * 1e-3) / 1e9 / PEAK_HBM_GBS},
                    "autotuned_tiles": None if tuned is None else
                    {k: v[0] for k, v in tuned.items() if k.split(" ", 1)[1].startswith(("W_vvvv", "ring"))}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_MFMA_TFLOPS, "unit": "TFLOP/s",
