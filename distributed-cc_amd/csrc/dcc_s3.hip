@@ -9,6 +9,7 @@
 // permuted access is then an LDS access, and the six output tiles are written
 // along their contiguous axis.  Each element is read and written exactly once.
 #include "dcc_common.h"
+#include "dcc_gemm_tile.h"   // xcd_remap
 
 namespace {
 
@@ -37,6 +38,30 @@ __device__ __forceinline__ void decode_orbit(int o, int &A, int &B, int &C) {
 }
 
 __host__ __device__ __forceinline__ int64_t n_orbits(int64_t nt) { return nt * (nt + 1) * (nt + 2) / 6; }
+
+// Workgroup b of a launch over njobs blocks -> (job, orbit index, tile triple R).  A tile row is 64
+// bytes of doubles (32 of floats), so a 128-byte line is shared with tiles whose coordinate along the
+// contiguous axis differs in the low bits; those belong to orbits that differ from this one in the
+// same bits of one tile coordinate.  Workgroups therefore walk 2x2x2 blocks of tile coordinates (nso
+// block orbits of 8 slots; slots with A >= B >= C violated or A >= nt exit at once), and consecutive
+// slots run on one XCD (xcd_remap), so the parts of a line are read while it is in that XCD's L2.
+// In plain orbit order the FP32 placement passes ran 1.3x longer at nocc=16, nvir=160 (4x4x4 blocks
+// for floats measured no better than 2x2x2).
+__device__ __forceinline__ bool orbit_of(int64_t b, int64_t total, int nt, int nso, int64_t &job, int &orbit,
+                                         int R[3]) {
+    const int64_t L = dcc_gemm::xcd_remap(b, total);
+    job = L / (8 * (int64_t)nso);
+    const int r = (int)(L - job * 8 * (int64_t)nso);
+    int SA, SB, SC;
+    decode_orbit(r >> 3, SA, SB, SC);
+    const int A = 2 * SA + ((r >> 2) & 1), B = 2 * SB + ((r >> 1) & 1), C = 2 * SC + (r & 1);
+    if (!(A >= B && B >= C && A < nt)) return false;
+    R[0] = A;
+    R[1] = B;
+    R[2] = C;
+    orbit = A * (A + 1) * (A + 2) / 6 + B * (B + 1) / 2 + C;
+    return true;
+}
 
 // Load the six permuted tiles of orbit (R0,R1,R2) of block `src` into LDS.
 template <typename T>
@@ -81,12 +106,12 @@ __device__ __forceinline__ double permuted(const T (*tiles)[TE], int s, int p, i
 template <typename T>
 __global__ __launch_bounds__(kThreads) void s3_transform_kernel(T *__restrict__ out, const T *__restrict__ src,
                                                                 int nvir, double beta,
-                                                                const dcc_s3_job *__restrict__ jobs, int norb) {
+                                                                const dcc_s3_job *__restrict__ jobs, int nt,
+                                                                int nso) {
     __shared__ T tiles[6][TE];
-    const int64_t job = blockIdx.x / norb;
-    const int orbit = blockIdx.x - (int)(job * norb);
-    int R[3];
-    decode_orbit(orbit, R[0], R[1], R[2]);
+    int64_t job;
+    int orbit, R[3];
+    if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
     const dcc_s3_job jb = jobs[job];
     load_orbit(tiles, src + jb.src, nvir, R);
     __syncthreads();
@@ -119,12 +144,11 @@ template <typename T>
 __global__ __launch_bounds__(kThreads) void s3_gather_kernel(T *__restrict__ out, const T *__restrict__ src,
                                                              int nvir, double beta,
                                                              const dcc_s3_gather_job *__restrict__ jobs,
-                                                             int norb) {
+                                                             int nt, int nso) {
     __shared__ T tiles[6][TE];
-    const int64_t job = blockIdx.x / norb;
-    const int orbit = blockIdx.x - (int)(job * norb);
-    int R[3];
-    decode_orbit(orbit, R[0], R[1], R[2]);
+    int64_t job;
+    int orbit, R[3];
+    if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
     const dcc_s3_gather_job jb = jobs[job];
     double acc[6][kPer];
 #pragma unroll
@@ -171,13 +195,13 @@ __global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const T *__restri
                                                                T *__restrict__ r3_out, int nvir,
                                                                const int *__restrict__ blk_ijk,
                                                                const double *__restrict__ eia, int nvir_diis,
-                                                               double *__restrict__ partials, int norb) {
+                                                               double *__restrict__ partials, int nt, int nso) {
     __shared__ T tiles[6][TE];
     __shared__ double red[2][kThreads / 64];
-    const int64_t blk = blockIdx.x / norb;
-    const int orbit = blockIdx.x - (int)(blk * norb);
-    int Rg[3];
-    decode_orbit(orbit, Rg[0], Rg[1], Rg[2]);
+    int64_t blk;
+    int orbit, Rg[3];
+    if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, blk, orbit, Rg)) return;
+    const int64_t norb = n_orbits(nt);
     const int64_t v = nvir, v2 = v * v, v3 = v2 * v;
     const T *Rb = R + blk * v3;
     load_orbit(tiles, Rb, nvir, Rg);
@@ -223,8 +247,8 @@ __global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const T *__restri
             a += red[0][w];
             b += red[1][w];
         }
-        partials[2 * blockIdx.x] = a;
-        partials[2 * blockIdx.x + 1] = b;
+        partials[2 * (blk * norb + orbit)] = a;
+        partials[2 * (blk * norb + orbit) + 1] = b;
     }
 }
 
@@ -262,11 +286,10 @@ int s3_gather(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_gath
     DCC_CHECK(nvir >= 0 && njobs >= 0, "negative extent");
     if (nvir == 0 || njobs == 0) return 0;
     DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
-    const int64_t nt = (nvir + E - 1) / E;
-    const int64_t norb = n_orbits(nt);
-    DCC_CHECK(norb * njobs < ((int64_t)1 << 31), "grid too large");
-    hipLaunchKernelGGL(s3_gather_kernel<T>, dim3((unsigned)(norb * njobs)), dim3(kThreads), 0,
-                       dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)norb);
+    const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
+    DCC_CHECK(8 * nso * njobs < ((int64_t)1 << 31), "grid too large");
+    hipLaunchKernelGGL(s3_gather_kernel<T>, dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+                       dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -277,11 +300,10 @@ int s3_transform(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_j
     DCC_CHECK(nvir >= 0 && njobs >= 0, "negative extent");
     if (nvir == 0 || njobs == 0) return 0;
     DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
-    const int64_t nt = (nvir + E - 1) / E;
-    const int64_t norb = n_orbits(nt);
-    DCC_CHECK(norb * njobs < ((int64_t)1 << 31), "grid too large");
-    hipLaunchKernelGGL(s3_transform_kernel<T>, dim3((unsigned)(norb * njobs)), dim3(kThreads), 0,
-                       dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)norb);
+    const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
+    DCC_CHECK(8 * nso * njobs < ((int64_t)1 << 31), "grid too large");
+    hipLaunchKernelGGL(s3_transform_kernel<T>, dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+                       dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -293,13 +315,12 @@ int r3_finalize(const T *R, T *t3, T *r3_out, int64_t nblk, int64_t nvir, const 
     if (nblk == 0 || nvir == 0) return 0;
     DCC_CHECK(R != nullptr && t3 != nullptr && blk_ijk != nullptr && eia != nullptr && partials != nullptr,
               "null pointer");
-    const int64_t nt = (nvir + E - 1) / E;
-    const int64_t norb = n_orbits(nt);
-    DCC_CHECK(norb * nblk < ((int64_t)1 << 31), "grid too large");
+    const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
+    DCC_CHECK(8 * nso * nblk < ((int64_t)1 << 31), "grid too large");
     if (nvir_diis < 0 || nvir_diis > nvir) nvir_diis = nvir;
-    hipLaunchKernelGGL(r3_finalize_kernel<T>, dim3((unsigned)(norb * nblk)), dim3(kThreads), 0,
+    hipLaunchKernelGGL(r3_finalize_kernel<T>, dim3((unsigned)(8 * nso * nblk)), dim3(kThreads), 0,
                        dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis, partials,
-                       (int)norb);
+                       (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
 }
