@@ -63,21 +63,39 @@ __device__ __forceinline__ bool orbit_of(int64_t b, int64_t total, int nt, int n
     return true;
 }
 
-// Load the six permuted tiles of orbit (R0,R1,R2) of block `src` into LDS.
+// Element h of this thread in orbit tile s: block coordinates g[3], offset in the v^3 block and
+// whether it lies inside the block.  Elements outside (partial edge tiles) get offset 0, the block's
+// first element, so that loads can be issued unconditionally: all of a thread's loads are then in
+// flight together (with a branch per element the compiler waited for each load before the next).
+__device__ __forceinline__ bool tile_elem(int nvir, const int R[3], int s, int h, int g[3], int64_t &off) {
+    const int e = threadIdx.x + h * kThreads;
+    g[0] = R[kP[s][0]] * E + (e >> 6);
+    g[1] = R[kP[s][1]] * E + ((e >> 3) & 7);
+    g[2] = R[kP[s][2]] * E + (e & 7);
+    const bool in = g[0] < nvir && g[1] < nvir && g[2] < nvir;
+    off = in ? ((int64_t)g[0] * nvir + g[1]) * nvir + g[2] : 0;
+    return in;
+}
+
+// Load the six permuted tiles of orbit (R0,R1,R2) of block `src` into LDS: all loads first, then
+// the LDS stores.
 template <typename T>
 __device__ __forceinline__ void load_orbit(T (*tiles)[TE], const T *__restrict__ src, int nvir, const int R[3]) {
-    const int64_t v = nvir, v2 = v * v;
-    for (int s = 0; s < 6; ++s) {
-        const int t0 = R[kP[s][0]], t1 = R[kP[s][1]], t2 = R[kP[s][2]];
+    T val[6][kPer];
+    bool in[6][kPer];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
 #pragma unroll
         for (int h = 0; h < kPer; ++h) {
-            const int e = threadIdx.x + h * kThreads;
-            const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
-            T val = T(0);
-            if (g0 < nvir && g1 < nvir && g2 < nvir) val = src[g0 * v2 + g1 * v + g2];
-            tiles[s][e] = val;
+            int g[3];
+            int64_t off;
+            in[s][h] = tile_elem(nvir, R, s, h, g, off);
+            val[s][h] = src[off];
         }
-    }
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) tiles[s][threadIdx.x + h * kThreads] = in[s][h] ? val[s][h] : T(0);
 }
 
 // Is output tile s a repeat of an earlier tile of the same orbit?
@@ -86,6 +104,21 @@ __device__ __forceinline__ bool duplicate_tile(int s, const int R[3]) {
     for (int u = 0; u < s; ++u)
         if (R[kP[u][0]] == a && R[kP[u][1]] == b && R[kP[u][2]] == c) return true;
     return false;
+}
+
+// This thread's current values of the six output tiles of orbit R in block dst (loads issued
+// together; elements outside the block read the block's first element and are never stored).
+template <typename T>
+__device__ __forceinline__ void prefetch_out(const T *dst, int nvir, const int R[3], T old[6][kPer]) {
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) {
+            int g[3];
+            int64_t off;
+            tile_elem(nvir, R, s, h, g, off);
+            old[s][h] = dst[off];
+        }
 }
 
 // Value of X.transpose(p)[x] for x = local element e of output tile s.
@@ -113,25 +146,25 @@ __global__ __launch_bounds__(kThreads) void s3_transform_kernel(T *__restrict__ 
     int orbit, R[3];
     if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
     const dcc_s3_job jb = jobs[job];
+    T *dst = out + jb.dst;
+    T old[6][kPer];
+    if (beta != 0.0) prefetch_out(dst, nvir, R, old);
     load_orbit(tiles, src + jb.src, nvir, R);
     __syncthreads();
-    const int64_t v = nvir, v2 = v * v;
-    T *dst = out + jb.dst;
+#pragma unroll
     for (int s = 0; s < 6; ++s) {
         if (duplicate_tile(s, R)) continue;
-        const int t0 = R[kP[s][0]], t1 = R[kP[s][1]], t2 = R[kP[s][2]];
 #pragma unroll
         for (int h = 0; h < kPer; ++h) {
-            const int e = threadIdx.x + h * kThreads;
-            const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
-            if (g0 >= nvir || g1 >= nvir || g2 >= nvir) continue;
+            int g[3];
+            int64_t off;
+            if (!tile_elem(nvir, R, s, h, g, off)) continue;
             double val = 0.0;
 #pragma unroll
             for (int p = 0; p < 6; ++p)
-                if (jb.coef[p] != 0.0) val += jb.coef[p] * permuted(tiles, s, p, e);
-            T *o = dst + g0 * v2 + g1 * v + g2;
-            if (beta != 0.0) val += beta * (double)(*o);
-            *o = (T)val;
+                if (jb.coef[p] != 0.0) val += jb.coef[p] * permuted(tiles, s, p, threadIdx.x + h * kThreads);
+            if (beta != 0.0) val += beta * (double)old[s][h];
+            dst[off] = (T)val;
         }
     }
 }
@@ -170,20 +203,20 @@ __global__ __launch_bounds__(kThreads) void s3_gather_kernel(T *__restrict__ out
             for (int h = 0; h < kPer; ++h)
                 acc[s][h] += jb.coef[p] * permuted(tiles, s, p, threadIdx.x + h * kThreads);
     }
-    const int64_t v = nvir, v2 = v * v;
     T *dst = out + jb.dst;
+    T old[6][kPer];
+    if (beta != 0.0) prefetch_out(dst, nvir, R, old);
+#pragma unroll
     for (int s = 0; s < 6; ++s) {
         if (duplicate_tile(s, R)) continue;
-        const int t0 = R[kP[s][0]], t1 = R[kP[s][1]], t2 = R[kP[s][2]];
 #pragma unroll
         for (int h = 0; h < kPer; ++h) {
-            const int e = threadIdx.x + h * kThreads;
-            const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
-            if (g0 >= nvir || g1 >= nvir || g2 >= nvir) continue;
+            int g[3];
+            int64_t off;
+            if (!tile_elem(nvir, R, s, h, g, off)) continue;
             double val = acc[s][h];
-            T *o = dst + g0 * v2 + g1 * v + g2;
-            if (beta != 0.0) val += beta * (double)(*o);
-            *o = (T)val;
+            if (beta != 0.0) val += beta * (double)old[s][h];
+            dst[off] = (T)val;
         }
     }
 }
@@ -198,35 +231,55 @@ __global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const T *__restri
                                                                double *__restrict__ partials, int nt, int nso) {
     __shared__ T tiles[6][TE];
     __shared__ double red[2][kThreads / 64];
+    __shared__ double sden[3][3][E];
     int64_t blk;
     int orbit, Rg[3];
     if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, blk, orbit, Rg)) return;
     const int64_t norb = n_orbits(nt);
     const int64_t v = nvir, v2 = v * v, v3 = v2 * v;
     const T *Rb = R + blk * v3;
-    load_orbit(tiles, Rb, nvir, Rg);
+    T *tb = t3 + blk * v3;
     const int oi = blk_ijk[3 * blk], oj = blk_ijk[3 * blk + 1], ok = blk_ijk[3 * blk + 2];
     const double *ei = eia + (int64_t)oi * v, *ej = eia + (int64_t)oj * v, *ek = eia + (int64_t)ok * v;
-    __syncthreads();
-    double nrm = 0.0, nact = 0.0;
-    for (int s = 0; s < 6; ++s) {
-        if (duplicate_tile(s, Rg)) continue;
-        const int t0 = Rg[kP[s][0]], t1 = Rg[kP[s][1]], t2 = Rg[kP[s][2]];
+    // the orbital energies of the three tile ranges of ei, ej, ek (3 x 3 x 8 values, index clamped at
+    // the block edge) in LDS; this thread's t3 elements, loaded together with the R orbit
+    const int w = threadIdx.x / (3 * E), c = (threadIdx.x / E) % 3, i = threadIdx.x % E;
+    const bool has_e = threadIdx.x < 9 * E;
+    const double *ew = w == 0 ? ei : (w == 1 ? ej : ek);
+    const double eval = ew[has_e ? min(Rg[c] * E + i, nvir - 1) : 0];
+    T old[6][kPer];
+#pragma unroll
+    for (int s = 0; s < 6; ++s)
 #pragma unroll
         for (int h = 0; h < kPer; ++h) {
+            int g[3];
+            int64_t off;
+            tile_elem(nvir, Rg, s, h, g, off);
+            old[s][h] = tb[off];
+        }
+    load_orbit(tiles, Rb, nvir, Rg);
+    if (has_e) sden[w][c][i] = eval;
+    __syncthreads();
+    double nrm = 0.0, nact = 0.0;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (duplicate_tile(s, Rg)) continue;
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) {
+            int g[3];
+            int64_t off;
+            if (!tile_elem(nvir, Rg, s, h, g, off)) continue;
             const int e = threadIdx.x + h * kThreads;
-            const int g0 = t0 * E + (e >> 6), g1 = t1 * E + ((e >> 3) & 7), g2 = t2 * E + (e & 7);
-            if (g0 >= nvir || g1 >= nvir || g2 >= nvir) continue;
             double sym = 0.0;
 #pragma unroll
             for (int p = 0; p < 6; ++p) sym += permuted(tiles, s, p, e);
             const double own = permuted(tiles, s, 0, e);
-            const double r = (own - sym / 6.0) / (ei[g0] + ej[g1] + ek[g2]);
-            const int64_t off = blk * v3 + g0 * v2 + g1 * v + g2;
-            t3[off] = (T)((double)t3[off] + r);
-            if (r3_out != nullptr) r3_out[off] = (T)r;
+            const double den = sden[0][kP[s][0]][e >> 6] + sden[1][kP[s][1]][(e >> 3) & 7] + sden[2][kP[s][2]][e & 7];
+            const double r = (own - sym / 6.0) / den;
+            tb[off] = (T)((double)old[s][h] + r);
+            if (r3_out != nullptr) r3_out[blk * v3 + off] = (T)r;
             nrm += r * r;
-            if (g0 < nvir_diis && g1 < nvir_diis && g2 < nvir_diis) nact += r * r;
+            if (g[0] < nvir_diis && g[1] < nvir_diis && g[2] < nvir_diis) nact += r * r;
         }
     }
     // fixed-order reduction: wave shuffles, then the four wave sums in order
