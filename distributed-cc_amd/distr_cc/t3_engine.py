@@ -511,18 +511,18 @@ class TriplesEngine:
         B = Operand(self.t3, 'pabc', (n3, v, v, v))
         Rq = Operand(self.R, 'qabc', (n3, v, v, v))
         launches.append(plan_term('W_oooo/F_oo s0', 'pq,pabc->qabc', A, B, Rq, 1.0, 1.0, [dict(S=0)], ext))
-        # s = 1..5 for a chunk of targets q at a time into Xp[s-1][q - q0], then one gather pass
+        # s = 1..5 for a chunk of targets q at a time into Xp[s-1][q - q0]: one GEMM whose rows are
+        # (s, q), so each T3 panel is read by neighbouring row tiles of all five terms while it is in
+        # L2 (five GEMMs of nq rows each re-read all of T3); then one gather pass
         v3 = v ** 3
         perm = {1: (0, 2, 1), 2: (1, 0, 2), 3: (2, 0, 1), 4: (1, 2, 0), 5: (2, 1, 0)}
         qc = self.Xp.shape[1]
         for q0 in range(0, n3, qc):
             nq = min(qc, n3 - q0)
-            ext_c = dict(ext, q=nq)
-            Aq = Operand.strided(self.pfs, {'S': n3 * n3, 'p': n3, 'q': 1}, base=q0)
-            for s in range(1, 6):
-                X = Operand(self.Xp, 'qabc', (nq, v, v, v), base=(s - 1) * qc * v3)
-                launches.append(plan_term('W_oooo/F_oo s%d' % s, 'pq,pabc->qabc', Aq, B, X, 1.0, 0.0,
-                                          [dict(S=s)], ext_c))
+            ext_c = dict(ext, q=nq, S=5)
+            Aq = Operand.strided(self.pfs, {'S': n3 * n3, 'p': n3, 'q': 1}, base=n3 * n3 + q0)
+            X = Operand.strided(self.Xp, {'S': qc * v3, 'q': v3, 'a': v * v, 'b': v, 'c': 1})
+            launches.append(plan_term('W_oooo/F_oo s1-5', 'pSq,pabc->Sqabc', Aq, B, X, 1.0, 0.0, [dict()], ext_c))
             jobs = []
             for q in range(q0, q0 + nq):
                 srcs, coef = [0] * 6, np.zeros(6)
