@@ -395,7 +395,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the triples path on one GPU; --gpus %d is not supported "
-              "(sharding over canonical blocks is described in DESIGN.md)" % args.gpus, file=sys.stderr)
+              "(the canonical i<=j<=k blocks would shard over GPUs as DistributedT3IJK distributes them; "
+              "no part of this path uses a second GPU)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     from distr_cc import _lib as L
