@@ -104,6 +104,16 @@ __device__ __forceinline__ void kstep(const KMap &m, KIter &it) {
     }
 }
 
+// Split-BF16x3 word of x: bf16(x) in the low half, bf16(x - bf16(x)) in the high half, both rounded
+// to nearest even (the v_cvt_pk_bf16_f32 conversion the split-BF16x3 GEMM applies to FP32 operands).
+__device__ __forceinline__ uint32_t pack_bf16x3(float x) {
+    typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+    const __bf16 h = (__bf16)x;
+    const float hf = __uint_as_float((uint32_t)__builtin_bit_cast(unsigned short, h) << 16);
+    const bf2 w = {h, (__bf16)(x - hf)};
+    return __builtin_bit_cast(uint32_t, w);
+}
+
 }  // namespace dcc
 
 // Return a nonzero status from an entry point when a HIP call fails.

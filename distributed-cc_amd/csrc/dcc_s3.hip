@@ -9,6 +9,7 @@
 // permuted access is then an LDS access, and the six output tiles are written
 // along their contiguous axis.  Each element is read and written exactly once.
 #include "dcc_common.h"
+#include <type_traits>
 #include "dcc_gemm_tile.h"   // xcd_remap
 
 namespace {
@@ -136,8 +137,10 @@ __device__ __forceinline__ double permuted(const T (*tiles)[TE], int s, int p, i
     return (double)tiles[sp][(y[0] << 6) | (y[1] << 3) | y[2]];
 }
 
-template <typename T>
-__global__ __launch_bounds__(kThreads) void s3_transform_kernel(T *__restrict__ out, const T *__restrict__ src,
+// O: the output element type -- T, or uint32_t for FP32 input written as split-BF16x3 words
+// (dcc::pack_bf16x3 of the float result; beta must then be 0).
+template <typename T, typename O = T>
+__global__ __launch_bounds__(kThreads) void s3_transform_kernel(O *__restrict__ out, const T *__restrict__ src,
                                                                 int nvir, double beta,
                                                                 const dcc_s3_job *__restrict__ jobs, int nt,
                                                                 int nso) {
@@ -146,9 +149,11 @@ __global__ __launch_bounds__(kThreads) void s3_transform_kernel(T *__restrict__ 
     int orbit, R[3];
     if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
     const dcc_s3_job jb = jobs[job];
-    T *dst = out + jb.dst;
-    T old[6][kPer];
-    if (beta != 0.0) prefetch_out(dst, nvir, R, old);
+    O *dst = out + jb.dst;
+    O old[6][kPer];
+    if constexpr (std::is_same<O, T>::value) {
+        if (beta != 0.0) prefetch_out(dst, nvir, R, old);
+    }
     load_orbit(tiles, src + jb.src, nvir, R);
     __syncthreads();
 #pragma unroll
@@ -163,8 +168,12 @@ __global__ __launch_bounds__(kThreads) void s3_transform_kernel(T *__restrict__ 
 #pragma unroll
             for (int p = 0; p < 6; ++p)
                 if (jb.coef[p] != 0.0) val += jb.coef[p] * permuted(tiles, s, p, threadIdx.x + h * kThreads);
-            if (beta != 0.0) val += beta * (double)old[s][h];
-            dst[off] = (T)val;
+            if constexpr (std::is_same<O, T>::value) {
+                if (beta != 0.0) val += beta * (double)old[s][h];
+                dst[off] = (T)val;
+            } else {
+                dst[off] = dcc::pack_bf16x3((float)val);
+            }
         }
     }
 }
@@ -347,15 +356,15 @@ int s3_gather(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_gath
     return 0;
 }
 
-template <typename T>
-int s3_transform(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_job *jobs, int64_t njobs,
+template <typename T, typename O = T>
+int s3_transform(O *out, const T *src, int64_t nvir, double beta, const dcc_s3_job *jobs, int64_t njobs,
                  dcc_stream_t stream) {
     DCC_CHECK(nvir >= 0 && njobs >= 0, "negative extent");
     if (nvir == 0 || njobs == 0) return 0;
     DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
     const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
     DCC_CHECK(8 * nso * njobs < ((int64_t)1 << 31), "grid too large");
-    hipLaunchKernelGGL(s3_transform_kernel<T>, dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+    hipLaunchKernelGGL((s3_transform_kernel<T, O>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
                        dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
@@ -390,6 +399,11 @@ int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double be
 int dcc_s3_transform_f32(float *out, const float *src, int64_t nvir, double beta, const dcc_s3_job *jobs,
                          int64_t njobs, dcc_stream_t stream) {
     return s3_transform<float>(out, src, nvir, beta, jobs, njobs, stream);
+}
+
+int dcc_s3_transform_pack_f32(uint32_t *out, const float *src, int64_t nvir, const dcc_s3_job *jobs, int64_t njobs,
+                              dcc_stream_t stream) {
+    return s3_transform<float, uint32_t>(out, src, nvir, 0.0, jobs, njobs, stream);
 }
 
 int dcc_s3_gather_f64(double *out, const double *src, int64_t nvir, double beta, const dcc_s3_gather_job *jobs,

@@ -92,15 +92,9 @@ __global__ __launch_bounds__(256) void sum_slices_kernel(T *__restrict__ dst, co
 // Split-BF16x3 packing: dst[i] = bf16(x) | bf16(x - bf16(x)) << 16 with x = src[i] (round to nearest
 // even, the v_cvt_pk_bf16_f32 conversion of the GEMM's own split).  dst may alias src.
 __global__ __launch_bounds__(256) void pack_bf16x3_kernel(const float *src, uint32_t *dst, int64_t n) {
-    typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        const float x = src[i];
-        const __bf16 h = (__bf16)x;
-        const float hf = __uint_as_float((uint32_t)__builtin_bit_cast(unsigned short, h) << 16);
-        const bf2 w = {h, (__bf16)(x - hf)};
-        dst[i] = __builtin_bit_cast(uint32_t, w);
-    }
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        dst[i] = dcc::pack_bf16x3(src[i]);
 }
 
 __global__ __launch_bounds__(256) void fill_zero_kernel(double *__restrict__ p) {

@@ -95,6 +95,7 @@ _SIGS = {
                                        IdxMap, c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_transform_f64": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_transform_f32": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
+    "dcc_s3_transform_pack_f32": ([c_vp, c_vp, c_i64, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_gather_f64": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_s3_gather_f32": ([c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp], ctypes.c_int),
     "dcc_r3_finalize_tiles": ([c_i64], c_i64),

@@ -437,7 +437,8 @@ class TriplesEngine:
         """Split-BF16x3 residual: point every GEMM operand at a packed (hi | lo << 16) copy of its
         tensor, so each element is split once per residual instead of once per tile that reads it.
         Y and Z are packed in place (they are rebuilt from T3 for every residual and read by nothing
-        else); the others get packed copies.  Returns the (source, packed, count) passes to run."""
+        else), by writing them as packed words in the first place; the others get packed copies.
+        Returns the (source, packed, count) passes to run."""
         sources = [self.t3, self.Yz, self.W1, self.W_ovvv, self.W_oovo, self.t2, self.WR1, self.WR3, self.pfs]
         packed = {}
         # Y/Z are packed in place unless a GEMM kept on FP32 (K <= 16, tiny systems) reads them
@@ -465,7 +466,10 @@ class TriplesEngine:
             args[8] = ctypes.c_void_p(b[1].ptr + (g.args[8].value - b[0].ptr))
             g.args = tuple(args)
             g.fn = "dcc_contract_packed_bf16x3"
-        return [(src, dst, int(np.prod(src.shape))) for src, dst in packed.values()]
+        # Y/Z packed in place are written as packed words by materialize_y itself
+        self._yz_direct = yz_in_place and id(self.Yz) in packed
+        return [(src, dst, int(np.prod(src.shape))) for src, dst in packed.values()
+                if not (self._yz_direct and src is self.Yz)]
 
     def pack_operands(self):
         """Run the split-BF16x3 packing passes (after T3, Y/Z and the intermediates are final)."""
@@ -845,12 +849,17 @@ class TriplesEngine:
 
     # -------------------------------------------------------------- compute
     def materialize_y(self):
+        """Y (and Z) from T3; in the split-BF16x3 engine written directly as packed words."""
         s = self.stream.handle if self.stream is not None else None
-        L.call("dcc_s3_transform_" + self.sfx, self.Y.vp, self.t3.vp, self.nvir, 0.0, self.y_jobs.vp,
-               self.n_y_jobs, s)
+        if getattr(self, '_yz_direct', False):
+            call = lambda dst, jobs: L.call("dcc_s3_transform_pack_f32", dst.vp, self.t3.vp, self.nvir, jobs.vp,
+                                            self.n_y_jobs, s)
+        else:
+            call = lambda dst, jobs: L.call("dcc_s3_transform_" + self.sfx, dst.vp, self.t3.vp, self.nvir, 0.0,
+                                            jobs.vp, self.n_y_jobs, s)
+        call(self.Y, self.y_jobs)
         if self.ring_fused:
-            L.call("dcc_s3_transform_" + self.sfx, self.Z.vp, self.t3.vp, self.nvir, 0.0, self.z_jobs.vp,
-                   self.n_y_jobs, s)
+            call(self.Z, self.z_jobs)
 
     def residual(self, timer=None):
         """R <- r3 of distr_cc/rccsdt.py:368-433 for the current t3 and intermediates.

@@ -187,6 +187,11 @@ int dcc_s3_transform_f64(double *out, const double *src, int64_t nvir, double be
 /* FP32 storage, FP64 arithmetic. */
 int dcc_s3_transform_f32(float *out, const float *src, int64_t nvir, double beta,
                          const dcc_s3_job *jobs, int64_t njobs, dcc_stream_t stream);
+/* dcc_s3_transform_f32 with beta = 0 whose output is written as split-BF16x3 words (dcc_pack_bf16x3
+ * of each float result): builds a packed GEMM operand in one pass instead of a transform and a
+ * packing pass.  No reference counterpart (fuses take_t3_ijk_single_ with the split). */
+int dcc_s3_transform_pack_f32(uint32_t *out, const float *src, int64_t nvir, const dcc_s3_job *jobs,
+                              int64_t njobs, dcc_stream_t stream);
 
 /*
  * Multi-source S3 transform: each job adds sum_p c[p] * (block at src[p]).transpose(p) to the out

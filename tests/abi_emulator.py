@@ -329,6 +329,15 @@ class Emulator:
     def dcc_s3_transform_f32(self, *a):
         return self._s3(np.float32, *a)
 
+    def dcc_s3_transform_pack_f32(self, out, src, nvir, jobs, njobs, stream):
+        # float results in place, then packed in place (the words the fused kernel writes)
+        self._s3(np.float32, out, src, nvir, 0.0, jobs, njobs, stream)
+        v3 = int(nvir) ** 3
+        J = self.dev.view(_ptr(jobs), np.int64, 8 * int(njobs)).reshape(-1, 8)
+        for dst in J[:, 0]:
+            self.dcc_pack_bf16x3(_ptr(out) + 4 * int(dst), _ptr(out) + 4 * int(dst), v3, stream)
+        return 0
+
     def _s3(self, dt, out, src, nvir, beta, jobs, njobs, stream):
         v = int(nvir)
         O = self.dev.view(_ptr(out), dt)

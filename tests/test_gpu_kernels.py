@@ -105,6 +105,25 @@ def test_s3_transform_all_perms(dev, v, dt):
         assert abs(out[n] - ref).max() < (1e-13 if dt == "f64" else 1e-6 * max(1.0, abs(ref).max())), (n, v)
 
 
+@pytest.mark.parametrize("v", [5, 8, 20])
+def test_s3_transform_pack_f32(dev, v):
+    """dcc_s3_transform_pack_f32 writes exactly the words of dcc_s3_transform_f32 followed by
+    dcc_pack_bf16x3."""
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    from distr_cc.t3_engine import s3_jobs_array
+    _say("s3 transform pack v=%d" % v)
+    rng = np.random.default_rng(70 + v)
+    X = rng.normal(size=(2, v, v, v)).astype(np.float32)
+    jobs = [(0, v ** 3, np.array([2.0, -1.0, 0.0, 0.0, 0.0, -1.0])), (v ** 3, 0, np.eye(6)[3])]
+    dX, dj = DeviceArray.from_host(X), DeviceArray.from_host(s3_jobs_array(jobs))
+    ref, got = DeviceArray.zeros((2, v, v, v), np.float32), DeviceArray((2, v, v, v), np.uint32)
+    L.call("dcc_s3_transform_f32", ref.vp, dX.vp, v, 0.0, dj.vp, len(jobs), None)
+    L.call("dcc_pack_bf16x3", ref.vp, ref.vp, ref.size, None)
+    L.call("dcc_s3_transform_pack_f32", got.vp, dX.vp, v, dj.vp, len(jobs), None)
+    assert np.array_equal(got.download(), ref.download().view(np.uint32))
+
+
 @pytest.mark.parametrize("shift", [0, 1])
 @pytest.mark.parametrize("dt", ["f64", "f32"])
 @pytest.mark.parametrize("v", [1, 5, 8, 13, 20, 33])
