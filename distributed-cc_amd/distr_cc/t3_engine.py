@@ -430,6 +430,7 @@ class TriplesEngine:
         self.ring_g_bytes = int(ring_g_bytes)
         self.launches = self._plan_residual()
         self._retarget(self.launches)
+        self._yz_direct = False   # Y/Z written as packed words (set by _setup_packing)
         self._packs = self._setup_packing() if contract == "bf16x3" else []
         self._lower = None
 
@@ -851,7 +852,7 @@ class TriplesEngine:
     def materialize_y(self):
         """Y (and Z) from T3; in the split-BF16x3 engine written directly as packed words."""
         s = self.stream.handle if self.stream is not None else None
-        if getattr(self, '_yz_direct', False):
+        if self._yz_direct:
             call = lambda dst, jobs: L.call("dcc_s3_transform_pack_f32", dst.vp, self.t3.vp, self.nvir, jobs.vp,
                                             self.n_y_jobs, s)
         else:
