@@ -162,6 +162,34 @@ def test_s3_gather_multi_source(dev, v, dt, shift):
         assert abs(out[q] - ref).max() < tol * max(1.0, abs(ref).max()), (q, v)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_residual_chunked_vs_oracle(dev, prec):
+    """Residual with the memory-bounded paths of large systems at nocc=10, nvir=9 (220 blocks): ring
+    operator in chunks of 26 of the 55 pairs, W_oooo/F_oo targets in 64-row chunks (a bound of 52
+    rounded up) and a partial one, terms 1..5 as one stacked GEMM; split-BF16x3 with Y/Z written as
+    packed words."""
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    from oracle import rccsdt_tri as tri
+    o, v = 10, 9
+    _say("residual chunked o=%d v=%d %s" % (o, v, prec))
+    s = SyntheticSystem(o, v, naux=30, seed=77)
+    rng = np.random.default_rng(78)
+    im = {k: x + 1e-3 * rng.normal(size=x.shape) for k, x in s.bare_intermediates().items()}
+    t2, t3 = s.t2(), s.t3_tri()
+    kw = dict(dtype=np.float32, contract="bf16x3") if prec == "bf16x3" else {}
+    eng = TriplesEngine(o, v, ring_g_bytes=52 * 5 * v ** 3 * (4 if prec == "bf16x3" else 8), **kw)
+    assert eng.Xp.shape[1] == 64 and eng.Gs.shape[0] == 26 and eng.npairs == 55
+    assert eng._yz_direct == (prec == "bf16x3")
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.residual()
+    R = eng.R.download().astype(np.float64)
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3, batch_size=13)
+    tol = 1e-12 if prec == "fp64" else 3e-4
+    assert abs(R - Ro).max() <= tol * abs(Ro).max(), abs(R - Ro).max() / abs(Ro).max()
+
+
 @pytest.mark.parametrize("o,v", [(1, 3), (3, 4), (4, 9), (5, 17), (3, 24)])
 def test_residual_and_update_vs_oracle(dev, o, v):
     from distr_cc.synthetic import SyntheticSystem
