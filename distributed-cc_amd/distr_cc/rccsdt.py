@@ -130,6 +130,9 @@ class RCCSDT:
     # "fp64" (the reference's arithmetic), "fp32" (T3 and the triples contractions in FP32 on MFMA)
     # or "bf16x3" (FP32 T3, contractions as split-BF16x3 on BF16 MFMA); everything else in FP64
     triples_precision = "fp64"
+    # time the workgroup tiles of every triples contraction once, when the device engine is built for
+    # a system, and keep the fastest (TriplesEngine.autotune); off: the library's default tiles
+    triples_autotune = False
 
     def __init__(self, mf, comm=None, frozen=None, mo_coeff=None, mo_occ=None):
         if comm is None:
@@ -233,6 +236,10 @@ class RCCSDT:
                                        level_shift=self.level_shift, nvir_diis=self.nvir_diis, t3_dtype=t3_dtype,
                                        triples_contract=contract)
             self.engine._eris = eris
+            if self.triples_autotune:
+                # runs the contractions on the engine's current buffers; every buffer they write is
+                # cleared or overwritten before the first iteration reads it
+                self.engine.tr.autotune()
         return self.engine
 
     def init_amps(self, eris=None):

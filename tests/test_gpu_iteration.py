@@ -193,3 +193,25 @@ def test_bf16x3_kernel_energy_error(dev):
         _say("%s converged=%s cycles=%d E=%.12f" % (prec, cc.converged, cc.cycles, res[prec]))
         assert cc.converged
     assert abs(res["fp64"] - res["bf16x3"]) < 1e-6
+
+
+def test_kernel_with_autotuned_tiles(dev):
+    """RCCSDT.triples_autotune: the tuned tiles give the energy of the default tiles (the tile changes
+    neither the K order of a sum nor any result beyond round-off)."""
+    from ccsdt_cases import DuckMF, molecule_like
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 4, 10
+    _say("autotuned kernel o=%d v=%d" % (o, v))
+    F, g, e, _ = molecule_like(o, v, 51)
+    res = {}
+    for tune in (False, True):
+        cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+        cc.verbose, cc.triples_autotune = 0, tune
+        cc.conv_tol, cc.conv_tol_normt = 1e-10, 1e-8
+        res[tune] = cc.kernel(eris=_PhysicistsERIs(F, e, g))[0]
+        assert cc.converged
+        if tune:
+            assert any(t is not None and t >= 0 for t in cc.engine.tr.tile_config())
+    _say("E default %.14f tuned %.14f" % (res[False], res[True]))
+    assert abs(res[False] - res[True]) < 1e-11
