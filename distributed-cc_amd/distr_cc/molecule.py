@@ -44,6 +44,17 @@ BASIS = {
                    (0.1220000, 0.5012400)]),
               (0, [(0.1220000, 1.0)]),
               (1, [(0.7270000, 1.0)])],
+        "N": [(0, [(9046.0000000, 0.0007000), (1357.0000000, 0.0053890), (309.3000000, 0.0274060),
+                   (87.7300000, 0.1032070), (28.5600000, 0.2787230), (10.2100000, 0.4485400),
+                   (3.8380000, 0.2782380), (0.7466000, 0.0154400), (0.2248000, -0.0028640)]),
+              (0, [(9046.0000000, -0.0001530), (1357.0000000, -0.0012080), (309.3000000, -0.0059920),
+                   (87.7300000, -0.0245440), (28.5600000, -0.0674590), (10.2100000, -0.1580780),
+                   (3.8380000, -0.1218310), (0.7466000, 0.5490030), (0.2248000, 0.5788150)]),
+              (0, [(0.2248000, 1.0)]),
+              (1, [(13.5500000, 0.0399190), (2.9170000, 0.2171690), (0.7973000, 0.5103190),
+                   (0.2185000, 0.4622140)]),
+              (1, [(0.2185000, 1.0)]),
+              (2, [(0.8170000, 1.0)])],
         "O": [(0, [(11720.0000000, 0.0007100), (1759.0000000, 0.0054700), (400.8000000, 0.0278370),
                    (113.7000000, 0.1048000), (37.0300000, 0.2830620), (13.2700000, 0.4487190),
                    (5.0250000, 0.2709520), (1.0130000, 0.0154580), (0.3023000, -0.0025850)]),
@@ -253,6 +264,12 @@ def _cart2sph(l):
             [s3 / 2, 0, 0, -s3 / 2, 0, 0],      # dx2-y2
         ])
     raise NotImplementedError("spherical transformation implemented for l <= 2")
+
+
+def chemcore(mol):
+    """Number of core orbitals to freeze (pyscf.data.elements.chemcore semantics for H..Ne: none for
+    H and He, the 1s orbital for Li..Ne), as the reference's examples pass to RCCSDT(frozen=...)."""
+    return int(sum(0 if z <= 2 else 1 for z in mol.charges))
 
 
 class Molecule:

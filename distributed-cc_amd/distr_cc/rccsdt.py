@@ -125,6 +125,7 @@ class RCCSDT:
     diis_start_cycle = 0
     diis_start_energy_diff = 1e9
     level_shift = 0.0
+    iterative_damping = 1.0
     verbose = 4
     callback = None
     # "fp64" (the reference's arithmetic), "fp32" (T3 and the triples contractions in FP32 on MFMA)
@@ -228,18 +229,27 @@ class RCCSDT:
 
     # -- amplitudes
     def _engine(self, eris):
-        if self.engine is None or self.engine._eris is not eris:
+        """The device engine of ``eris``: built once per (eris, triples_precision); the level shift and
+        nvir_diis are pushed into it at every call, and the tiles are tuned once when
+        triples_autotune is set."""
+        if self.engine is None or self.engine._eris is not eris or \
+                self.engine._precision != self.triples_precision:
             nocc, nmo = self.nocc, self.nmo
             t3_dtype, contract = {"fp64": (np.float64, "native"), "fp32": (np.float32, "native"),
                                   "bf16x3": (np.float32, "bf16x3")}[self.triples_precision]
+            self.engine = None   # release the previous engine's device memory first
             self.engine = RCCSDTEngine(nocc, nmo - nocc, eris.fock, eris.pppp, eris.mo_energy,
                                        level_shift=self.level_shift, nvir_diis=self.nvir_diis, t3_dtype=t3_dtype,
                                        triples_contract=contract)
             self.engine._eris = eris
-            if self.triples_autotune:
-                # runs the contractions on the engine's current buffers; every buffer they write is
-                # cleared or overwritten before the first iteration reads it
-                self.engine.tr.autotune()
+            self.engine._precision = self.triples_precision
+            self.engine._tuned = False
+        self.engine.configure(level_shift=self.level_shift, nvir_diis=self.nvir_diis)
+        if self.triples_autotune and not self.engine._tuned:
+            # runs the contractions on the engine's current buffers; every buffer they write is
+            # cleared or overwritten before the first iteration reads it
+            self.engine.tr.autotune()
+            self.engine._tuned = True
         return self.engine
 
     def init_amps(self, eris=None):
@@ -295,8 +305,13 @@ def kernel(mycc, eng, tol=1e-8, tolnormt=1e-6, max_cycle=50):
     e_corr = eng.energy()
     if log:
         print("Init E_corr(RCCSDT) = %.15g" % e_corr, file=log)
-    adiis = DeviceDIIS(eng.diis_size(), mycc.diis_space) if mycc.diis else None
-    vec = DeviceArray((eng.diis_size(),)) if adiis is not None else None
+    # unified T1/T2/T3 DIIS (run_diis_t3) or, with do_diis_max_t = False, the standard DIIS over
+    # T1/T2 only, T3 not extrapolated (rccsdt.py:1096-1099,1114-1122)
+    with_t3 = bool(mycc.do_diis_max_t)
+    if getattr(mycc, "iterative_damping", 1.0) < 1.0:
+        raise NotImplementedError("Damping is not implemented")
+    adiis = DeviceDIIS(eng.diis_size(with_t3), mycc.diis_space) if mycc.diis else None
+    vec = DeviceArray((eng.diis_size(with_t3),)) if adiis is not None else None
     converged = False
     mycc.cycles = 0
     for istep in range(max_cycle):
@@ -306,9 +321,9 @@ def kernel(mycc, eng, tol=1e-8, tolnormt=1e-6, max_cycle=50):
         normt = float(np.linalg.norm(res_norm))
         if adiis is not None and istep >= mycc.diis_start_cycle and abs(e_corr - e_corr_old) < \
                 mycc.diis_start_energy_diff:
-            eng.pack(vec)
+            eng.pack(vec, with_t3)
             xnew = adiis.update(vec)
-            eng.unpack(xnew)
+            eng.unpack(xnew, with_t3)
             xnew.free()
         e_corr_old, e_corr = e_corr, eng.energy()
         mycc.cycles = istep + 1

@@ -128,7 +128,11 @@ class RCCSDTEngine:
         self.nocc, self.nvir, self.nmo = o, v, n
         self.stream = stream
         self.level_shift = float(level_shift)
+        # resolve_nvir_diis (_runtime.py:121-128): None -> nvir, positive, at most nvir
         self.nvir_diis = v if nvir_diis is None else int(nvir_diis)
+        if self.nvir_diis <= 0:
+            raise ValueError("nvir_diis must be a positive integer")
+        self.nvir_diis = min(self.nvir_diis, v)
         self.tr = tr = TriplesEngine(o, v, stream=stream, device=device, dtype=t3_dtype,
                                      contract=triples_contract)
         self.mixed = tr.dtype != np.float64
@@ -141,8 +145,10 @@ class RCCSDTEngine:
         self.G0 = DeviceArray.from_host(pppp, stream)
         self.F0 = DeviceArray.from_host(fock, stream)
         self.mo_energy = np.asarray(mo_energy, dtype=f64).copy()
-        eia = self.mo_energy[:o, None] - self.mo_energy[None, o:] - self.level_shift
-        self.eia = DeviceArray.from_host(np.ascontiguousarray(eia), stream)
+        eia0 = self.mo_energy[:o, None] - self.mo_energy[None, o:]
+        # unshifted e_i - e_a for the MP2 guess (rccsdt.py:212-214); shifted for the residual division
+        self.eia0 = DeviceArray.from_host(np.ascontiguousarray(eia0), stream)
+        self.eia = DeviceArray.from_host(np.ascontiguousarray(eia0 - self.level_shift), stream)
         # work buffers
         self.G = DeviceArray((n, n, n, n), f64)
         self.Hc = DeviceArray((n, n), f64)
@@ -409,11 +415,12 @@ class RCCSDTEngine:
         return float(sc[2] + 2.0 * sc[3])
 
     # ------------------------------------------------------------- DIIS packing
-    def diis_size(self):
+    def diis_size(self, with_t3=True):
         """Length of the unified DIIS vector of run_diis_t3 (rccsdt.py:1038-1049):
-        t1, t2 over pairs i <= j, and t3[:, :nd, :nd, :nd]."""
+        t1, t2 over pairs i <= j, and t3[:, :nd, :nd, :nd]; with_t3=False: the T1/T2 vector of the
+        standard DIIS (do_diis_max_t = False, rccsdt.py:1096-1099,1119-1122)."""
         o, v, nd = self.nocc, self.nvir, self.nvir_diis
-        return o * v + (o * (o + 1) // 2) * v * v + self.tr.n3 * nd ** 3
+        return o * v + (o * (o + 1) // 2) * v * v + (self.tr.n3 * nd ** 3 if with_t3 else 0)
 
     def _copy(self, dst, dst_off, src, src_off, shape, dst_strides, src_strides, alpha=1.0, beta=0.0):
         lay = L.Strided()
@@ -426,7 +433,7 @@ class RCCSDTEngine:
                src.offset_ptr(src_off), ctypes.byref(lay), float(alpha), float(beta),
                self.stream.handle if self.stream is not None else None)
 
-    def pack(self, x):
+    def pack(self, x, with_t3=True):
         o, v, nd, n3 = self.nocc, self.nvir, self.nvir_diis, self.tr.n3
         v2 = v * v
         self._copy(x, 0, self.t1, 0, (o * v,), (1,), (1,))
@@ -435,9 +442,10 @@ class RCCSDTEngine:
             cnt = (o - i) * v2
             self._copy(x, off, self.t2, (i * o + i) * v2, (cnt,), (1,), (1,))
             off += cnt
-        self._copy(x, off, self.tr.t3, 0, (n3, nd, nd, nd), (nd ** 3, nd * nd, nd, 1), (v ** 3, v2, v, 1))
+        if with_t3:
+            self._copy(x, off, self.tr.t3, 0, (n3, nd, nd, nd), (nd ** 3, nd * nd, nd, 1), (v ** 3, v2, v, 1))
 
-    def unpack(self, x):
+    def unpack(self, x, with_t3=True):
         o, v, nd, n3 = self.nocc, self.nvir, self.nvir_diis, self.tr.n3
         v2 = v * v
         self._copy(self.t1, 0, x, 0, (o * v,), (1,), (1,))
@@ -448,16 +456,31 @@ class RCCSDTEngine:
             for j in range(i + 1, o):   # t2[j,i,b,a] = t2[i,j,a,b]
                 self._copy(self.t2, (j * o + i) * v2, x, off + (j - i) * v2, (v, v), (1, v), (v, 1))
             off += cnt
-        self._copy(self.tr.t3, 0, x, off, (n3, nd, nd, nd), (v ** 3, v2, v, 1), (nd ** 3, nd * nd, nd, 1))
+        if with_t3:
+            self._copy(self.tr.t3, 0, x, off, (n3, nd, nd, nd), (v ** 3, v2, v, 1), (nd ** 3, nd * nd, nd, 1))
+
+    def configure(self, level_shift=None, nvir_diis=None):
+        """Change the level shift of the denominators and the DIIS virtual subspace of an existing
+        engine (the RCCSDT attributes of rccsdt.py:1164-1191 read again at each kernel() call)."""
+        o = self.nocc
+        if level_shift is not None and float(level_shift) != self.level_shift:
+            self.level_shift = float(level_shift)
+            eia = self.mo_energy[:o, None] - self.mo_energy[None, o:] - self.level_shift
+            self.eia.upload(np.ascontiguousarray(eia), self.stream)
+        nd = self.nvir if nvir_diis is None else int(nvir_diis)
+        if nd <= 0:
+            raise ValueError("nvir_diis must be a positive integer")
+        self.nvir_diis = min(nd, self.nvir)
 
     def init_amps(self):
-        """t1 = f_ov / eia, t2 = <ij|ab> / eijab, T3 = 0 (rccsdt.py:197-251), on the device."""
+        """t1 = f_ov / eia, t2 = <ij|ab> / eijab, T3 = 0 (rccsdt.py:197-251), on the device; eia here
+        is e_i - e_a without the level shift (rccsdt.py:212-214)."""
         o, v = self.nocc, self.nvir
         st = self.stream
         sh = st.handle if st is not None else None
         tensor_axpby(self.nview(self.t1, "ia"), self.nview(self.fov0, "ia"), "ia", (o, v), 1.0, 0.0, st)
-        L.call("dcc_denom_divide_f64", self.t1.vp, o, v, 1, self.eia.vp, sh)
+        L.call("dcc_denom_divide_f64", self.t1.vp, o, v, 1, self.eia0.vp, sh)
         tensor_axpby(Operand(self.t2, "ijab", (o, o, v, v)), self.gview(self.G0, "ijab"), "ijab", (o, o, v, v),
                      1.0, 0.0, st)
-        L.call("dcc_denom_divide_f64", self.t2.vp, o, v, 2, self.eia.vp, sh)
+        L.call("dcc_denom_divide_f64", self.t2.vp, o, v, 2, self.eia0.vp, sh)
         self.tr.t3.fill(0.0, st)

@@ -142,12 +142,23 @@ def compute_oooo_oovv_contraction(nocc, nvir, imds, t3_local, r3, batch_size=100
     n3 = len(ijk_list)
     batch_size = min(batch_size, n3)
     targets = canonical_triples(nocc)
+    # with a target subset, a source block is only used when one of its index pairs is an index pair
+    # of a sampled target (every ring condition below compares two target indices with two source
+    # indices); the others add exactly nothing, so their transposes are not formed
+    pairs = None
+    if tset is not None:
+        pairs = set()
+        for q in tset:
+            i, j, k = targets[q]
+            pairs.update(((j, k), (i, k), (i, j)))
     for b0 in range(0, n3, batch_size):
         b1 = min(b0 + batch_size, n3)
         batch_ijk = ijk_list[b0:b1]
         t3_data = t3_local[b0:b1]
         compute_r3_Foo_Woooo_contribution_batch(nocc, r3, t3_data, batch_ijk, imds, tset)
         for idx, (o00, o10, o20) in enumerate(batch_ijk):
+            if pairs is not None and not pairs.intersection(((o00, o10), (o00, o20), (o10, o20))):
+                continue
             t = t3_data[idx]
             t_2 = t3_transpose_add(t)                                 # :534
             p0, p1, p2 = t3_triple_sym(t)                             # :535
@@ -276,16 +287,21 @@ def compute_r3_Foo_Woooo_contribution_batch(nocc, r3, t3_data, batch_ijk, imds, 
             if o00 != o10 and i == o10 and j == o20:
                 pfs[4, idx, q] -= F_oo[o00, k]
     if tset is None:
-        out, sel = r3, pfs
-    else:  # only the sampled target rows are nonzero in pfs
+        out, sel, src = r3, pfs, t3_data
+    else:
+        # only the sampled target columns of pfs are nonzero, and only the source rows that meet one
+        # of them: the GEMMs run over those (the skipped terms are exact zeros)
         qs = np.array(sorted(tset), dtype=np.int64)
-        out, sel = r3[qs], pfs[:, :, qs]
-    einsum('pq,pabc->qabc', sel[0], t3_data, out, 1.0, 1.0)            # :726-731
-    einsum('pq,pacb->qabc', sel[1], t3_data, out, 1.0, 1.0)
-    einsum('pq,pbac->qabc', sel[2], t3_data, out, 1.0, 1.0)
-    einsum('pq,pabc->qcab', sel[3], t3_data, out, 1.0, 1.0)
-    einsum('pq,pabc->qbca', sel[4], t3_data, out, 1.0, 1.0)
-    einsum('pq,pabc->qcba', sel[5], t3_data, out, 1.0, 1.0)
+        ps = np.nonzero(np.any(pfs[:, :, qs] != 0.0, axis=(0, 2)))[0]
+        if len(ps) == 0:
+            return r3
+        out, sel, src = r3[qs], pfs[:, ps][:, :, qs], t3_data[ps]
+    einsum('pq,pabc->qabc', sel[0], src, out, 1.0, 1.0)            # :726-731
+    einsum('pq,pacb->qabc', sel[1], src, out, 1.0, 1.0)
+    einsum('pq,pbac->qabc', sel[2], src, out, 1.0, 1.0)
+    einsum('pq,pabc->qcab', sel[3], src, out, 1.0, 1.0)
+    einsum('pq,pabc->qbca', sel[4], src, out, 1.0, 1.0)
+    einsum('pq,pabc->qcba', sel[5], src, out, 1.0, 1.0)
     if tset is not None:
         r3[qs] = out
     return r3

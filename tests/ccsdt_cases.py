@@ -45,3 +45,52 @@ class DuckMF:
         self.mo_occ = np.array([2.0] * nocc + [0.0] * nvir)
         self.mo_energy = mo_energy
         self.e_tot = 0.0
+
+
+def with_fov(F, nocc, scale, seed):
+    """F with a symmetric occupied-virtual block added (a non-canonical Fock matrix, so t1 != 0)."""
+    rng = np.random.default_rng(seed)
+    F = F.copy()
+    X = scale * rng.standard_normal((nocc, F.shape[0] - nocc))
+    F[:nocc, nocc:] += X
+    F[nocc:, :nocc] += X.T
+    return F
+
+
+def oracle_kernel(nocc, F, g, e, cycles, level_shift=0.0, nvir_diis=None, diis=True, with_t3=True, space=6):
+    """``cycles`` iterations of the reference kernel loop (distr_cc/rccsdt.py:1069-1148) on the oracle:
+    MP2 start (unshifted denominators), oracle update_amps, then DIIS (oracle.diis) over the unified
+    vector [t1, t2 over pairs i <= j, t3[:, :nd, :nd, :nd]] (run_diis_t3, rccsdt.py:1026-1067) or, with
+    with_t3=False, over [t1, t2 pairs] only (the standard DIIS of do_diis_max_t = False, T3 not
+    extrapolated).  Returns (t1, t2, t3, energy after every cycle)."""
+    from oracle import rccsdt_lower as lo, rccsdt_tri as tri
+    from oracle.diis import DIIS
+    o = nocc
+    v = F.shape[0] - o
+    _, t1, t2 = tri.init_amps_rhf(o, F, e, g)
+    n3 = o * (o + 1) * (o + 2) // 6
+    t3 = np.zeros((n3, v, v, v))
+    nd = v if nvir_diis is None else min(int(nvir_diis), v)
+    d = DIIS(space) if diis else None
+    pairs = [(i, j) for i in range(o) for j in range(i, o)]
+    energies = []
+    for _ in range(cycles):
+        t1, t2, t3, _res = lo.update_amps(o, F, g, e, t1, t2, t3, level_shift=level_shift, nvir_diis=nd)
+        if d is not None:
+            parts = [t1.ravel(), np.concatenate([t2[i, j].ravel() for i, j in pairs])]
+            if with_t3:
+                parts.append(t3[:, :nd, :nd, :nd].ravel())
+            x = d.update(np.concatenate(parts))
+            t1 = x[:o * v].reshape(o, v).copy()
+            off = o * v
+            t2 = t2.copy()
+            for i, j in pairs:
+                blk = x[off:off + v * v].reshape(v, v)
+                off += v * v
+                t2[i, j] = blk
+                t2[j, i] = blk.T
+            if with_t3:
+                t3 = t3.copy()
+                t3[:, :nd, :nd, :nd] = x[off:].reshape(n3, nd, nd, nd)
+        energies.append(tri.energy_rhf(o, F, g, t1, t2))
+    return t1, t2, t3, energies

@@ -310,3 +310,72 @@ def test_batched_ensemble_plan(monkeypatch):
     for e, (nrm, t3ref) in zip(bt.engines, refs):
         assert abs(e.get_t3() - t3ref).max() <= 1e-13 * abs(t3ref).max()
         assert abs(e.read_norms()[0] - nrm[0]) <= 1e-12 * nrm[0]
+
+
+def test_init_amps_unshifted_with_level_shift(monkeypatch):
+    """The MP2 guess divides by e_i - e_a without the level shift (rccsdt.py:212-214), also when the
+    iteration's denominators carry one; t1 != 0 through a non-canonical f_ov."""
+    abi_emulator.install(monkeypatch)
+    from ccsdt_cases import DuckMF, molecule_like, with_fov
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 3, 4
+    F, g, e, _ = molecule_like(o, v, 31)
+    F = with_fov(F, o, 0.02, 32)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.level_shift, cc.verbose = 0.05, 0
+    emp2, tamps = cc.init_amps(_PhysicistsERIs(F, e, g))
+    e_ref, t1, t2 = tri.init_amps_rhf(o, F, e, g)
+    assert abs(t1).max() > 1e-4
+    assert abs(tamps[0] - t1).max() <= 1e-14 * abs(t1).max()
+    assert abs(tamps[1] - t2).max() <= 1e-14 * abs(t2).max()
+    assert abs(emp2 - e_ref) <= 1e-14 * abs(e_ref) and not tamps[2][1].any()
+
+
+@pytest.mark.parametrize("with_t3,nvir_diis,diis,shift", [(True, None, True, 0.0), (True, 2, True, 0.05),
+                                                         (False, None, True, 0.0), (False, 2, True, 0.05),
+                                                         (True, None, False, 0.0)])
+def test_kernel_diis_trajectory(monkeypatch, with_t3, nvir_diis, diis, shift):
+    """RCCSDT.kernel after 4 cycles == the oracle loop after 4 cycles: unified T1/T2/T3 DIIS
+    (do_diis_max_t = True, with an nvir_diis subspace), standard T1/T2 DIIS (do_diis_max_t = False)
+    and no DIIS, with and without a level shift (distr_cc/rccsdt.py:1026-1148)."""
+    abi_emulator.install(monkeypatch)
+    from ccsdt_cases import DuckMF, molecule_like, oracle_kernel, with_fov
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 2, 4
+    F, g, e, _ = molecule_like(o, v, 41)
+    F = with_fov(F, o, 0.01, 42)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.do_diis_max_t, cc.nvir_diis, cc.diis, cc.level_shift = with_t3, nvir_diis, diis, shift
+    cc.max_cycle, cc.verbose = 4, 0
+    e_cc, tamps = cc.kernel(eris=_PhysicistsERIs(F, e, g))
+    t1, t2, t3, energies = oracle_kernel(o, F, g, e, 4, level_shift=shift, nvir_diis=nvir_diis, diis=diis,
+                                         with_t3=with_t3)
+    assert cc.cycles == 4
+    assert abs(e_cc - energies[-1]) <= 1e-13
+    for a, b in ((tamps[0], t1), (tamps[1], t2), (tamps[2][1], t3)):
+        assert abs(a - b).max() <= 1e-11 * abs(b).max()
+
+
+def test_engine_cache_follows_attributes(monkeypatch):
+    """A second kernel() on the same eris picks up changed level_shift / nvir_diis / triples_precision."""
+    abi_emulator.install(monkeypatch)
+    from ccsdt_cases import DuckMF, molecule_like
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 2, 3
+    F, g, e, _ = molecule_like(o, v, 43)
+    eris = _PhysicistsERIs(F, e, g)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.max_cycle, cc.verbose = 2, 0
+    cc.kernel(eris=eris)
+    eng = cc.engine
+    cc.level_shift, cc.nvir_diis = 0.1, 2
+    cc.kernel(eris=eris)
+    assert cc.engine is eng and eng.level_shift == 0.1 and eng.nvir_diis == 2
+    ref = e[:o, None] - e[None, o:] - 0.1
+    assert abs(eng.eia.download() - ref).max() < 1e-15
+    cc.triples_precision = "fp32"
+    cc.kernel(eris=eris)
+    assert cc.engine is not eng and cc.engine.mixed

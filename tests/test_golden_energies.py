@@ -33,6 +33,15 @@ def _h2o(basis, geometry):
     return mf
 
 
+_MF = {}
+
+
+def _mf_cached(key, build):
+    if key not in _MF:
+        _MF[key] = build()
+    return _MF[key]
+
+
 def _rccsdt(mf, conv_tol):
     from distr_cc.distribute_t3 import SINGLE_RANK
     from distr_cc.rccsdt import RCCSDT
@@ -60,11 +69,79 @@ def test_h2o_ccpvdz_q_emulated(monkeypatch):
 
 @pytest.mark.gpu
 def test_h2o_631g_rccsdt_gpu():
+    """The reference's known value (tests/test_rccsdt.py:26, its tolerance 5e-8 at conv_tol 1e-8),
+    here converged to 1e-10 and held to 1e-9."""
     from distr_cc import device
     device.init(0)
-    cc = _rccsdt(_h2o("6-31g", "sym"), 1e-8)
+    cc = _rccsdt(_h2o("6-31g", "sym"), 1e-10)
     print("E_corr 6-31G %.15f" % cc.e_corr, flush=True)
-    assert abs(cc.e_corr - E_RCCSDT_631G) < 5e-8
+    assert abs(cc.e_corr - E_RCCSDT_631G) < 1e-9
+
+
+# the option variants of the reference's tests/test_rccsdt.py:80-135,166-171 (batch_size,
+# do_diis_max_t, nvir_diis, no DIIS), each converged to 1e-10 and held to 1e-9 of the known value
+H2O_VARIANTS = [dict(batch_size=3), dict(batch_size=11), dict(batch_size=17), dict(do_diis_max_t=False),
+                dict(do_diis_max_t=True), dict(nvir_diis=4), dict(nvir_diis=1), dict(nvir_diis=13),
+                dict(diis=False)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("opts", H2O_VARIANTS, ids=lambda d: ",".join("%s=%s" % kv for kv in d.items()))
+def test_h2o_631g_options_gpu(opts):
+    from distr_cc import device
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT
+    device.init(0)
+    mf = _mf_cached("h2o_631g", lambda: _h2o("6-31g", "sym"))
+    cc = RCCSDT(mf, SINGLE_RANK)
+    cc.conv_tol, cc.verbose, cc.max_cycle = 1e-10, 0, 200
+    for k, val in opts.items():
+        setattr(cc, k, val)
+    cc.kernel()
+    print("%s: cycles %d E_corr %.15f" % (opts, cc.cycles, cc.e_corr), flush=True)
+    assert cc.converged and abs(cc.e_corr - E_RCCSDT_631G) < 1e-9
+
+
+def _restart_from_disk(mf, cc1, tmp_path, frozen=None, **attrs):
+    """tests/test_rccsdt.py:188-219 and examples/02_restart.py:38-68: t1/t2 as .npy files, T3 through
+    DistributedT3IJK.save_to_disk / load_from_disk, the SCF fields through a file, then one cycle."""
+    from types import SimpleNamespace
+    from distr_cc.distribute_t3 import SINGLE_RANK, DistributedT3IJK
+    from distr_cc.rccsdt import RCCSDT
+    prefix = str(tmp_path) + os.sep
+    np.save(prefix + "t1.npy", cc1.tamps[0])
+    np.save(prefix + "t2.npy", cc1.tamps[1])
+    np.savez(prefix + "hf.npz", e_tot=mf.e_tot, mo_energy=mf.mo_energy, mo_coeff=mf.mo_coeff, mo_occ=mf.mo_occ)
+    dt3, t3_local = cc1.tamps[2]
+    dt3.save_to_disk(t3_local, prefix)
+    chk = np.load(prefix + "hf.npz")
+    mf2 = SimpleNamespace(mol=mf.mol, e_tot=float(chk["e_tot"]), mo_energy=chk["mo_energy"],
+                          mo_coeff=chk["mo_coeff"], mo_occ=chk["mo_occ"], _eri=mf.mol.int2e(),
+                          get_hcore=lambda *a: sum(mf.mol.int1e()[1:]), energy_nuc=mf.mol.energy_nuc,
+                          converged=True)
+    cc2 = RCCSDT(mf2, SINGLE_RANK, frozen=frozen)
+    cc2.conv_tol, cc2.max_cycle, cc2.verbose = 1e-8, 1, 0
+    for k, val in attrs.items():
+        setattr(cc2, k, val)
+    t1, t2 = np.load(prefix + "t1.npy"), np.load(prefix + "t2.npy")
+    dt3, t3_local = DistributedT3IJK.load_from_disk(prefix, SINGLE_RANK, mmap_mode=None)
+    assert np.array_equal(t3_local, cc1.tamps[2][1])
+    cc2.kernel(tamps=[t1, t2, (dt3, t3_local)])
+    return cc2
+
+
+@pytest.mark.gpu
+def test_h2o_631g_restart_gpu(tmp_path):
+    """tests/test_rccsdt.py:173-219: converge, write the checkpoint files, read them back and restart
+    with max_cycle = 1: converged at once, at the known value."""
+    from distr_cc import device
+    device.init(0)
+    mf = _mf_cached("h2o_631g", lambda: _h2o("6-31g", "sym"))
+    cc1 = _rccsdt(mf, 1e-8)
+    cc2 = _restart_from_disk(mf, cc1, tmp_path)
+    print("restart E_corr %.15f" % cc2.e_corr, flush=True)
+    assert cc2.converged and cc2.cycles == 1
+    assert abs(cc2.e_corr - E_RCCSDT_631G) < 5e-9
 
 
 @pytest.mark.gpu
@@ -98,3 +175,112 @@ def test_c1_matches_cpu_oracle_gpu():
     print("C1 E_corr %.15f [Q] %.15f (Q) %.15f" % (cc.e_corr, eb, ep), flush=True)
     assert abs(cc.e_corr - ref["e_corr_rccsdt"]) < 1e-9
     assert abs(eb - ref["e_bracket_q"]) < 1e-9 and abs(ep - ref["e_paren_q"]) < 1e-9
+
+
+# examples/02_restart.py:21-25 and 00_rccsdt_q_water.py:12-19 (Angstrom)
+WATER_02 = [("O", (-0.066999140, 0.0, 1.494354740)), ("H", (0.815734270, 0.0, 1.865866390)),
+            ("H", (0.068855100, 0.0, 0.539142770))]
+WATER_DIMER_00 = WATER_02 + [("O", (0.062547750, 0.0, -1.422632080)),
+                             ("H", (-0.406965400, -0.760178410, -1.771744500)),
+                             ("H", (-0.406965400, 0.760178410, -1.771744500))]
+E_WATER_02 = (-0.2146745758026043, -0.000440859544, -0.000488539476)        # examples/02_restart.py:70,78
+E_N2 = -0.3217858674891447                                                   # distr_cc/rccsdt.py:1318-1325
+
+
+def _water_frozen_core_restart(tmp_path):
+    """examples/02_restart.py: water/cc-pVDZ with frozen=chemcore (max_cycle 100, reference defaults
+    otherwise), checkpoint files, a one-cycle restart (conv_tol 1e-8, batch_size 4), then (Q) at
+    blksize 8."""
+    from distr_cc import rccsdt_q
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.molecule import RHF, Molecule, chemcore
+    from distr_cc.rccsdt import RCCSDT
+    mol = Molecule(WATER_02, "cc-pvdz")
+    mf = RHF(mol, conv_tol=1e-12)
+    mf.kernel()
+    cc1 = RCCSDT(mf, SINGLE_RANK, frozen=chemcore(mol))
+    cc1.max_cycle, cc1.verbose = 100, 0
+    cc1.kernel()
+    assert cc1.converged and cc1.nocc == 4 and cc1.nmo == 23
+    cc2 = _restart_from_disk(mf, cc1, tmp_path, frozen=chemcore(mol), batch_size=4)
+    eb, ep = rccsdt_q.kernel(cc2, comm=SINGLE_RANK, blksize=8)
+    print("water frozen core: E_corr %.15f restart %.15f [Q] %.14f (Q) %.14f" % (cc1.e_corr, cc2.e_corr, eb, ep),
+          flush=True)
+    return cc1, cc2, eb, ep
+
+
+def _check_water_02(cc2, eb, ep):
+    # the reference's value comes from its default conv_tol 1e-7 run plus the one-cycle restart;
+    # [Q]/(Q) are printed to 12 decimals
+    assert cc2.converged and abs(cc2.e_corr - E_WATER_02[0]) < 1e-8
+    assert abs(eb - E_WATER_02[1]) < 5e-10 and abs(ep - E_WATER_02[2]) < 5e-10
+
+
+def _n2():
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.molecule import RHF, Molecule, chemcore
+    from distr_cc.rccsdt import RCCSDT
+    mol = Molecule([("N", (0., 0., 0.)), ("N", (0., 0., 1.1))], "cc-pvdz")
+    mf = RHF(mol, conv_tol=1e-14, max_cycle=1000)
+    mf.kernel()
+    cc = RCCSDT(mf, SINGLE_RANK, frozen=chemcore(mol))
+    cc.conv_tol, cc.conv_tol_normt, cc.max_cycle, cc.batch_size, cc.nvir_diis = 1e-10, 1e-8, 100, 11, 7
+    cc.verbose = 0
+    cc.kernel()
+    print("N2 frozen core: E_corr %.16f" % cc.e_corr, flush=True)
+    assert cc.converged and cc.nocc == 5 and cc.nmo == 26
+    return cc
+
+
+def test_water_frozen_core_restart_q_emulated(monkeypatch, tmp_path):
+    abi_emulator.install(monkeypatch)
+    _check_water_02(*_water_frozen_core_restart(tmp_path)[1:])
+
+
+def test_n2_frozen_core_emulated(monkeypatch):
+    abi_emulator.install(monkeypatch)
+    assert abs(_n2().e_corr - E_N2) < 1e-9
+
+
+@pytest.mark.gpu
+def test_water_frozen_core_restart_q_gpu(tmp_path):
+    from distr_cc import device
+    device.init(0)
+    _check_water_02(*_water_frozen_core_restart(tmp_path)[1:])
+
+
+@pytest.mark.gpu
+def test_n2_frozen_core_gpu():
+    from distr_cc import device
+    device.init(0)
+    assert abs(_n2().e_corr - E_N2) < 1e-9
+
+
+E_DIMER_00 = (-0.431122120377, -0.00087424734798, -0.00099299809898)   # examples/00_rccsdt_q_water.py:58,62
+
+
+@pytest.mark.gpu
+def test_water_dimer_frozen_core_q_gpu():
+    """examples/00_rccsdt_q_water.py: water dimer/cc-pVDZ, frozen=chemcore (nocc 8, nvir 38), RHF
+    conv_tol 1e-12, RCCSDT conv_tol 1e-9 / conv_tol_normt 1e-7 with nvir_diis 13 and batch_size 11,
+    then (Q) at blksize 4 summed over 4 jobs; the reference prints its values to 12 and 14 decimals."""
+    from distr_cc import device, rccsdt_q
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.molecule import RHF, Molecule, chemcore
+    from distr_cc.rccsdt import RCCSDT
+    device.init(0)
+    mol = Molecule(WATER_DIMER_00, "cc-pvdz")
+    mf = RHF(mol, conv_tol=1e-12)
+    mf.kernel()
+    cc = RCCSDT(mf, SINGLE_RANK, frozen=chemcore(mol))
+    cc.max_cycle, cc.conv_tol, cc.conv_tol_normt, cc.verbose = 100, 1e-9, 1e-7, 0
+    cc.batch_size, cc.diis, cc.do_diis_max_t, cc.nvir_diis = 11, True, True, 13
+    cc.kernel()
+    assert cc.converged and cc.nocc == 8 and cc.nmo == 46
+    eb = ep = 0.0
+    for i in range(4):
+        b, p = rccsdt_q.kernel(cc, comm=SINGLE_RANK, blksize=4, job_idx=i, n_jobs=4)
+        eb, ep = eb + b, ep + p
+    print("water dimer frozen core: E_corr %.15f [Q] %.15f (Q) %.15f" % (cc.e_corr, eb, ep), flush=True)
+    assert abs(cc.e_corr - E_DIMER_00[0]) < 1e-9
+    assert abs(eb - E_DIMER_00[1]) < 5e-10 and abs(ep - E_DIMER_00[2]) < 5e-10

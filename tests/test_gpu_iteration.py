@@ -215,3 +215,78 @@ def test_kernel_with_autotuned_tiles(dev):
             assert any(t is not None and t >= 0 for t in cc.engine.tr.tile_config())
     _say("E default %.14f tuned %.14f" % (res[False], res[True]))
     assert abs(res[False] - res[True]) < 1e-11
+
+
+def test_init_amps_level_shift_vs_oracle(dev):
+    """RCCSDT.init_amps with level_shift = 0.05 against the oracle's init_amps_rhf (unshifted
+    denominators, rccsdt.py:197-251); t1 != 0 through a non-canonical f_ov."""
+    from ccsdt_cases import DuckMF, molecule_like, with_fov
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 4, 9
+    _say("init_amps o=%d v=%d level_shift 0.05" % (o, v))
+    F, g, e, _ = molecule_like(o, v, 51)
+    F = with_fov(F, o, 0.02, 52)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.level_shift, cc.verbose = 0.05, 0
+    emp2, tamps = cc.init_amps(_PhysicistsERIs(F, e, g))
+    e_ref, t1, t2 = tri.init_amps_rhf(o, F, e, g)
+    assert abs(tamps[0] - t1).max() <= 1e-14 * abs(t1).max()
+    assert abs(tamps[1] - t2).max() <= 1e-14 * abs(t2).max()
+    assert abs(emp2 - e_ref) <= 1e-13 * abs(e_ref) and not tamps[2][1].any()
+
+
+@pytest.mark.parametrize("with_t3,nvir_diis,diis,shift", [(True, None, True, 0.0), (True, 3, True, 0.05),
+                                                         (False, None, True, 0.0), (False, 3, True, 0.05),
+                                                         (True, None, False, 0.0)])
+def test_kernel_diis_trajectory_vs_oracle(dev, with_t3, nvir_diis, diis, shift):
+    """Five cycles of RCCSDT.kernel against five cycles of the oracle loop with the oracle DIIS:
+    unified T1/T2/T3 DIIS with and without an nvir_diis subspace (run_diis_t3, rccsdt.py:1026-1067),
+    standard T1/T2 DIIS (do_diis_max_t = False, rccsdt.py:1096-1099,1119-1122) and none."""
+    from ccsdt_cases import DuckMF, molecule_like, oracle_kernel, with_fov
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT, _PhysicistsERIs
+    o, v = 4, 7
+    _say("DIIS trajectory with_t3=%s nvir_diis=%s diis=%s shift=%g" % (with_t3, nvir_diis, diis, shift))
+    F, g, e, _ = molecule_like(o, v, 61)
+    F = with_fov(F, o, 0.01, 62)
+    cc = RCCSDT(DuckMF(o, v, e), SINGLE_RANK)
+    cc.do_diis_max_t, cc.nvir_diis, cc.diis, cc.level_shift = with_t3, nvir_diis, diis, shift
+    cc.max_cycle, cc.verbose = 5, 0
+    e_cc, tamps = cc.kernel(eris=_PhysicistsERIs(F, e, g))
+    t1, t2, t3, energies = oracle_kernel(o, F, g, e, 5, level_shift=shift, nvir_diis=nvir_diis, diis=diis,
+                                         with_t3=with_t3)
+    assert cc.cycles == 5 and abs(e_cc - energies[-1]) <= 1e-12
+    for a, b in ((tamps[0], t1), (tamps[1], t2), (tamps[2][1], t3)):
+        assert abs(a - b).max() <= 1e-10 * abs(b).max()
+
+
+def test_device_diis_vs_oracle(dev):
+    """DeviceDIIS step by step against oracle.diis (distr_cc/diis.py:105-177): B matrix, coefficients
+    and extrapolated vectors, through a wrap of the history (space 4, 9 steps)."""
+    from distr_cc.device import DeviceArray
+    from distr_cc.diis import DeviceDIIS
+    from oracle.diis import DIIS
+    n = 200003
+    _say("DeviceDIIS n=%d" % n)
+    rng = np.random.default_rng(71)
+    dd, od = DeviceDIIS(n, space=4), DIIS(space=4)
+    x = DeviceArray((n,))
+    vec = rng.standard_normal(n)
+    for step in range(9):
+        vec = 0.7 * vec + 0.3 ** step * rng.standard_normal(n)
+        x.upload(vec)
+        got = dd.update(x)
+        want = od.update(vec)
+        g = got.download()
+        got.free()
+        assert abs(g - want).max() <= 1e-12 * abs(want).max()
+        if step > 0:
+            # B-matrix entries are dot products summed in another order: compare each against the
+            # Cauchy-Schwarz scale sqrt(B_ii B_jj) of its terms
+            nd = min(step, 4)
+            hd, ho = dd._H[1:nd + 1, 1:nd + 1], od._H[1:nd + 1, 1:nd + 1]
+            scale = np.sqrt(np.outer(np.diag(ho), np.diag(ho)))
+            assert np.all(np.abs(hd - ho) <= 1e-12 * scale)
+            cd, co = dd.coefficients(nd), od.coefficients(nd)
+            assert np.abs(cd - co).max() <= 1e-9 * np.abs(co).max()

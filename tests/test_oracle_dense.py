@@ -74,3 +74,16 @@ def test_denominator_closed_form():
         for a, b, c in [(0, 1, 2), (3, 3, 0), (2, 0, 1)]:
             d = e[i] + e[j] + e[k] - e[o + a] - e[o + b] - e[o + c] - 0.3
             assert abs(out[n, a, b, c] - r[n, a, b, c] / d) < 1e-15 * abs(r[n, a, b, c] / d) + 1e-300
+
+
+@pytest.mark.parametrize("o,v,bs", [(5, 4, 100), (6, 3, 7)])
+def test_r3_target_subset_equals_full(o, v, bs):
+    """compute_r3_tri(targets=S) (which skips the source blocks and W_oooo/F_oo rows that meet no
+    sampled target) equals the rows S of the full residual; the other rows stay zero."""
+    s, im, t2, t3 = _system(o, v, 5 * o + v)
+    full = tri.compute_r3_tri(o, v, im, t2, t3, batch_size=bs)
+    S = [0, 3, len(t3) // 2, len(t3) - 1]
+    sub = tri.compute_r3_tri(o, v, im, t2, t3, batch_size=bs, targets=S)
+    assert abs(sub[S] - full[S]).max() <= 1e-14 * abs(full).max()
+    rest = np.setdiff1d(np.arange(len(t3)), S)
+    assert not sub[rest].any()
