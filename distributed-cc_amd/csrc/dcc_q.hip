@@ -36,6 +36,75 @@ __global__ void t3_to_abc_kernel(const double *__restrict__ t3, int no, int nv, 
     out[t] = val;
 }
 
+// Canonical index of a sorted triple x <= y <= z of n values, lexicographic (distribute_t3.py:51-60,
+// the same closed form for occupied and virtual triples).
+__device__ __forceinline__ int64_t tri_lin(int64_t n, int64_t x, int64_t y, int64_t z) {
+    const int64_t m = n - x;
+    return (n * (n + 1) * (n + 2) - m * (m + 1) * (m + 2)) / 6 + (y - x) * (2 * n - y - x + 1) / 2 + (z - y);
+}
+
+// Inverse of tri_lin (x <= y <= z < n) by counting.
+__device__ __forceinline__ void tri_decode(int64_t n, int64_t lin, int &x, int &y, int &z) {
+    x = 0;
+    while (x + 1 < n && tri_lin(n, x + 1, x + 1, x + 1) <= lin) ++x;
+    y = x;
+    while (y + 1 < n && tri_lin(n, x, y + 1, y + 1) <= lin) ++y;
+    z = (int)(lin - tri_lin(n, x, y, y)) + y;
+}
+
+// Sort three keys ascending, carrying three values along (ties keep their order).
+__device__ __forceinline__ void sort3_carry(int k[3], int w[3]) {
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int u = 0; u < 2 - s; ++u)
+            if (k[u] > k[u + 1]) {
+                int t = k[u]; k[u] = k[u + 1]; k[u + 1] = t;
+                t = w[u]; w[u] = w[u + 1]; w[u + 1] = t;
+            }
+}
+
+// IJK -> ABC: out[lin(a,b,c), i, j, k] = T[ijk][abc] for a <= b <= c and every ordered (i,j,k), from the
+// canonical i <= j <= k blocks.  One workgroup per virtual triple; threads run over (i,j,k), so every
+// output row of o^3 values is written contiguously.
+__global__ __launch_bounds__(256) void t3_tri_to_abc_packed_kernel(const double *__restrict__ t3, int no, int nv,
+                                                                   double *__restrict__ out) {
+    __shared__ int sabc[3];
+    const int64_t lin = blockIdx.x;
+    if (threadIdx.x == 0) tri_decode(nv, lin, sabc[0], sabc[1], sabc[2]);
+    __syncthreads();
+    const int64_t o = no, v = nv, o3 = o * o * o;
+    for (int64_t t = threadIdx.x; t < o3; t += blockDim.x) {
+        int occ[3] = {(int)(t / (o * o)), (int)((t / o) % o), (int)(t % o)};
+        int vir[3] = {sabc[0], sabc[1], sabc[2]};
+        sort3_carry(occ, vir);
+        out[lin * o3 + t] = t3[tri_lin(o, occ[0], occ[1], occ[2]) * v * v * v + ((int64_t)vir[0] * v + vir[1]) * v +
+                               vir[2]];
+    }
+}
+
+// ABC -> dense: out[a,b,c,i,j,k] (a,b,c < vpad) = T_{ijk}^{abc} for a,b,c < nvir (0 in the padding) from
+// the ABC array t3_abc[lin(a<=b<=c), i, j, k]; (a,b,c) is sorted carrying (i,j,k) along.  One workgroup
+// per (a,b,c) of the dense output.
+__global__ __launch_bounds__(256) void t3_abc_packed_to_dense_kernel(const double *__restrict__ t3_abc, int no,
+                                                                     int nv, int vp, double *__restrict__ out) {
+    const int64_t abc = blockIdx.x;
+    const int a = (int)(abc / ((int64_t)vp * vp)), b = (int)((abc / vp) % vp), c = (int)(abc % vp);
+    const int64_t o = no, o3 = o * o * o;
+    double *dst = out + abc * o3;
+    if (a >= nv || b >= nv || c >= nv) {
+        for (int64_t t = threadIdx.x; t < o3; t += blockDim.x) dst[t] = 0.0;
+        return;
+    }
+    int key[3] = {a, b, c}, pos[3] = {0, 1, 2};
+    sort3_carry(key, pos);
+    const double *src = t3_abc + tri_lin(nv, key[0], key[1], key[2]) * o3;
+    for (int64_t t = threadIdx.x; t < o3; t += blockDim.x) {
+        const int y[3] = {(int)(t / (o * o)), (int)((t / o) % o), (int)(t % o)};
+        dst[t] = src[((int64_t)y[pos[0]] * o + y[pos[1]]) * o + y[pos[2]]];
+    }
+}
+
 // factor of the restricted a<=b<=c<=d sum (distr_cc/rccsdt_q.py:410-428)
 __device__ __forceinline__ double q_factor(int a, int b, int c, int d) {
     if (a < b && b < c && c < d) return 24.0;
@@ -138,6 +207,32 @@ int dcc_t3_to_abc_f64(const double *t3_tri, int64_t nocc, int64_t nvir, int64_t 
     DCC_CHECK((total + 255) / 256 < ((int64_t)1 << 31), "grid too large");
     hipLaunchKernelGGL(t3_to_abc_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                        dcc::as_stream(stream), t3_tri, (int)nocc, (int)nvir, (int)vpad, out);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+int dcc_t3_tri_to_abc_packed_f64(const double *t3_tri, int64_t nocc, int64_t nvir, double *t3_abc,
+                                 dcc_stream_t stream) {
+    DCC_CHECK(nocc >= 0 && nvir >= 0, "bad extents");
+    const int64_t nabc = nvir * (nvir + 1) * (nvir + 2) / 6;
+    if (nabc == 0 || nocc == 0) return 0;
+    DCC_CHECK(t3_tri != nullptr && t3_abc != nullptr, "null pointer");
+    DCC_CHECK(nabc < ((int64_t)1 << 31), "grid too large");
+    hipLaunchKernelGGL(t3_tri_to_abc_packed_kernel, dim3((unsigned)nabc), dim3(256), 0, dcc::as_stream(stream),
+                       t3_tri, (int)nocc, (int)nvir, t3_abc);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
+int dcc_t3_abc_packed_to_dense_f64(const double *t3_abc, int64_t nocc, int64_t nvir, int64_t vpad, double *out,
+                                   dcc_stream_t stream) {
+    DCC_CHECK(nocc >= 0 && nvir >= 0 && vpad >= nvir, "bad extents");
+    const int64_t n = vpad * vpad * vpad;
+    if (n == 0 || nocc == 0) return 0;
+    DCC_CHECK(t3_abc != nullptr && out != nullptr, "null pointer");
+    DCC_CHECK(n < ((int64_t)1 << 31), "grid too large");
+    hipLaunchKernelGGL(t3_abc_packed_to_dense_kernel, dim3((unsigned)n), dim3(256), 0, dcc::as_stream(stream),
+                       t3_abc, (int)nocc, (int)nvir, (int)vpad, out);
     DCC_LAUNCH_CHECK();
     return 0;
 }

@@ -6,7 +6,8 @@ contractions write disjoint outputs (each system's own R), so they run as ONE pe
 GEMM launch (dcc_contract_grouped_f64) per term; the bandwidth passes (Y gather, W_oooo/F_oo
 permuted adds, fused finalize) stay per system.  ``capture()`` records a whole step -- the residual
 operands from the intermediates, R reset, Y gather, all grouped GEMMs, all finalizes -- into one HIP
-graph, so a step is one graph launch.
+graph, so a step is one graph launch.  ``BatchedRCCSDT`` does the same for whole RCCSDT iterations
+(lower order, T3 -> r1/r2/W, triples residual, update and energy of every system).
 ``autotune()`` times the eight tile variants of the grouped kernel for every term and keeps the fastest.
 """
 import ctypes
@@ -59,21 +60,20 @@ class _GroupedPhase:
                stream.handle if stream is not None else None)
 
 
-class BatchedTriples:
-    """FP64 triples updates of an ensemble ``systems = [(nocc, nvir), ...]`` on one stream."""
+class _Grouped:
+    """Grouping of the triples residual launches of several TriplesEngines (launch p of every engine is the
+    same term), the graph of one step, and the tile autotuner of the grouped launches."""
 
-    def __init__(self, systems, stream, device=0):
+    def _group(self, tr_engines, stream):
         if stream is None:
-            raise ValueError("BatchedTriples needs a non-default stream (graph capture)")
+            raise ValueError("a batched ensemble needs a non-default stream (graph capture)")
         self.stream = stream
-        self.systems = [(int(o), int(v)) for o, v in systems]
-        self.engines = [TriplesEngine(o, v, stream=stream, device=device) for o, v in self.systems]
-        nl = len(self.engines[0].launches)
-        if any(len(e.launches) != nl for e in self.engines):
+        nl = len(tr_engines[0].launches)
+        if any(len(e.launches) != nl for e in tr_engines):
             raise ValueError("launch lists differ between systems")
         self.phases = []
         for p in range(nl):
-            items = [e.launches[p] for e in self.engines]
+            items = [e.launches[p] for e in tr_engines]
             if all(isinstance(g, GemmLaunch) for g in items):
                 ph = _GroupedPhase(items)
                 ph.prepare(0, stream)
@@ -82,27 +82,13 @@ class BatchedTriples:
                 self.phases.append(items)
         self.graph = None
 
-    def set_system(self, s, t3, imds, t2, mo_energy, level_shift=0.0, nvir_diis=None):
-        e = self.engines[s]
-        e.set_t3(t3)
-        e.set_intermediates(imds, t2)
-        e.set_denominators(mo_energy, level_shift, nvir_diis)
-
-    def issue_step(self):
-        """Issue one triples update of every system on the stream (no host synchronisation)."""
-        st = self.stream
-        for e in self.engines:
-            e.prepare_residual()
-            e.R.fill(0.0, st)
-            e.materialize_y()
+    def _run_phases(self):
         for ph in self.phases:
             if isinstance(ph, _GroupedPhase):
-                ph.run(st)
+                ph.run(self.stream)
             else:
                 for g in ph:
-                    g.run(st)
-        for e in self.engines:
-            e.finalize_launch()
+                    g.run(self.stream)
 
     def capture(self):
         if self.graph is not None:
@@ -118,10 +104,6 @@ class BatchedTriples:
             L.call("dcc_graph_launch", self.graph, self.stream.handle)
         else:
             self.issue_step()
-
-    def norms(self):
-        self.stream.synchronize()
-        return [e.read_norms() for e in self.engines]
 
     def autotune(self, reps=2):
         """Pick the tile shape of every grouped phase by timing; returns {phase: (shape, ms)}.
@@ -156,3 +138,72 @@ class BatchedTriples:
         if self.graph is not None:
             L.call("dcc_graph_destroy", self.graph)
             self.graph = None
+
+
+class BatchedTriples(_Grouped):
+    """FP64 triples updates of an ensemble ``systems = [(nocc, nvir), ...]`` on one stream."""
+
+    def __init__(self, systems, stream, device=0):
+        self.systems = [(int(o), int(v)) for o, v in systems]
+        self.engines = [TriplesEngine(o, v, stream=stream, device=device) for o, v in self.systems]
+        self._group(self.engines, stream)
+
+    def set_system(self, s, t3, imds, t2, mo_energy, level_shift=0.0, nvir_diis=None):
+        e = self.engines[s]
+        e.set_t3(t3)
+        e.set_intermediates(imds, t2)
+        e.set_denominators(mo_energy, level_shift, nvir_diis)
+
+    def issue_step(self):
+        """Issue one triples update of every system on the stream (no host synchronisation)."""
+        st = self.stream
+        for e in self.engines:
+            e.prepare_residual()
+            e.R.fill(0.0, st)
+            e.materialize_y()
+        self._run_phases()
+        for e in self.engines:
+            e.finalize_launch()
+
+    def norms(self):
+        self.stream.synchronize()
+        return [e.read_norms() for e in self.engines]
+
+
+class BatchedRCCSDT(_Grouped):
+    """Full FP64 RCCSDT iterations of an ensemble of systems on one stream: every system has its own
+    RCCSDTEngine (T1 dressing, lower order, T3 -> r1/r2/W, triples residual, update, energy), the
+    triples residual contractions of all systems run as one grouped persistent launch per term, and a
+    whole iteration of the ensemble -- followed by every system's energy -- is one HIP graph.
+
+    ``systems`` is a list of (nocc, nvir, fock, pppp, mo_energy).  No DIIS (its B-matrix solve is a host
+    step per iteration): the iteration is the Jacobi update of update_amps_rccsdt_tri_
+    (distr_cc/rccsdt.py:890-977)."""
+
+    def __init__(self, systems, stream, level_shift=0.0, device=0):
+        from distr_cc.rccsdt_engine import RCCSDTEngine
+        self.systems = [(int(o), int(v)) for o, v, _, _, _ in systems]
+        self.engines = [RCCSDTEngine(o, v, F, g, e, level_shift=level_shift, stream=stream, device=device)
+                        for o, v, F, g, e in systems]
+        for e in self.engines:
+            e.tr.set_denominators(e.mo_energy, e.level_shift, e.nvir_diis)
+        self._group([e.tr for e in self.engines], stream)
+
+    def set_amplitudes(self, s, t1, t2, t3):
+        self.engines[s].set_amplitudes(t1, t2, t3)
+
+    def issue_step(self):
+        """One iteration of every system and then its energy, with no host synchronisation."""
+        st = self.stream
+        for e in self.engines:
+            e.issue_lower()
+            e.tr.R.fill(0.0, st)
+            e.tr.materialize_y()
+        self._run_phases()
+        for e in self.engines:
+            e.tr.finalize_launch()
+            e.energy_launch()
+
+    def energies(self):
+        self.stream.synchronize()
+        return [e.read_energy() for e in self.engines]

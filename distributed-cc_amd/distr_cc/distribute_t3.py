@@ -178,3 +178,89 @@ class DistributedT3IJK:
             full[order] = t3
             t3 = full
         return obj, t3
+
+
+class _AbcIndex:
+    """Mapping (a, b, c) with a <= b <= c -> lexicographic canonical index over nvir (the closed form of
+    ijk_to_linear), the abc_to_global_idx table of distribute_t3.py:1130-1133 without storing it."""
+
+    def __init__(self, nvir):
+        self.nvir = int(nvir)
+
+    def __getitem__(self, abc):
+        a, b, c = (int(x) for x in abc)
+        if not 0 <= a <= b <= c < self.nvir:
+            raise KeyError(abc)
+        return ijk_to_linear(a, b, c, self.nvir)
+
+    def get(self, abc, default=None):
+        try:
+            return self[abc]
+        except KeyError:
+            return default
+
+    def __contains__(self, abc):
+        return self.get(abc) is not None
+
+    def __len__(self):
+        return nocc3(self.nvir)
+
+
+class DistributedT3ABC:
+    """T3 stored as t3_local[lin(a<=b<=c), i, j, k] = T_{ijk}^{abc} for every ordered (i,j,k), at one rank
+    (reference distribute_t3.py:1098-1221: with one rank every virtual triple is local, in lexicographic
+    a<=b<=c order)."""
+
+    def __init__(self, nocc, nvir, comm=None, blksize_abc=None, dtype=np.float64):
+        self.nocc, self.nvir = int(nocc), int(nvir)
+        self.dtype = np.dtype(dtype)
+        self.comm = comm if comm is not None else SINGLE_RANK
+        if self.comm.Get_size() != 1:
+            raise ValueError("this build runs the triples path on one GPU; comm size must be 1")
+        self.rank, self.size = 0, 1
+        self.blksize_abc = 1 if blksize_abc is None else int(blksize_abc)
+        self.n_abc_triplets = self.nvir3 = nocc3(self.nvir)
+        self.local_nvir3 = self.n_abc_triplets
+        self.abc_to_global_idx = _AbcIndex(self.nvir)
+
+    @property
+    def local_abc_triplets(self):
+        return [tuple(int(x) for x in t) for t in _triples_table(self.nvir)]
+
+    def get_owner_abc(self, a, b, c):
+        return 0
+
+    def abc_idx_to_abc(self, abc_idx):
+        if 0 <= abc_idx < self.n_abc_triplets:
+            return tuple(int(x) for x in _triples_table(self.nvir)[abc_idx])
+        raise ValueError(f"Invalid abc_idx: {abc_idx}")
+
+    def get_local_index(self, a, b, c, i, j, k):
+        """Flat index of T_{ijk}^{abc} in t3_local (distribute_t3.py:1192-1199)."""
+        off = self.abc_to_global_idx[tuple(sorted((a, b, c)))]
+        return off * self.nocc ** 3 + (i * self.nocc + j) * self.nocc + k
+
+    def allocate_local(self):
+        return np.zeros((self.local_nvir3, self.nocc, self.nocc, self.nocc), dtype=self.dtype)
+
+    def memory_usage_gb(self):
+        return self.local_nvir3 * self.nocc ** 3 * self.dtype.itemsize / (1024 ** 3)
+
+
+def redistribute_ijk_to_abc(dt3_ijk, t3_local_ijk, comm=None, blksize_abc=None, chunk_size=None,
+                            log_redistribution=True, verbose=None, stdout=None):
+    """IJK -> ABC layout (distribute_t3.py:1544-1642) as one device gather (dcc_t3_tri_to_abc_packed_f64);
+    returns (DistributedT3ABC, t3_local_abc) on the host, like the reference."""
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    o, v = dt3_ijk.nocc, dt3_ijk.nvir
+    dt3_abc = DistributedT3ABC(o, v, comm, blksize_abc=blksize_abc)
+    src = DeviceArray.from_host(np.ascontiguousarray(t3_local_ijk, dtype=np.float64))
+    dst = DeviceArray((dt3_abc.local_nvir3, o, o, o))
+    try:
+        L.call("dcc_t3_tri_to_abc_packed_f64", src.vp, o, v, dst.vp, None)
+        out = dst.download()
+    finally:
+        src.free()
+        dst.free()
+    return dt3_abc, out

@@ -155,8 +155,11 @@ class QEngine:
             chunks.append((nt, launches, tk))
         return z4, t4, part, tot, chunks
 
-    def kernel(self, pppp, t2_oovv, t3_dev, mo_energy, job_idx=0, n_jobs=1):
-        """(e_[Q], e_(Q)) for the canonical-block T3 already on the device (``t3_dev``)."""
+    def kernel(self, pppp, t2_oovv, t3_dev, mo_energy, job_idx=0, n_jobs=1, t3_layout="ijk"):
+        """(e_[Q], e_(Q)) for the T3 already on the device (``t3_dev``): canonical i<=j<=k blocks
+        (t3_layout "ijk") or the ABC array t3_abc[lin(a<=b<=c), i, j, k] ("abc")."""
+        if t3_layout not in ("ijk", "abc"):
+            raise ValueError("t3_layout must be 'ijk' or 'abc'")
         o, v, vp, bs = self.o, self.v, self.vp, self.bs
         if self.dev is None:
             self._setup()
@@ -174,7 +177,10 @@ class QEngine:
         dev['t2'].upload(_pad(t2q, (0, 1), vp, o), st)
         dev['e_occ'].upload(np.ascontiguousarray(e[:o]), st)
         dev['e_vir'].upload(np.concatenate([e[o:], np.zeros(vp - v)]), st)
-        L.call("dcc_t3_to_abc_f64", t3_dev.vp, o, v, vp, dev['T'].vp, self._s())
+        if t3_layout == "ijk":
+            L.call("dcc_t3_to_abc_f64", t3_dev.vp, o, v, vp, dev['T'].vp, self._s())
+        else:
+            L.call("dcc_t3_abc_packed_to_dense_f64", t3_dev.vp, o, v, vp, dev['T'].vp, self._s())
         for g in self.wplans:
             g.run(st)
         key = (int(job_idx), int(n_jobs))

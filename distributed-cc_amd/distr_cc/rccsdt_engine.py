@@ -178,6 +178,7 @@ class RCCSDTEngine:
         self._setup_constants()
         tr._setup_lower()
         self.prog = self._build_program()
+        self._eprog = self._build_energy_program()   # planned here: its launch tables are uploaded once
 
     # ------------------------------------------------------------------ views
     def gview(self, buf, letters):
@@ -370,11 +371,13 @@ class RCCSDTEngine:
         s = self.stream
         return self.t1.download(stream=s), self.t2.download(stream=s), self.tr.get_t3()
 
-    def update_amps(self, timer=None):
+    def update_amps(self, timer=None, launch_timer=None):
         """One iteration of update_amps_rccsdt_tri_ (rccsdt.py:890-977) on the device.
 
         Returns [|r1|, |r2|, |r3|] (the res_norm of :929 and :966).  ``timer`` (optional) is
-        called as timer(phase, 'start'|'end') around each phase, e.g. to record HIP events."""
+        called as timer(phase, 'start'|'end') around each phase, ``launch_timer`` as
+        launch_timer(launch, 'start'|'end') around each contraction of the triples residual, e.g. to
+        record HIP events."""
         st = self.stream
         tick = timer if timer is not None else (lambda name, what: None)
         phases = [("dress", lambda: self.prog["dress"].run(st)),
@@ -383,7 +386,7 @@ class RCCSDTEngine:
                   ("triples intermediates", lambda: self.prog["imds3"].run(st)),
                   ("T3->W", lambda: self.tr.t3_to_w(reuse_z=True)),
                   ("ring/oooo operands", self.tr.prepare_residual),
-                  ("triples residual", self.tr.residual)]
+                  ("triples residual", lambda: self.tr.residual(timer=launch_timer))]
         for name, fn in phases:
             tick(name, "start")
             fn()
@@ -397,22 +400,41 @@ class RCCSDTEngine:
 
     def energy(self):
         """E_corr = sum (2<ij|ab> - <ij|ba>)(t2 + t1 t1) + 2 sum f_ia t1_ia (rccsdt.py:218-221 closed form)."""
+        self.energy_launch()
+        return self.read_energy()
+
+    def read_energy(self):
+        """The energy of the last energy_launch (downloads two device scalars)."""
+        sc = self.scal.download(stream=self.stream)
+        return float(sc[2] + 2.0 * sc[3])
+
+    def issue_lower(self):
+        """Device work of update_amps before the triples residual, with no host synchronisation (so it
+        can be captured in a HIP graph): T1 dressing, lower-order residual, T3 -> r1/r2 and the t1/t2
+        update, the T2-dressed triples intermediates, T3 -> W, the residual's operands."""
+        st = self.stream
+        for name in ("dress", "r1r2", "update12", "imds3"):
+            self.prog[name].run(st)
+        self.tr.t3_to_w(reuse_z=True)
+        self.tr.prepare_residual()
+
+    def _build_energy_program(self):
         o, v = self.nocc, self.nvir
         st = self.stream
-        if not hasattr(self, "_eprog"):
-            p = _Program(self)
-            tau = self.nview(self.tau, "ijab")
-            p.axpby(tau, Operand(self.t2, "ijab", (o, o, v, v)), "ijab", 1.0, 0.0)
-            p.ein("tau", "ia,jb->ijab", self.nview(self.t1, "ia"), self.nview(self.t1, "jb"), tau, 1.0, 1.0)
-            sh = lambda: st.handle if st is not None else None
-            p.call(lambda: L.call("dcc_dot_f64", self.Loovv.vp, self.tau.vp, self.tau.size, self.partials.vp,
-                                  self.scal.offset_ptr(2), sh()))
-            p.call(lambda: L.call("dcc_dot_f64", self.fov0.vp, self.t1.vp, self.t1.size, self.partials.vp,
-                                  self.scal.offset_ptr(3), sh()))
-            self._eprog = p
-        self._eprog.run(st)
-        sc = self.scal.download(stream=st)
-        return float(sc[2] + 2.0 * sc[3])
+        p = _Program(self)
+        tau = self.nview(self.tau, "ijab")
+        p.axpby(tau, Operand(self.t2, "ijab", (o, o, v, v)), "ijab", 1.0, 0.0)
+        p.ein("tau", "ia,jb->ijab", self.nview(self.t1, "ia"), self.nview(self.t1, "jb"), tau, 1.0, 1.0)
+        sh = lambda: st.handle if st is not None else None
+        p.call(lambda: L.call("dcc_dot_f64", self.Loovv.vp, self.tau.vp, self.tau.size, self.partials.vp,
+                              self.scal.offset_ptr(2), sh()))
+        p.call(lambda: L.call("dcc_dot_f64", self.fov0.vp, self.t1.vp, self.t1.size, self.partials.vp,
+                              self.scal.offset_ptr(3), sh()))
+        return p
+
+    def energy_launch(self):
+        """Issue the energy dot products into device scalars (no host synchronisation)."""
+        self._eprog.run(self.stream)
 
     # ------------------------------------------------------------- DIIS packing
     def diis_size(self, with_t3=True):

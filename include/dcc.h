@@ -270,6 +270,24 @@ int dcc_t3_to_abc_f64(const double *t3_tri, int64_t nocc, int64_t nvir, int64_t 
                       dcc_stream_t stream);
 
 /*
+ * IJK -> ABC at one GPU: t3_abc[lin(a,b,c), i, j, k] = T[ijk][abc] for every a <= b <= c (lexicographic
+ * canonical index over nvir) and every ordered (i,j,k), from the canonical i<=j<=k block array.  The
+ * layout of DistributedT3ABC.t3_local at one rank; replaces redistribute_ijk_to_abc
+ * (distr_cc/distribute_t3.py:1544-1642) and its packing helpers (:1228-1420).
+ */
+int dcc_t3_tri_to_abc_packed_f64(const double *t3_tri, int64_t nocc, int64_t nvir, double *t3_abc,
+                                 dcc_stream_t stream);
+
+/*
+ * out[a,b,c,i,j,k] (a,b,c < vpad) = T_{ijk}^{abc} for a,b,c < nvir and 0 in the padding, from an ABC
+ * array t3_abc[lin(a<=b<=c), i, j, k] (the ABC-prepared tamps rccsdt_q.kernel accepts,
+ * distr_cc/rccsdt_q.py:63-96); the per-task gather of fill_t3_from_ptr_array (src/rccsdt_q.c:665-828)
+ * for all tasks at once.
+ */
+int dcc_t3_abc_packed_to_dense_f64(const double *t3_abc, int64_t nocc, int64_t nvir, int64_t vpad, double *out,
+                                   dcc_stream_t stream);
+
+/*
  * Per (task, a, b, c, d) of a chunk of (Q) tasks (tasks: int32 (ntask, 4) block
  * starts a0,b0,c0,d0): t4 <- (t4 + z4) / D with D -> 0 when |D| <= 1e-15,
  * P4_444 on the occupied indices, times the 24/12/6/0 factor; then

@@ -109,3 +109,27 @@ def abc_to_linear(a, b, c, nvir):
     idx += (b - a) * (2 * nvir - a - b + 1) // 2
     idx += c - b
     return idx
+
+
+def t3_abc_from_tri(t3_local, nocc):
+    """IJK -> ABC at one rank, as the reference's packing loop (distr_cc/distribute_t3.py:1228-1420):
+    for every canonical block (i,j,k) and each of the six simultaneous permutations p of (a,b,c) and
+    (i,j,k), the entries whose permuted virtual triple is sorted are stored at
+    t3_abc[abc_to_linear(p(a,b,c)), p(i,j,k)]."""
+    nvir = t3_local.shape[-1]
+    out = np.zeros((nvir * (nvir + 1) * (nvir + 2) // 6, nocc, nocc, nocc), dtype=t3_local.dtype)
+    a, b, c = np.mgrid[0:nvir, 0:nvir, 0:nvir]
+    perms = [((a, b, c), (0, 1, 2)), ((a, c, b), (0, 2, 1)), ((b, a, c), (1, 0, 2)),
+             ((b, c, a), (1, 2, 0)), ((c, a, b), (2, 0, 1)), ((c, b, a), (2, 1, 0))]
+    maps = []
+    for (pa, pb, pc), order in perms:
+        mask = (pa <= pb) & (pb <= pc)
+        idx = np.array([abc_to_linear(x, y, z, nvir) for x, y, z in zip(pa[mask], pb[mask], pc[mask])],
+                       dtype=np.int64)
+        maps.append((mask, idx, order))
+    for n, (i, j, k) in enumerate(canonical_triples(nocc)):
+        blk = t3_local[n]
+        ijk = (i, j, k)
+        for mask, idx, order in maps:
+            out[idx, ijk[order[0]], ijk[order[1]], ijk[order[2]]] = blk[mask]
+    return out

@@ -449,6 +449,32 @@ class Emulator:
         Pd[...] = P.astype(dt)
         return 0
 
+    def dcc_t3_tri_to_abc_packed_f64(self, t3, nocc, nvir, out, stream):
+        o, v = int(nocc), int(nvir)
+        n3 = o * (o + 1) * (o + 2) // 6
+        T = self.dev.view(_ptr(t3), np.float64, n3 * v ** 3).reshape(n3, v, v, v)
+        lin = {t: n for n, t in enumerate((i, j, k) for i in range(o) for j in range(i, o) for k in range(j, o))}
+        full = np.zeros((v, v, v, o, o, o))
+        for i, j, k in itertools.product(range(o), repeat=3):
+            full[:, :, :, i, j, k] = T[lin[tuple(sorted((i, j, k)))]].transpose(_PERMS[_case(i, j, k)])
+        abc = [(a, b, c) for a in range(v) for b in range(a, v) for c in range(b, v)]
+        packed = np.array([full[a, b, c] for (a, b, c) in abc]).reshape(len(abc), o, o, o)
+        self.dev.view(_ptr(out), np.float64, packed.size)[:] = packed.ravel()
+        return 0
+
+    def dcc_t3_abc_packed_to_dense_f64(self, t3_abc, nocc, nvir, vpad, out, stream):
+        o, v, vp = int(nocc), int(nvir), int(vpad)
+        abc = [(a, b, c) for a in range(v) for b in range(a, v) for c in range(b, v)]
+        P = self.dev.view(_ptr(t3_abc), np.float64, len(abc) * o ** 3).reshape(len(abc), o, o, o)
+        full = np.zeros((vp, vp, vp, o, o, o))
+        for n, (a, b, c) in enumerate(abc):
+            for perm in itertools.permutations(range(3)):
+                x = (a, b, c)
+                # T_{y[perm]}^{x[perm]} = T_y^x: place the block at the permuted virtual indices
+                full[x[perm[0]], x[perm[1]], x[perm[2]]] = P[n].transpose(perm)
+        self.dev.view(_ptr(out), np.float64, full.size)[:] = full.ravel()
+        return 0
+
     def dcc_t3_to_abc_f64(self, t3, nocc, nvir, vpad, out, stream):
         o, v, vp = int(nocc), int(nvir), int(vpad)
         n3 = o * (o + 1) * (o + 2) // 6

@@ -379,3 +379,33 @@ def test_engine_cache_follows_attributes(monkeypatch):
     cc.triples_precision = "fp32"
     cc.kernel(eris=eris)
     assert cc.engine is not eng and cc.engine.mixed
+
+
+def test_batched_rccsdt_plan(monkeypatch):
+    """Whole RCCSDT iterations of an uneven ensemble (grouped triples launches, one graph per step) ==
+    each system's own RCCSDTEngine.update_amps, energies and amplitudes after two steps."""
+    abi_emulator.install(monkeypatch)
+    from ccsdt_cases import molecule_like, random_amplitudes
+    from distr_cc.batched import BatchedRCCSDT
+    from distr_cc.device import Stream
+    from distr_cc.rccsdt_engine import RCCSDTEngine
+    shapes = [(2, 3, 81), (3, 4, 82), (1, 4, 83), (3, 3, 84)]
+    st = Stream.__new__(Stream)
+    st.handle, st._own = ctypes.c_void_p(7), False
+    syss = [(o, v) + molecule_like(o, v, seed)[:3] for o, v, seed in shapes]
+    amps = [random_amplitudes(o, v, seed + 1) for o, v, seed in shapes]
+    bt = BatchedRCCSDT(syss, st)
+    for n, a in enumerate(amps):
+        bt.set_amplitudes(n, *a)
+    bt.capture()
+    for _ in range(2):
+        bt.step()
+        got = bt.energies()
+    for n, ((o, v, F, g, e), a) in enumerate(zip(syss, amps)):
+        ref = RCCSDTEngine(o, v, F, g, e)
+        ref.set_amplitudes(*a)
+        for _ in range(2):
+            ref.update_amps()
+        assert abs(got[n] - ref.energy()) <= 1e-14
+        for x, y in zip(bt.engines[n].get_amplitudes(), ref.get_amplitudes()):
+            assert abs(x - y).max() <= 1e-13 * max(abs(y).max(), 1e-30)

@@ -161,6 +161,45 @@ def test_h2o_ccpvdz_q_gpu():
         assert abs(eb - E_BRACKET_Q) < 1e-10 and abs(ep - E_PAREN_Q) < 1e-10
 
 
+def _q_prepared_and_fed(cc):
+    """tests/test_rccsdt_q.py:96-110: (Q) of a fresh RCCSDT object fed the converged amplitudes and its
+    own ERIs, with T3 given as canonical blocks and, after prepare_tamps_for_q(blksize=5), ABC-prepared."""
+    from distr_cc import rccsdt_q
+    from distr_cc.distribute_t3 import SINGLE_RANK
+    from distr_cc.rccsdt import RCCSDT
+    cc2 = RCCSDT(cc._scf, SINGLE_RANK)
+    eris = cc2.ao2mo()
+    fed = rccsdt_q.kernel(cc2, tamps=cc.tamps, eris=eris)
+    tamps_q = rccsdt_q.prepare_tamps_for_q(cc2, tamps=cc.tamps, blksize=5, comm=SINGLE_RANK)
+    assert hasattr(tamps_q[2][0], "abc_to_global_idx") and tamps_q[1].shape == (19, 19, 5, 5)
+    prepared = rccsdt_q.kernel(cc2, tamps=tamps_q, blksize=5, eris=eris)
+    for eb, ep in (fed, prepared):
+        assert abs(eb - E_BRACKET_Q) < 1e-10 and abs(ep - E_PAREN_Q) < 1e-10
+
+
+def test_q_engine_abc_input_emulated(monkeypatch):
+    """The (Q) engine fed T3 ABC-prepared gives the energies of the canonical-block input."""
+    abi_emulator.install(monkeypatch)
+    from distr_cc.device import DeviceArray
+    from distr_cc.distribute_t3 import DistributedT3IJK, redistribute_ijk_to_abc
+    from distr_cc.q_engine import QEngine
+    from distr_cc.synthetic import SyntheticSystem
+    o, v = 3, 7
+    s = SyntheticSystem(o, v, naux=20, seed=17)
+    pppp, t2, t3 = s.pppp(), s.t2(), s.t3_tri()
+    _, t3_abc = redistribute_ijk_to_abc(DistributedT3IJK(o, v), t3)
+    e_ijk = QEngine(o, v, blksize=3).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy)
+    e_abc = QEngine(o, v, blksize=3).kernel(pppp, t2, DeviceArray.from_host(t3_abc), s.mo_energy, t3_layout="abc")
+    assert abs(e_ijk[0] - e_abc[0]) <= 1e-14 * abs(e_ijk[0]) and abs(e_ijk[1] - e_abc[1]) <= 1e-14 * abs(e_ijk[1])
+
+
+@pytest.mark.gpu
+def test_h2o_ccpvdz_q_prepared_tamps_gpu():
+    from distr_cc import device
+    device.init(0)
+    _q_prepared_and_fed(_rccsdt(_mf_cached("h2o_ccpvdz_asym", lambda: _h2o("cc-pvdz", "asym")), 1e-10))
+
+
 @pytest.mark.gpu
 def test_c1_matches_cpu_oracle_gpu():
     """H2O/cc-pVDZ (the reference's CPU-runnable configuration): GPU FP64 == CPU FP64 to 1e-9 Eh."""

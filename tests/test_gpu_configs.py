@@ -6,12 +6,14 @@
   applied to the device residual.
 * nocc=16, nvir=160 (816 blocks of 160^3): the FP64 residual against the oracle on two target
   blocks; the FP32 and split-BF16x3 residuals against the FP64 device residual on all blocks (relative
-  Frobenius error below 2e-6, max error below 2e-5 of max |R|; FP32 accumulation over K up to
-  nvir^2 = 25600 with products rounded to 2^-24 or, for BF16x3, ~2^-16 per product); the split-BF16x3
-  update leaves no totally abc-symmetric part in any block.
+  Frobenius error below 2e-6 for FP32 and 2e-5 for split-BF16x3, max error below 2e-5 and 1e-4 of
+  max |R|: FP32 accumulation over K up to nvir^2 = 25600 with products rounded to 2^-24, or ~2^-16 per
+  split-BF16x3 product); the split-BF16x3 update leaves no totally abc-symmetric part in any block.
 * 96 systems (nocc 3..8, nvir 20..64, seeds 20261202+s) updated by one replay of the captured HIP
   graph of grouped persistent launches: the smallest, a middle and the largest system against the
-  oracle (residual rows, and the update of every block from the device residual).
+  oracle (residual rows, and the update of every block from the device residual); and the same 96
+  systems iterated as whole RCCSDT iterations (two graph replays) against every system's own ungrouped
+  iterations and, for the smallest and a middle system, two oracle update_amps iterations.
 Inputs are the seeded synthetic generator (B-factor sigma 0.05, naux 300, T3 noise 1e-4 / D).
 """
 import gc
@@ -132,12 +134,15 @@ def test_c3_mixed_residuals_vs_fp64(dev):
     assert err <= 1e-12
     del Ro
     gc.collect()
+    # FP32 products round to 2^-24; a split-BF16x3 product drops lo*lo and keeps 8 bits of each lo
+    # part, ~2^-16 relative per product, accumulated in FP32 over K up to nvir^2 = 25600
+    bounds = {"fp32": (2e-6, 2e-5), "bf16x3": (2e-5, 1e-4)}
     for name, contract in (("fp32", "native"), ("bf16x3", "bf16x3")):
         _say("C3 %s residual" % name)
         R, fin = _residual(o, v, s, im, t2, t3, np.float32, contract, finalize=(name == "bf16x3"))
         frob, mx = _frob_and_max(R, R64)
         _say("C3 %s vs FP64: rel Frobenius %.2e, max %.2e" % (name, frob, mx))
-        assert frob <= 2e-6 and mx <= 2e-5
+        assert frob <= bounds[name][0] and mx <= bounds[name][1]
         del R
         gc.collect()
     r3, norms = fin
@@ -189,4 +194,54 @@ def test_c4_batched_ensemble_vs_oracle(dev):
         err = _rel_rows(R, Ro, S)
         assert err <= 1e-12, (k, err)
         _check_update_all_blocks(o, R, t3, t3_new, s.mo_energy, norms[k])
+    bt.close()
+
+
+def test_c4_batched_rccsdt_iterations(dev):
+    """96 systems iterated together (full RCCSDT iterations, grouped triples launches, one HIP graph per
+    step): after two steps every system's energy and amplitudes equal its own ungrouped iterations, and
+    the smallest and a middle system equal the oracle's two update_amps iterations."""
+    from distr_cc.batched import BatchedRCCSDT
+    from distr_cc.device import Stream
+    from distr_cc.synthetic import SyntheticSystem
+    from oracle import rccsdt_lower as lo
+    syss, amps = [], []
+    for k in range(96):
+        r = np.random.default_rng(SEED + k)
+        o, v = int(r.integers(3, 9)), int(r.integers(20, 65))
+        s = SyntheticSystem(o, v, naux=300, seed=SEED + k)
+        syss.append((o, v, s.fock, s.pppp(), s.mo_energy))
+        amps.append((np.zeros((o, v)), s.t2(), s.t3_tri()))
+    _say("C4 full iterations: building 96 engines")
+    st = Stream()
+    bt = BatchedRCCSDT(syss, st)
+    for k, a in enumerate(amps):
+        bt.set_amplitudes(k, *a)
+    bt.capture()
+    for step in range(2):
+        bt.step()
+        e_batched = bt.energies()
+        _say("C4 step %d: E_corr of system 0 %.12f" % (step, e_batched[0]))
+    got = [e.get_amplitudes() for e in bt.engines]
+    worst = 0.0
+    for k, e in enumerate(bt.engines):
+        e.set_amplitudes(*amps[k])
+        for _ in range(2):
+            e.update_amps()
+        assert abs(e.energy() - e_batched[k]) <= 1e-12 * max(1.0, abs(e_batched[k])), k
+        for x, y in zip(got[k], e.get_amplitudes()):
+            worst = max(worst, float(np.abs(x - y).max() / max(np.abs(y).max(), 1e-300)))
+    _say("C4 grouped vs ungrouped iterations: worst amplitude rel diff %.2e" % worst)
+    assert worst <= 1e-11
+    cost = [o * (o + 1) * (o + 2) * v ** 5 for (o, v, _, _, _) in syss]
+    order = np.argsort(cost)
+    for k in (int(order[0]), int(order[len(order) // 2])):
+        o, v, F, g, e = syss[k]
+        _say("C4 system %d nocc=%d nvir=%d: two oracle iterations" % (k, o, v))
+        t1, t2, t3 = amps[k]
+        for _ in range(2):
+            t1, t2, t3, _res = lo.update_amps(o, F, g, e, t1, t2, t3)
+        assert abs(tri.energy_rhf(o, F, g, t1, t2) - e_batched[k]) <= 1e-10 * max(1.0, abs(e_batched[k]))
+        for x, y in zip(got[k], (t1, t2, t3)):
+            assert np.abs(x - y).max() <= 1e-10 * np.abs(y).max()
     bt.close()

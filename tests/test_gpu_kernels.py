@@ -487,3 +487,35 @@ def test_residual_autotuned_vs_oracle(dev, o, v):
         eng.residual()
         Ro = tri.compute_r3_tri(o, v, im, t2, t3)
         assert abs(eng.R.download() - Ro).max() <= 1e-12 * abs(Ro).max(), tile
+
+
+@pytest.mark.parametrize("o,v,vp", [(4, 9, 12), (5, 19, 20)])
+def test_t3_abc_layout_kernels_vs_oracle(dev, o, v, vp):
+    """IJK -> ABC (DistributedT3ABC layout) against the oracle's restatement of the reference packing,
+    bit for bit, and the ABC -> dense (Q) gather against the full tensor."""
+    import itertools
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    from oracle import t3_layout as TL
+    _say("ABC layout kernels o=%d v=%d vpad=%d" % (o, v, vp))
+    rng = np.random.default_rng(o * 100 + v)
+    M = o * v
+    table = rng.standard_normal(M ** 3)
+    # exactly symmetric under simultaneous (ijk)(abc) permutations
+    T = np.empty((o,) * 3 + (v,) * 3)
+    for i, j, k in itertools.product(range(o), repeat=3):
+        p = np.sort(np.stack(np.broadcast_arrays((i * v + np.arange(v))[:, None, None],
+                                                 (j * v + np.arange(v))[None, :, None],
+                                                 (k * v + np.arange(v))[None, None, :])), axis=0)
+        T[i, j, k] = table[(p[0] * M + p[1]) * M + p[2]]
+    t3 = np.ascontiguousarray([T[i, j, k] for (i, j, k) in TL.canonical_triples(o)])
+    src = DeviceArray.from_host(t3)
+    abc = DeviceArray((v * (v + 1) * (v + 2) // 6, o, o, o))
+    L.call("dcc_t3_tri_to_abc_packed_f64", src.vp, o, v, abc.vp, None)
+    got = abc.download()
+    assert np.array_equal(got, TL.t3_abc_from_tri(t3, o))
+    dense = DeviceArray((vp, vp, vp, o, o, o))
+    L.call("dcc_t3_abc_packed_to_dense_f64", abc.vp, o, v, vp, dense.vp, None)
+    d = dense.download()
+    assert np.array_equal(d[:v, :v, :v], T.transpose(3, 4, 5, 0, 1, 2))
+    assert not d[v:].any() and not d[:, v:].any() and not d[:, :, v:].any()
