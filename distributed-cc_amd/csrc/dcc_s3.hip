@@ -9,6 +9,7 @@
 // permuted access is then an LDS access, and the six output tiles are written
 // along their contiguous axis.  Each element is read and written exactly once.
 #include "dcc_common.h"
+#include <mutex>
 #include <type_traits>
 #include "dcc_gemm_tile.h"   // xcd_remap
 
@@ -387,6 +388,30 @@ int r3_finalize(const T *R, T *t3, T *r3_out, int64_t nblk, int64_t nvir, const 
     return 0;
 }
 
+// Allocation-free finalize.  The (block, tile orbit) partial norms live in this file-scope device
+// array instead of a caller workspace; each device holds its own copy from module load until the
+// process ends.  In host code the name is a stand-in: the device address is looked up once per device
+// with hipGetSymbolAddress and only that address is ever passed to a kernel.
+constexpr int64_t kScratchDoubles = (int64_t)3 << 20;  // 24 MiB: 1572864 (block, orbit) pairs
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double *g_scratch_ptr[kMaxDevices];
+
+int scratch_on_current_device(double **out) {
+    int dev = -1;
+    DCC_HIP(hipGetDevice(&dev));
+    DCC_CHECK(dev >= 0 && dev < kMaxDevices, "device ordinal out of range for the finalize scratch");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        DCC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -431,6 +456,22 @@ int dcc_r3_finalize_f32(const float *R, float *t3, float *r3_out, int64_t nblk, 
                         const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
                         double *partials, dcc_stream_t stream) {
     return r3_finalize<float>(R, t3, r3_out, nblk, nvir, blk_ijk, eia, nocc, nvir_diis, partials, stream);
+}
+
+int dcc_r3_finalize_noalloc_f64(const double *R, double *t3, double *r3_out, int64_t nblk, int64_t nvir,
+                                const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
+                                double *norms, dcc_stream_t stream) {
+    DCC_CHECK(nblk >= 0 && nvir >= 0 && nocc >= 0, "negative extent");
+    DCC_CHECK(norms != nullptr, "null norms");
+    const int64_t tiles = dcc_r3_finalize_tiles(nvir);
+    DCC_CHECK(tiles == 0 || nblk <= kScratchDoubles / (2 * tiles),
+              "nblk * dcc_r3_finalize_tiles(nvir) exceeds the 1572864 (block, tile orbit) partial sums of the "
+              "allocation-free scratch; use dcc_r3_finalize_f64 with a workspace");
+    double *ws = nullptr;
+    if (int st = scratch_on_current_device(&ws)) return st;
+    if (int st = r3_finalize<double>(R, t3, r3_out, nblk, nvir, blk_ijk, eia, nocc, nvir_diis, ws, stream))
+        return st;
+    return dcc_reduce_pairs_f64(ws, nblk * tiles, norms, stream);
 }
 
 int dcc_reduce_pairs_f64(const double *x, int64_t n, double *out, dcc_stream_t stream) {

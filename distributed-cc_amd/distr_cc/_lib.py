@@ -103,6 +103,8 @@ _SIGS = {
                             ctypes.c_int),
     "dcc_r3_finalize_f32": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp],
                             ctypes.c_int),
+    "dcc_r3_finalize_noalloc_f64": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp],
+                                    ctypes.c_int),
     "dcc_reduce_pairs_f64": ([c_vp, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_oooo_prefactors_f64": ([c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_ring_weights_f64": ([c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp], ctypes.c_int),

@@ -230,6 +230,21 @@ int dcc_r3_finalize_f32(const float *R, float *t3, float *r3_out, int64_t nblk, 
                         const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
                         double *partials, dcc_stream_t stream);
 
+/*
+ * dcc_r3_finalize_f64 followed by dcc_reduce_pairs_f64 of its partials into norms[0..1] (device
+ * memory: sum r^2, and the nvir_diis sub-norm), with the partials in a file-scope device array of
+ * the library instead of a caller workspace: this entry point allocates nothing (no hipMalloc, no
+ * hipFree, no allocating library call).  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  Limit:
+ * nblk * dcc_r3_finalize_tiles(nvir) <= 1572864 (24 MiB of partials per device, held until the
+ * process ends; nocc=16, nvir=160 needs 1256640), otherwise status 2 and nothing is launched.
+ * Every partial it reads is written by the same call, so nothing has to be cleared between calls;
+ * calls that overlap on one device (several streams or threads) share the array and race.
+ */
+int dcc_r3_finalize_noalloc_f64(const double *R, double *t3, double *r3_out, int64_t nblk, int64_t nvir,
+                                const int32_t *blk_ijk, const double *eia, int64_t nocc, int64_t nvir_diis,
+                                double *norms, dcc_stream_t stream);
+
 /* Deterministic fixed-order sum of `n` pairs (x[2i], x[2i+1]) into out[0..1]. */
 int dcc_reduce_pairs_f64(const double *x, int64_t n, double *out, dcc_stream_t stream);
 
