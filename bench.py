@@ -292,18 +292,20 @@ def q_time(o, v, blksize=4, seed=20261202):
 
 
 def adjoint_cost(o, v, reps=3, seed=20261202):
-    """Forward T2->T3 residual vs its hand-written VJP (distr_cc.adjoint) on the synthetic system:
-    mean device time of each, their ratio, and the device memory the adjoint engine holds."""
-    from distr_cc.adjoint import T2T3Adjoint
+    """The whole triples residual r3(t2, <pq|rs>) of one iteration at T1 = 0 (T2-dressed intermediates,
+    T3 -> W terms, residual) and its hand-written VJP (distr_cc.adjoint.TriplesResidualAdjoint) on the
+    synthetic system: mean device time of each, their ratio, and the device memory the adjoint holds."""
+    from distr_cc.adjoint import TriplesResidualAdjoint
     from distr_cc.device import Event, Stream, mem_get_info
     from distr_cc.synthetic import SyntheticSystem
     sysm = SyntheticSystem(o, v, naux=300, seed=seed)
-    im = sysm.bare_intermediates()
+    pppp, t2, t3 = sysm.pppp(), sysm.t2(), sysm.t3_tri()
     st = Stream()
+    gc.collect()
     free0, _ = mem_get_info()
-    adj = T2T3Adjoint(o, v, stream=st)
-    adj.set_inputs(sysm.t2(), im["W_vvvo"], im["W_vooo"])
-    adj.lam.upload(np.ascontiguousarray(sysm.t3_tri()), st)
+    adj = TriplesResidualAdjoint(o, v, sysm.fock, pppp, t3, stream=st)
+    adj.set_inputs(t2, pppp)
+    adj.lam.upload(np.ascontiguousarray(t3), st)
     free1, _ = mem_get_info()
 
     def timed(fn):
@@ -318,11 +320,20 @@ def adjoint_cost(o, v, reps=3, seed=20261202):
             out.append(a.elapsed_ms(b))
         return float(np.mean(out))
 
-    fwd, bwd = timed(adj.forward), timed(adj.backward)
-    return {"workload": "T2->T3 residual and its VJP (dt2, dW_vvvo, dW_vooo), nocc=%d nvir=%d" % (o, v),
-            "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd,
-            "adjoint_engine_bytes": int(free0 - free1),
-            "dense_adjoint_bytes": int(o ** 3 * v ** 3 * 8)}
+    fwd = timed(adj.forward)
+    bwd = timed(adj.backward)
+    free2, _ = mem_get_info()
+    out = {"workload": "whole triples residual of one iteration (T2-dressed intermediates, T3->W terms, "
+                       "residual; T1 = 0) and its VJP w.r.t. t2 and the (ov|vv)/(oo|ov) integrals, "
+                       "nocc=%d nvir=%d" % (o, v),
+           "forward_ms": fwd, "backward_ms": bwd, "backward_over_forward": bwd / fwd,
+           "device_bytes_held": int(free0 - min(free1, free2)),
+           "dense_workspace_bytes": int(2 * o ** 3 * v ** 3 * 8),
+           "note": "backward rebuilds the full Lambda and T3 from the canonical blocks on every call "
+                   "(dense_workspace_bytes of the held memory); no allocation inside forward or backward"}
+    del adj
+    gc.collect()
+    return out
 
 
 def batched_ensemble(nsys, steps=3, seed=20261202):

@@ -115,7 +115,9 @@ template <int R, bool KFAST, int Q, int NT> struct Stage3 {
 // MODE 0: FP32 operands loaded in pairs; 1: packed hi/lo words instead of FP32 values (the pointers
 // of p are reinterpreted) loaded in pairs; 2: packed words loaded in quads (16-byte loads).
 // NW waves per workgroup: 4 as a 2 x 2 grid, or 8 (256 x 128 and 128 x 256 tiles, 64 x 64 per wave,
-// one workgroup per CU), which loads a quarter fewer operand bytes per flop than two 128 x 128 tiles.
+// one workgroup per CU), which loads a quarter fewer operand bytes per flop than two 128 x 128 tiles;
+// 256 x 256 on 8 waves as 4 x 2 (64 x 128 per wave) loads half the operand bytes per flop of
+// 128 x 128 and reads a quarter fewer fragment bytes from LDS per MFMA.
 template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int MODE, int NW>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 void contract_bf16x3_kernel(const ContractArgs<float> p) {
@@ -129,7 +131,9 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
     constexpr int FM = WM / 16, FN = WN / 16;
     constexpr int BUF = 2 * (SA::PLANE + SB::PLANE);            // one buffer: hi + lo planes, 16-bit units
     constexpr int MAIN_BYTES = 2 * BUF * 2;                      // two buffers
-    constexpr int EPI_BYTES = C_MFAST ? BN * (BM + 1) * 4 : 0;
+    // the M-contiguous epilogue stages at most 128 columns of C at a time (two passes for BN = 256)
+    constexpr int EBN = BN > 128 ? 128 : BN;
+    constexpr int EPI_BYTES = C_MFAST ? EBN * (BM + 1) * 4 : 0;
     __shared__ __align__(16) unsigned char smem[MAIN_BYTES > EPI_BYTES ? MAIN_BYTES : EPI_BYTES];
     __shared__ int64_t ktab[2][2][kBK3];  // [tile parity][A, B][k]
     unsigned short *const lds = reinterpret_cast<unsigned short *>(smem);
@@ -357,27 +361,37 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
     else mainloop(std::false_type{});
 
     if constexpr (C_MFAST) {
+        // C staged through LDS column block by column block (EBN columns per pass), then written along
+        // its contiguous M index; every thread reaches every barrier
         float *Cs = reinterpret_cast<float *>(smem);
-#pragma unroll
-        for (int i = 0; i < FM; ++i)
-#pragma unroll
-            for (int j = 0; j < FN; ++j)
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    Cs[(wn * WN + j * 16 + lrow) * (BM + 1) + wm * WM + i * 16 + 4 * lk + r] = acc[i][j][r];
-        __syncthreads();
         constexpr int NSTEP = NT / BM;
         const int ml = tid % BM;
         const int64_t m = m0 + ml;
-        if (m >= m_end) return;
-        const int64_t crow = bt.c_off + map_off(p.cm, (uint32_t)m, batch);
-        for (int nl = tid / BM; nl < BN; nl += NSTEP) {
-            const int n = n0 + nl;
-            if (n >= p.N) break;
-            float *c = p.C + crow + map_off(p.cn, (uint32_t)n, batch);
-            double v = p.alpha * (double)Cs[nl * (BM + 1) + ml];
-            if (p.beta != 0.0) v += p.beta * (double)*c;
-            *c = (float)v;
+        const bool mok = m < m_end;
+        const int64_t crow = mok ? bt.c_off + map_off(p.cm, (uint32_t)m, batch) : 0;
+#pragma unroll
+        for (int pass = 0; pass < BN / EBN; ++pass) {
+            if (pass > 0) __syncthreads();   // the previous pass has read its columns
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    const int col = wn * WN + j * 16 + lrow - pass * EBN;
+                    if (col < 0 || col >= EBN) continue;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) Cs[col * (BM + 1) + wm * WM + i * 16 + 4 * lk + r] = acc[i][j][r];
+                }
+            __syncthreads();
+            if (mok) {
+                for (int nl = tid / BM; nl < EBN; nl += NSTEP) {
+                    const int n = n0 + pass * EBN + nl;
+                    if (n >= p.N) break;
+                    float *c = p.C + crow + map_off(p.cn, (uint32_t)n, batch);
+                    double v = p.alpha * (double)Cs[nl * (BM + 1) + ml];
+                    if (p.beta != 0.0) v += p.beta * (double)*c;
+                    *c = (float)v;
+                }
+            }
         }
         return;
     }
@@ -574,7 +588,7 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
                   valid_map(cn),
               "invalid index map");
     DCC_CHECK(ak.table == nullptr && bk.table == nullptr, "K index maps must be mixed-radix maps");
-    DCC_CHECK(tile_shape >= -1 && tile_shape <= 5, "tile shape must be -1..5");
+    DCC_CHECK(tile_shape >= -1 && tile_shape <= 6, "tile shape must be -1..6");
     if (M == 0 || N == 0 || nbatch == 0 || max_rows == 0) return 0;
     DCC_CHECK(A != nullptr && B != nullptr && C != nullptr && batches != nullptr, "null pointer");
     ContractArgs<float> args;
@@ -613,6 +627,7 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
         case 2: return run_bf16x3<64, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         case 4: return run_bf16x3<256, 128, 8>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         case 5: return run_bf16x3<128, 256, 8>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
+        case 6: return run_bf16x3<256, 256, 8>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         default: return run_bf16x3<128, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
     }
 }

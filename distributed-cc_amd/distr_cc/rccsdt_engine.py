@@ -69,8 +69,11 @@ class _Program:
         self.eng = engine
         self.steps = []
         self.keep = []
+        # the operations in order, (kind, ...), for programs that are differentiated (adjoint.py)
+        self.ops = []
 
     def ein(self, name, sub, A, B, C, alpha, beta):
+        self.ops.append(("ein", name, sub, A, B, C, alpha, beta))
         ext = self.eng.ext
         x = _split_k_letter(sub, A, B, C, ext)
         if x is None:
@@ -97,12 +100,14 @@ class _Program:
                                             stream.handle if stream is not None else None))
 
     def axpby(self, dst, src, letters, alpha, beta):
+        self.ops.append(("axpby", dst, src, letters, alpha, beta))
         ext = self.eng.ext
         shape = tuple(ext[x] for x in letters)
         stream = self.eng.stream
         self.steps.append(lambda _s: tensor_axpby(dst, src, letters, shape, alpha, beta, stream))
 
     def call(self, fn):
+        self.ops.append(("call", fn))
         self.steps.append(lambda _s: fn())
 
     def run(self, stream):
@@ -196,6 +201,15 @@ class RCCSDTEngine:
     @staticmethod
     def nview(buf, letters):
         return Operand(buf, letters, buf.shape)
+
+    def set_integrals(self, pppp):
+        """Replace the integrals <pq|rs> (same nmo) and rebuild what is derived from them."""
+        n = self.nmo
+        pppp = np.asarray(pppp, dtype=np.float64)
+        if pppp.shape != (n, n, n, n):
+            raise ValueError("pppp must be (nmo,)*4")
+        self.G0.upload(np.ascontiguousarray(pppp), self.stream)
+        self._setup_constants()
 
     def _setup_constants(self):
         o, v, n = self.nocc, self.nvir, self.nmo

@@ -8,14 +8,39 @@ Generator (host NumPy, FP64):
      chemist ``(pq|rs) = sum_Q B[Q,p,q] B[Q,r,s]``; physicist
      ``pppp[p,q,r,s] = <pq|rs> = (pr|qs)``.
   4. ``t1 = 0``; ``t2[i,j,a,b] = (ia|jb) / (e_i + e_j - e_a - e_b)``.
-  5. each canonical block (i<=j<=k) is N(0, sigma_t3^2) noise, symmetrized
+  5. each canonical block n = lin(i<=j<=k) is N(0, sigma_t3^2) noise drawn from
+     its own stream ``default_rng(SeedSequence([seed, 7919, n]))``, symmetrized
      for tied occupied indices so that the simultaneous-permutation symmetry
-     holds, then divided by D_ijk^abc.
+     holds, then divided by D_ijk^abc (blocks are generated in parallel threads).
 
 The ERI tensor is only materialized when ``nmo`` is small enough; large shapes
-build the slices the triples path needs directly from ``B``.
+build the slices the triples path needs directly from ``B``.  Integral slices are
+one GEMM over the auxiliary index.
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
+
+
+def _threads():
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def _parallel_transpose_0213(X):
+    """np.ascontiguousarray(X.transpose(0, 2, 1, 3)) with the copy split over threads along axis 0."""
+    out = np.empty((X.shape[0], X.shape[2], X.shape[1], X.shape[3]), dtype=X.dtype)
+
+    def part(p):
+        out[p] = X[p].transpose(1, 0, 2)
+
+    if X.shape[0] * X.shape[1] * X.shape[2] * X.shape[3] < (1 << 22):
+        for p in range(X.shape[0]):
+            part(p)
+    else:
+        with ThreadPoolExecutor(_threads()) as ex:
+            list(ex.map(part, range(X.shape[0])))
+    return out
 
 
 class SyntheticSystem:
@@ -37,10 +62,14 @@ class SyntheticSystem:
 
     # physicist <pq|rs> = (pr|qs) = sum_Q B[Q,p,r] B[Q,q,s]
     def eri_phys(self, ps, qs, rs, ss):
-        """Slice <p q | r s> for index slices (ps, qs, rs, ss) of the MO range."""
+        """Slice <p q | r s> for index slices (ps, qs, rs, ss) of the MO range: one GEMM
+        (p r) x Q x (q s), then the (p r q s) -> (p q r s) reordering."""
         Bpr = self.B[:, ps, rs]
         Bqs = self.B[:, qs, ss]
-        return np.einsum('Qpr,Qqs->pqrs', Bpr, Bqs, optimize=True)
+        nq, np_, nr = Bpr.shape
+        _, nq2, ns = Bqs.shape
+        X = Bpr.reshape(nq, np_ * nr).T @ Bqs.reshape(nq, nq2 * ns)
+        return _parallel_transpose_0213(X.reshape(np_, nr, nq2, ns))
 
     def pppp(self):
         """Full physicist ERI tensor (nmo^4); only for small systems."""
@@ -58,24 +87,35 @@ class SyntheticSystem:
         """Canonical-block T3, shape (nocc3, v, v, v), lexicographic i<=j<=k."""
         o, v = self.nocc, self.nvir
         eia = self.mo_energy[:o, None] - self.mo_energy[None, o:]
-        rng = np.random.default_rng(self.seed + 7919)
-        n3 = o * (o + 1) * (o + 2) // 6
-        out = np.empty((n3, v, v, v), dtype=self.dtype)
-        n = 0
-        for i in range(o):
-            for j in range(i, o):
-                for k in range(j, o):
-                    x = rng.normal(0.0, self.sigma_t3, (v, v, v))
-                    if i == j == k:
-                        x = (x + x.transpose(0, 2, 1) + x.transpose(1, 0, 2) + x.transpose(1, 2, 0)
-                             + x.transpose(2, 0, 1) + x.transpose(2, 1, 0)) / 6.0
-                    elif i == j:
-                        x = 0.5 * (x + x.transpose(1, 0, 2))
-                    elif j == k:
-                        x = 0.5 * (x + x.transpose(0, 2, 1))
-                    d = eia[i, :, None, None] + eia[j, None, :, None] + eia[k, None, None, :]
-                    out[n] = x / d
-                    n += 1
+        triples = [(i, j, k) for i in range(o) for j in range(i, o) for k in range(j, o)]
+        out = np.empty((len(triples), v, v, v), dtype=self.dtype)
+
+        def block(n):
+            # drawn straight into the output block and divided slab by slab, so that no block-sized
+            # temporaries are allocated (first-touch page faults dominate at nvir = 160)
+            i, j, k = triples[n]
+            rng = np.random.default_rng(np.random.SeedSequence([self.seed, 7919, n]))
+            x = out[n] if out.dtype == np.float64 else np.empty((v, v, v))
+            rng.standard_normal(out=x)
+            x *= self.sigma_t3
+            if i == j == k:
+                x[...] = (x + x.transpose(0, 2, 1) + x.transpose(1, 0, 2) + x.transpose(1, 2, 0)
+                          + x.transpose(2, 0, 1) + x.transpose(2, 1, 0)) / 6.0
+            elif i == j:
+                x[...] = 0.5 * (x + x.transpose(1, 0, 2))
+            elif j == k:
+                x[...] = 0.5 * (x + x.transpose(0, 2, 1))
+            for a in range(v):
+                x[a] /= (eia[i, a] + eia[j, :])[:, None] + eia[k, None, :]
+            if x is not out[n]:
+                out[n] = x
+
+        if len(triples) * v ** 3 < (1 << 22):
+            for n in range(len(triples)):
+                block(n)
+        else:
+            with ThreadPoolExecutor(_threads()) as ex:
+                list(ex.map(block, range(len(triples))))
         return out
 
     def bare_intermediates(self):

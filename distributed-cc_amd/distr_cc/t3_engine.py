@@ -925,31 +925,38 @@ class TriplesEngine:
                 g.tile = int(t)
 
     def tiles_by_name(self):
-        """{launch name: tile} of the contraction launches (for transfer to an engine of another size)."""
-        return {g.name: g.tile for g in self._tunable() if isinstance(g, GemmLaunch)}
+        """{launch name: tile} of the contraction launches (for transfer to an engine of another size);
+        of several launches with one name (chunks of a term) the one with the most flops decides."""
+        best = {}
+        for g in self._tunable():
+            if isinstance(g, GemmLaunch) and (g.name not in best or g.flops > best[g.name][0]):
+                best[g.name] = (g.flops, g.tile)
+        return {k: t for k, (_, t) in best.items()}
 
     def set_tiles_by_name(self, tiles):
         """Apply a tiles_by_name() map; launches without an entry keep their tile.  A split-BF16x3
-        tile with a 256-wide side (4: 256x128, 5: 128x256) becomes 128x128 on a launch whose extent on
-        that side is at most 128 here (a chunked launch of a larger system), where it would be mostly
-        padding."""
+        tile side of 256 (4: 256x128, 5: 128x256, 6: 256x256) is halved to 128 on a launch whose
+        extent on that side is at most 128 here (a chunked launch of a larger system), where it would
+        be mostly padding."""
         for g in self._tunable():
             if isinstance(g, GemmLaunch) and g.name in tiles:
                 t = int(tiles[g.name])
-                if g.fn.endswith("bf16x3") and ((t == 4 and g.max_rows <= 128) or (t == 5 and g.args[1] <= 128)):
-                    t = 3
+                if g.fn.endswith("bf16x3") and t >= 4:
+                    wide_m = t in (4, 6) and g.max_rows > 128
+                    wide_n = t in (5, 6) and g.args[1] > 128
+                    t = {(True, True): 6, (True, False): 4, (False, True): 5, (False, False): 3}[(wide_m, wide_n)]
                 g.tile = t
 
     def autotune(self, reps=2, include_lower=True):
         """Time the workgroup tiles of every contraction launch and keep the fastest: eight for the
-        FP64/FP32 kernels (four shapes x one- or two-stage prefetch), six shapes for split-BF16x3
-        (the four of 256 threads and 256x128, 128x256 of 512 threads).
+        FP64/FP32 kernels (four shapes x one- or two-stage prefetch), seven shapes for split-BF16x3
+        (the four of 256 threads and 256x128, 128x256, 256x256 of 512 threads).
 
         Run it before a residual (it accumulates into R and the scratch buffers); returns
         {launch name: (tile, ms)}."""
         from distr_cc.device import Event
         shapes = ("64x64", "128x64", "64x128", "128x128", "64x64s1", "128x64s1", "64x128s1", "128x128s1")
-        shapes3 = ("64x64", "128x64", "64x128", "128x128", "256x128", "128x256")
+        shapes3 = ("64x64", "128x64", "64x128", "128x128", "256x128", "128x256", "256x256")
         launches = self._tunable(include_lower)
         out = {}
         st = self.stream

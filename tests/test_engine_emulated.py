@@ -27,7 +27,8 @@ def test_engine_residual_and_update_plan(monkeypatch, o, v):
     n2, na = eng.finalize(s.mo_energy, level_shift=0.02, nvir_diis=max(1, v - 1))
     t3o, _, n2o, nao = tri.t3_update(o, Ro, t3, s.mo_energy, 0.02, max(1, v - 1))
     assert abs(eng.get_t3() - t3o).max() <= 1e-13 * abs(t3o).max()
-    assert abs(n2 - n2o) <= 1e-12 * n2o
+    # (an i=j=k block is entirely abc-symmetric, so its projected update is round-off: absolute floor)
+    assert abs(n2 - n2o) <= 1e-12 * n2o + 1e-26
 
 
 @pytest.mark.parametrize("o,v,pairs,fused", [(4, 5, 3, True), (5, 3, 1, True), (4, 4, 4, False)])
