@@ -116,62 +116,32 @@ __device__ __forceinline__ double q_factor(int a, int b, int c, int d) {
 constexpr int kQThreads = 256;
 constexpr int kQMaxO4 = 6561;  // nocc <= 9: two o^4 FP64 buffers in LDS
 
-// One workgroup per (task, a, b, c, d): t4 = P4_444[(t4 + z4_p) / D] * factor on
-// the o^4 occupied block, then partial sums <t4, z4_p> and <t4, z4_sq> with
-// z4_sq built on the fly from <ab|ij> (vvoo) and t2 (vvoo layout).
-__global__ __launch_bounds__(kQThreads) void q_finalize_kernel(
-    const double *__restrict__ t4, const double *__restrict__ z4, const int *__restrict__ tasks, int bs, int no,
-    int nv, int vp, const double *__restrict__ e_occ, const double *__restrict__ e_vir,
-    const double *__restrict__ vvoo, const double *__restrict__ t2, double *__restrict__ partials) {
-    extern __shared__ double lds[];
-    __shared__ double red[2][kQThreads / 64];
-    const int o = no, o2 = o * o, o3 = o2 * o, o4 = o3 * o;
-    double *b0 = lds, *b1 = lds + o4;
+// Task block start letters and the (a,b,c,d) of workgroup blockIdx.x of a chunk of tasks.
+struct QQuad {
+    int a, b, c, d;
+};
+
+__device__ __forceinline__ QQuad q_quad(const int *__restrict__ tasks, int bs) {
     const int bs4 = bs * bs * bs * bs;
     const int64_t task = blockIdx.x / bs4;
     const int e = (int)(blockIdx.x % bs4);
     const int ia = e / (bs * bs * bs), ib = (e / (bs * bs)) % bs, ic = (e / bs) % bs, id = e % bs;
-    const int a = tasks[4 * task] + ia, b = tasks[4 * task + 1] + ib, c = tasks[4 * task + 2] + ic,
-              d = tasks[4 * task + 3] + id;
-    double paren = 0.0, bracket = 0.0;
-    const double fac = (a < nv && b < nv && c < nv && d < nv) ? q_factor(a, b, c, d) : 0.0;
-    if (fac != 0.0) {  // uniform over the workgroup
-        const int64_t base = blockIdx.x * (int64_t)o4;
-        const double ev = e_vir[a] + e_vir[b] + e_vir[c] + e_vir[d];
-        for (int x = threadIdx.x; x < o4; x += kQThreads) {
-            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
-            const double den = e_occ[i] + e_occ[j] + e_occ[k] + e_occ[l] - ev;
-            const double num = t4[base + x] + z4[base + x];
-            b0[x] = fabs(den) > 1e-15 ? num / den : 0.0;
-        }
-        __syncthreads();
-        // (2 - P_ab - P_ac - P_ad) on the occupied positions
-        for (int x = threadIdx.x; x < o4; x += kQThreads) {
-            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
-            b1[x] = 2.0 * b0[x] - b0[((j * o + i) * o + k) * o + l] - b0[((k * o + j) * o + i) * o + l] -
-                    b0[((l * o + j) * o + k) * o + i];
-        }
-        __syncthreads();
-        // (2 - P_bc - P_bd)
-        for (int x = threadIdx.x; x < o4; x += kQThreads) {
-            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
-            b0[x] = 2.0 * b1[x] - b1[((i * o + k) * o + j) * o + l] - b1[((i * o + l) * o + k) * o + j];
-        }
-        __syncthreads();
-        const int64_t v = vp;
-        auto VV = [&](int p, int q, int r, int s) { return vvoo[((p * v + q) * o + r) * o + s]; };
-        auto T2 = [&](int p, int q, int r, int s) { return t2[((p * v + q) * o + r) * o + s]; };
-        for (int x = threadIdx.x; x < o4; x += kQThreads) {
-            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
-            // (2 - P_cd), then the factor
-            const double t = fac * (2.0 * b0[x] - b0[((i * o + j) * o + l) * o + k]);
-            const double zsq = VV(a, b, i, j) * T2(c, d, k, l) + VV(a, c, i, k) * T2(b, d, j, l) +
-                               VV(a, d, i, l) * T2(b, c, j, k) + VV(b, c, j, k) * T2(a, d, i, l) +
-                               VV(b, d, j, l) * T2(a, c, i, k) + VV(c, d, k, l) * T2(a, b, i, j);
-            paren += t * z4[base + x];
-            bracket += t * zsq;
-        }
-    }
+    return {tasks[4 * task] + ia, tasks[4 * task + 1] + ib, tasks[4 * task + 2] + ic, tasks[4 * task + 3] + id};
+}
+
+// z4_sq[i,j,k,l] of (a,b,c,d) from <ab|ij> and t2, both (vp, vp, o, o) (rccsdt_q.py:1143-1150).
+__device__ __forceinline__ double q_zsq(const double *__restrict__ vvoo, const double *__restrict__ t2, int64_t v,
+                                        int o, const QQuad &q, int i, int j, int k, int l) {
+    auto VV = [&](int p, int r, int s, int u) { return vvoo[((p * v + r) * o + s) * o + u]; };
+    auto T2 = [&](int p, int r, int s, int u) { return t2[((p * v + r) * o + s) * o + u]; };
+    const int a = q.a, b = q.b, c = q.c, d = q.d;
+    return VV(a, b, i, j) * T2(c, d, k, l) + VV(a, c, i, k) * T2(b, d, j, l) + VV(a, d, i, l) * T2(b, c, j, k) +
+           VV(b, c, j, k) * T2(a, d, i, l) + VV(b, d, j, l) * T2(a, c, i, k) + VV(c, d, k, l) * T2(a, b, i, j);
+}
+
+// Workgroup sums of (paren, bracket) into partials[2 * blockIdx.x + {0, 1}].
+__device__ __forceinline__ void q_store_partials(double paren, double bracket, double *__restrict__ partials) {
+    __shared__ double red[2][kQThreads / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         paren += __shfl_down(paren, off, 64);
@@ -192,6 +162,94 @@ __global__ __launch_bounds__(kQThreads) void q_finalize_kernel(
         partials[2 * blockIdx.x] = p;
         partials[2 * blockIdx.x + 1] = q;
     }
+}
+
+// One workgroup per (task, a, b, c, d): t4 = P4_444[(t4 + z4_p) / D] * factor on
+// the o^4 occupied block, then partial sums <t4, z4_p> and <t4, z4_sq> with
+// z4_sq built on the fly from <ab|ij> (vvoo) and t2 (vvoo layout).  nocc <= 9:
+// both intermediates of P4_444 live in LDS.
+__global__ __launch_bounds__(kQThreads) void q_finalize_kernel(
+    const double *__restrict__ t4, const double *__restrict__ z4, const int *__restrict__ tasks, int bs, int no,
+    int nv, int vp, const double *__restrict__ e_occ, const double *__restrict__ e_vir,
+    const double *__restrict__ vvoo, const double *__restrict__ t2, double *__restrict__ partials) {
+    extern __shared__ double lds[];
+    const int o = no, o2 = o * o, o3 = o2 * o, o4 = o3 * o;
+    double *b0 = lds, *b1 = lds + o4;
+    const QQuad q = q_quad(tasks, bs);
+    double paren = 0.0, bracket = 0.0;
+    const double fac = (q.a < nv && q.b < nv && q.c < nv && q.d < nv) ? q_factor(q.a, q.b, q.c, q.d) : 0.0;
+    if (fac != 0.0) {  // uniform over the workgroup
+        const int64_t base = blockIdx.x * (int64_t)o4;
+        const double ev = e_vir[q.a] + e_vir[q.b] + e_vir[q.c] + e_vir[q.d];
+        for (int x = threadIdx.x; x < o4; x += kQThreads) {
+            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            const double den = e_occ[i] + e_occ[j] + e_occ[k] + e_occ[l] - ev;
+            const double num = t4[base + x] + z4[base + x];
+            b0[x] = fabs(den) > 1e-15 ? num / den : 0.0;
+        }
+        __syncthreads();
+        // (2 - P_ab - P_ac - P_ad) on the occupied positions
+        for (int x = threadIdx.x; x < o4; x += kQThreads) {
+            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            b1[x] = 2.0 * b0[x] - b0[((j * o + i) * o + k) * o + l] - b0[((k * o + j) * o + i) * o + l] -
+                    b0[((l * o + j) * o + k) * o + i];
+        }
+        __syncthreads();
+        // (2 - P_bc - P_bd)
+        for (int x = threadIdx.x; x < o4; x += kQThreads) {
+            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            b0[x] = 2.0 * b1[x] - b1[((i * o + k) * o + j) * o + l] - b1[((i * o + l) * o + k) * o + j];
+        }
+        __syncthreads();
+        for (int x = threadIdx.x; x < o4; x += kQThreads) {
+            const int i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            // (2 - P_cd), then the factor
+            const double t = fac * (2.0 * b0[x] - b0[((i * o + j) * o + l) * o + k]);
+            paren += t * z4[base + x];
+            bracket += t * q_zsq(vvoo, t2, vp, o, q, i, j, k, l);
+        }
+    }
+    q_store_partials(paren, bracket, partials);
+}
+
+// The same update for any nocc, with one o^4 intermediate per workgroup in the global workspace ws
+// (o^4 doubles at ws + blockIdx.x * o^4).  The denominator is invariant under permutations of
+// (i,j,k,l), so b1 = (2 - P_ab - P_ac - P_ad)(t4 + z4) / D is formed in one pass from t4 and z4 at
+// the four positions; (2 - P_cd)(2 - P_bc - P_bd) b1 expands to
+//   4 b1[ijkl] - 2 b1[ikjl] - 2 b1[ilkj] - 2 b1[ijlk] + b1[iljk] + b1[iklj].
+__global__ __launch_bounds__(kQThreads) void q_finalize_ws_kernel(
+    const double *__restrict__ t4, const double *__restrict__ z4, const int *__restrict__ tasks, int bs, int no,
+    int nv, int vp, const double *__restrict__ e_occ, const double *__restrict__ e_vir,
+    const double *__restrict__ vvoo, const double *__restrict__ t2, double *__restrict__ ws,
+    double *__restrict__ partials) {
+    const int64_t o = no, o2 = o * o, o3 = o2 * o, o4 = o3 * o;
+    const QQuad q = q_quad(tasks, bs);
+    double paren = 0.0, bracket = 0.0;
+    const double fac = (q.a < nv && q.b < nv && q.c < nv && q.d < nv) ? q_factor(q.a, q.b, q.c, q.d) : 0.0;
+    if (fac != 0.0) {  // uniform over the workgroup
+        const int64_t base = blockIdx.x * o4;
+        const double *s4 = t4 + base, *z = z4 + base;
+        double *b1 = ws + base;
+        const double ev = e_vir[q.a] + e_vir[q.b] + e_vir[q.c] + e_vir[q.d];
+        auto S = [&](int64_t y) { return s4[y] + z[y]; };
+        for (int64_t x = threadIdx.x; x < o4; x += kQThreads) {
+            const int64_t i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            const double den = e_occ[i] + e_occ[j] + e_occ[k] + e_occ[l] - ev;
+            const double num = 2.0 * S(x) - S(((j * o + i) * o + k) * o + l) - S(((k * o + j) * o + i) * o + l) -
+                               S(((l * o + j) * o + k) * o + i);
+            b1[x] = fabs(den) > 1e-15 ? num / den : 0.0;
+        }
+        __syncthreads();
+        for (int64_t x = threadIdx.x; x < o4; x += kQThreads) {
+            const int64_t i = x / o3, j = (x / o2) % o, k = (x / o) % o, l = x % o;
+            const double t = fac * (4.0 * b1[x] - 2.0 * b1[((i * o + k) * o + j) * o + l] -
+                                    2.0 * b1[((i * o + l) * o + k) * o + j] - 2.0 * b1[((i * o + j) * o + l) * o + k] +
+                                    b1[((i * o + l) * o + j) * o + k] + b1[((i * o + k) * o + l) * o + j]);
+            paren += t * z[x];
+            bracket += t * q_zsq(vvoo, t2, vp, (int)o, q, (int)i, (int)j, (int)k, (int)l);
+        }
+    }
+    q_store_partials(paren, bracket, partials);
 }
 
 }  // namespace
@@ -239,17 +297,23 @@ int dcc_t3_abc_packed_to_dense_f64(const double *t3_abc, int64_t nocc, int64_t n
 
 int dcc_q_finalize_f64(const double *t4, const double *z4, const int32_t *tasks, int64_t ntask, int64_t blksize,
                        int64_t nocc, int64_t nvir, int64_t vpad, const double *e_occ, const double *e_vir,
-                       const double *vvoo, const double *t2, double *partials, dcc_stream_t stream) {
+                       const double *vvoo, const double *t2, double *ws, double *partials, dcc_stream_t stream) {
     DCC_CHECK(ntask >= 0 && blksize > 0 && nocc > 0 && nvir > 0 && vpad >= nvir, "bad extents");
     const int64_t o4 = nocc * nocc * nocc * nocc;
-    DCC_CHECK(o4 <= kQMaxO4, "nocc > 9 is not supported by the (Q) finalize kernel");
+    DCC_CHECK(ws != nullptr || o4 <= kQMaxO4, "nocc > 9 needs the workspace ws (ntask * blksize^4 * nocc^4 doubles)");
     const int64_t nwg = ntask * blksize * blksize * blksize * blksize;
     if (nwg == 0) return 0;
     DCC_CHECK(t4 && z4 && tasks && e_occ && e_vir && vvoo && t2 && partials, "null pointer");
     DCC_CHECK(nwg < ((int64_t)1 << 31), "grid too large");
-    hipLaunchKernelGGL(q_finalize_kernel, dim3((unsigned)nwg), dim3(kQThreads), (size_t)(2 * o4 * sizeof(double)),
-                       dcc::as_stream(stream), t4, z4, tasks, (int)blksize, (int)nocc, (int)nvir, (int)vpad, e_occ,
-                       e_vir, vvoo, t2, partials);
+    if (ws == nullptr) {
+        hipLaunchKernelGGL(q_finalize_kernel, dim3((unsigned)nwg), dim3(kQThreads), (size_t)(2 * o4 * sizeof(double)),
+                           dcc::as_stream(stream), t4, z4, tasks, (int)blksize, (int)nocc, (int)nvir, (int)vpad,
+                           e_occ, e_vir, vvoo, t2, partials);
+    } else {
+        hipLaunchKernelGGL(q_finalize_ws_kernel, dim3((unsigned)nwg), dim3(kQThreads), 0, dcc::as_stream(stream), t4,
+                           z4, tasks, (int)blksize, (int)nocc, (int)nvir, (int)vpad, e_occ, e_vir, vvoo, t2, ws,
+                           partials);
+    }
     DCC_LAUNCH_CHECK();
     return 0;
 }

@@ -114,7 +114,7 @@ _SIGS = {
     "dcc_t3_tri_to_abc_packed_f64": ([c_vp, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_t3_abc_packed_to_dense_f64": ([c_vp, c_i64, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
     "dcc_q_finalize_f64": ([c_vp, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp, c_vp, c_vp,
-                            c_vp], ctypes.c_int),
+                            c_vp, c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f64": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f32": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),
     "dcc_tensor_axpby_f32_f64": ([c_vp, c_vp, ctypes.POINTER(Strided), c_dbl, c_dbl, c_vp], ctypes.c_int),

@@ -307,13 +307,15 @@ int dcc_t3_abc_packed_to_dense_f64(const double *t3_abc, int64_t nocc, int64_t n
  * starts a0,b0,c0,d0): t4 <- (t4 + z4) / D with D -> 0 when |D| <= 1e-15,
  * P4_444 on the occupied indices, times the 24/12/6/0 factor; then
  * partials[wg] = (<t4, z4>, <t4, z4_sq>) with z4_sq built from vvoo (vpad,vpad,o,o)
- * and t2 (vpad,vpad,o,o).  t4/z4 are (ntask, bs, bs, bs, bs, o, o, o, o); nocc <= 9.
+ * and t2 (vpad,vpad,o,o).  t4/z4 are (ntask, bs, bs, bs, bs, o, o, o, o) and are
+ * not modified.  ws = NULL keeps the P4_444 intermediates in LDS (nocc <= 9);
+ * otherwise ws holds ntask * bs^4 * nocc^4 doubles of scratch and any nocc works.
  * Replaces t4_add_, e_abcdijkl_division_, t4_spin_summation_inplace_,
  * t4_multiply_factor_ and the dot products of distr_cc/rccsdt_q.py:1131-1153.
  */
 int dcc_q_finalize_f64(const double *t4, const double *z4, const int32_t *tasks, int64_t ntask, int64_t blksize,
                        int64_t nocc, int64_t nvir, int64_t vpad, const double *e_occ, const double *e_vir,
-                       const double *vvoo, const double *t2, double *partials, dcc_stream_t stream);
+                       const double *vvoo, const double *t2, double *ws, double *partials, dcc_stream_t stream);
 
 /* ------------------------------------- lower-order iteration (around T3) */
 

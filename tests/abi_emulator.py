@@ -494,7 +494,7 @@ class Emulator:
         self.dev.view(_ptr(out), np.float64, full.size)[:] = full.ravel()
         return 0
 
-    def dcc_q_finalize_f64(self, t4, z4, tasks, ntask, bs, nocc, nvir, vpad, e_occ, e_vir, vvoo, t2, partials,
+    def dcc_q_finalize_f64(self, t4, z4, tasks, ntask, bs, nocc, nvir, vpad, e_occ, e_vir, vvoo, t2, ws, partials,
                            stream):
         nt, bs, o, v, vp = int(ntask), int(bs), int(nocc), int(nvir), int(vpad)
         n = nt * bs ** 4 * o ** 4

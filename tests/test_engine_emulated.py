@@ -115,7 +115,10 @@ def _t3_to_r1r2_and_w(monkeypatch, o, v, qc):
 
 
 @pytest.mark.parametrize("o,v,bs", [(2, 5, 2), (3, 4, 3), (2, 5, 5), (1, 3, 2)])
-def test_q_engine_plan(monkeypatch, o, v, bs):
+@pytest.mark.parametrize("w_mode", ["dense", "task"])
+def test_q_engine_plan(monkeypatch, o, v, bs, w_mode):
+    """Both W_vvvvoo forms: dense, and per-chunk slots of the distinct ordered block triples (with a
+    one-task chunk size, so that every chunk builds its own slots)."""
     abi_emulator.install(monkeypatch)
     from distr_cc.device import DeviceArray
     from distr_cc.q_engine import QEngine
@@ -125,9 +128,15 @@ def test_q_engine_plan(monkeypatch, o, v, bs):
     s = SyntheticSystem(o, v, naux=20, seed=3 + o, sigma_t3=1e-3)
     pppp, t2, t3 = s.pppp(), s.t2(), s.t3_tri()
     ref = Q.kernel(o, v, pppp, t2, TL.full_t3_from_tri(t3, o), s.mo_energy, blksize=bs)
-    got = QEngine(o, v, blksize=bs).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy)
+    chunk_bytes = bs ** 4 * o ** 4 * 8 if w_mode == "task" else 1 << 30
+    eng = QEngine(o, v, blksize=bs, w_mode=w_mode, max_chunk_bytes=chunk_bytes)
+    got = eng.kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy)
     for g, r in zip(got, ref):
         assert abs(g - r) <= 1e-12 * abs(r) + 1e-25
+    if w_mode == "task":
+        from distr_cc.q_engine import create_full_task_list
+        assert "Wv" not in eng.dev
+        assert len(next(iter(eng.task_plans.values()))[-1]) == len(create_full_task_list(v, bs))
 
 
 @pytest.mark.parametrize("o,v", [(3, 4), (4, 5)])

@@ -159,6 +159,17 @@ def test_h2o_ccpvdz_q_gpu():
                  for i in range(n_jobs)]
         eb, ep = (sum(x[0] for x in parts), sum(x[1] for x in parts))
         assert abs(eb - E_BRACKET_Q) < 1e-10 and abs(ep - E_PAREN_Q) < 1e-10
+    # the forms used beyond the dense limits: W_vvvvoo slots per chunk of 8 tasks, workspace finalize
+    from distr_cc.device import DeviceArray
+    from distr_cc.q_engine import QEngine
+    eris = cc.engine._eris
+    t2, t3 = cc.tamps[1], np.ascontiguousarray(cc.tamps[2][1])
+    for bs in (3, 4):
+        eng = QEngine(cc.nocc, cc.nmo - cc.nocc, blksize=bs, w_mode="task", finalize_ws=True,
+                      max_chunk_bytes=8 * bs ** 4 * cc.nocc ** 4 * 8)
+        eb, ep = eng.kernel(eris.pppp, t2, DeviceArray.from_host(t3), eris.mo_energy)
+        print("blksize %d, slots + workspace: [Q] %.17f (Q) %.17f" % (bs, eb, ep), flush=True)
+        assert abs(eb - E_BRACKET_Q) < 1e-10 and abs(ep - E_PAREN_Q) < 1e-10
 
 
 def _q_prepared_and_fed(cc):

@@ -256,20 +256,32 @@ def test_t3_to_r1r2_and_w_vs_oracle(dev, o, v):
     assert abs(eng.R.download() - Ro).max() <= 1e-12 * abs(Ro).max()
 
 
-@pytest.mark.parametrize("o,v,bs", [(2, 5, 2), (3, 7, 4), (5, 9, 4), (4, 11, 3)])
-def test_q_energy_vs_oracle(dev, o, v, bs):
+@pytest.mark.parametrize("o,v,bs,form", [(2, 5, 2, "dense"), (3, 7, 4, "dense"), (5, 9, 4, "dense"),
+                                         (4, 11, 3, "dense"), (3, 7, 4, "task+ws"), (5, 9, 4, "task+ws"),
+                                         (4, 11, 3, "ws"), (10, 5, 3, "task+ws"), (12, 4, 2, "dense")])
+def test_q_energy_vs_oracle(dev, o, v, bs, form):
+    """(Q)/[Q] against the oracle.  form: "dense" = dense W_vvvvoo (and the LDS finalize up to nocc 9,
+    the workspace finalize beyond); "task" = W_vvvvoo slots per chunk of two tasks; "ws" = the workspace
+    finalize at any nocc."""
     from distr_cc.device import DeviceArray
     from distr_cc.q_engine import QEngine
     from distr_cc.synthetic import SyntheticSystem
     from oracle import q_energy as Q
     from oracle import t3_layout as TL
-    _say("(Q) o=%d v=%d blksize=%d" % (o, v, bs))
+    _say("(Q) o=%d v=%d blksize=%d %s" % (o, v, bs, form))
     s = SyntheticSystem(o, v, naux=30, seed=500 + o * v, sigma_t3=1e-3)
     pppp, t2, t3 = s.pppp(), s.t2(), s.t3_tri()
     ref = Q.kernel(o, v, pppp, t2, TL.full_t3_from_tri(t3, o), s.mo_energy, blksize=bs)
-    got = QEngine(o, v, blksize=bs).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy)
+    opts = {}
+    if "task" in form:
+        opts.update(w_mode="task", max_chunk_bytes=2 * bs ** 4 * o ** 4 * 8)
+    if "ws" in form:
+        opts["finalize_ws"] = True
+    got = QEngine(o, v, blksize=bs, **opts).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy)
     for g, r in zip(got, ref):
         assert abs(g - r) <= 1e-12 * abs(r)
+    if form != "dense":
+        return
     # job splitting is additive (tests/test_rccsdt_q.py:70-94)
     parts = [QEngine(o, v, blksize=bs).kernel(pppp, t2, DeviceArray.from_host(t3), s.mo_energy, job_idx=j,
                                               n_jobs=3) for j in range(3)]
