@@ -31,6 +31,7 @@ import numpy as np
 from distr_cc import _lib as L
 from distr_cc.device import DeviceArray, init as device_init
 from distr_cc.distribute_t3 import TP_T3, canonical_case, ijk_to_linear, nocc3
+from distr_cc.rccsdt_engine import plan_einsum_steps, split_k_partial_elems
 from distr_cc.t3_engine import Operand, plan_einsum, plan_t2t3, s3_jobs_array, tensor_axpby
 
 
@@ -261,6 +262,7 @@ class TriplesResidualAdjoint:
         self.lam_jobs = DeviceArray.from_host(s3_jobs_array(lam_jobs))
         self.t_jobs = DeviceArray.from_host(s3_jobs_array(t_jobs))
         self.njobs = len(lam_jobs)
+        self.keep = []
         self._plan_residual_adjoint()
         self._plan_program_adjoint()
 
@@ -269,7 +271,6 @@ class TriplesResidualAdjoint:
         o, v = self.nocc, self.nvir
         ext = {x: v for x in "abcdef"}
         ext.update({x: o for x in "ijklm"})
-        one = [({}, {}, {})]
         six = (o, o, o, v, v, v)
         lam = lambda s: Operand(self.Lam, s, six)
         T = lambda s: Operand(self.T, s, six)
@@ -285,8 +286,9 @@ class TriplesResidualAdjoint:
         W_oovo = Operand(self.tr.W_oovo, "ijal", (o, o, v, o))
         t2 = lambda s: Operand(self.tr.t2, s, (o, o, v, v))
         dt2 = lambda s: Operand(self.dt2, s, (o, o, v, v))
-        P = lambda name, sub, A, B, C, alpha: plan_einsum("adj r3 " + name, sub, A, B, C, alpha, 1.0, ext, one)
-        self.res_adj = [
+        terms = []
+        P = lambda name, sub, A, B, C, alpha: terms.append(("adj r3 " + name, sub, A, B, C, alpha))
+        [
             # T2 -> T3: the twelve terms are the six simultaneous images of
             # X = W_vvvo[a,b,d,j] t2[i,k,d,c] - W_vooo[a,l,i,j] t2[l,k,b,c]   (rccsdt.py:388-399)
             P("dW_vvvo", "ijkabc,ikdc->jabd", lam("ijkabc"), t2("ikdc"), dW_ovvv, 6.0),
@@ -310,6 +312,17 @@ class TriplesResidualAdjoint:
             P("dW_ovvo 2", "ljkcbd,ijkabc->ladi", T("ljkcbd"), lam("ijkabc"), dW_ovvo, -1.5),
             P("dW_ovvo 3", "ljkbdc,ijkabc->ladi", T("ljkbdc"), lam("ijkabc"), dW_ovvo, -1.5),
         ]
+        # reductions whose output fills few tiles against a long K (dW_oooo, dF_vv, dF_oo, the ring
+        # terms, dt2) are split along one K letter; their partial slices share one scratch buffer
+        split = dict(tile=128, min_tiles=256)
+        need = max(split_k_partial_elems(sub, A, B, C, ext, **split) for _, sub, A, B, C, _ in terms)
+        self.split_scratch = DeviceArray((max(need, 1),))
+        self.res_adj = []
+        for name, sub, A, B, C, alpha in terms:
+            steps, keep = plan_einsum_steps(name, sub, A, B, C, alpha, 1.0, ext, self.stream,
+                                            scratch=self.split_scratch, **split)
+            self.keep += keep
+            self.res_adj.append((name, steps))
 
     def _plan_program_adjoint(self):
         """Reverse of the intermediates program (RCCSDTEngine 'imds3'), as planned launches."""
@@ -317,7 +330,6 @@ class TriplesResidualAdjoint:
         grads = {id(eng.G): self.dG, id(eng.t2): self.dt2, id(eng.h_ovvo): self.dh}
         grads.update({id(b): self.d_imd[k] for k, b in eng.imd.items()})
         ext = eng.ext
-        one = [({}, {}, {})]
         st = self.stream
         steps = []
         for op in reversed(eng.prog["imds3"].ops):
@@ -328,14 +340,16 @@ class TriplesResidualAdjoint:
                 sb, sc = rest.split("->")
                 gC = _grad_view(C, grads)
                 gA, gB = _grad_view(A, grads), _grad_view(B, grads)
-                if gA is not None:
-                    steps.append(plan_einsum("adj " + name + " A", "%s,%s->%s" % (sc, sb, sa), gC, B, gA, alpha,
-                                             1.0, ext, one).run)
-                if gB is not None:
-                    steps.append(plan_einsum("adj " + name + " B", "%s,%s->%s" % (sa, sc, sb), A, gC, gB, alpha,
-                                             1.0, ext, one).run)
+                for nm, grad, rsub, X, Y in (("A", gA, "%s,%s->%s" % (sc, sb, sa), gC, B),
+                                             ("B", gB, "%s,%s->%s" % (sa, sc, sb), A, gC)):
+                    if grad is None:
+                        continue
+                    ps, keep = plan_einsum_steps("adj %s %s" % (name, nm), rsub, X, Y, grad, alpha, 1.0, ext, st,
+                                                 tile=128, min_tiles=256, scratch=self.split_scratch)
+                    self.keep += keep
+                    steps += [(getattr(ps[0].__self__, "name", name), p) for p in ps]
                 if beta != 1.0:
-                    steps.append(self._scale_step(gC, sc, beta))
+                    steps.append(("adj elementwise", self._scale_step(gC, sc, beta)))
             elif kind == "axpby":
                 _, dst, src, letters, alpha, beta = op
                 gd, gs = _grad_view(dst, grads), _grad_view(src, grads)
@@ -343,10 +357,10 @@ class TriplesResidualAdjoint:
                 if gs is not None:
                     if any(x not in src.strides for x in letters):
                         raise ValueError("broadcast copy in a differentiated program")
-                    steps.append(lambda _s, gs=gs, gd=gd, l=letters, sh=shape, a=alpha:
-                                 tensor_axpby(gs, gd, l, sh, a, 1.0, st))
+                    steps.append(("adj elementwise", lambda _s, gs=gs, gd=gd, l=letters, sh=shape, a=alpha:
+                                  tensor_axpby(gs, gd, l, sh, a, 1.0, st)))
                 if beta != 1.0:
-                    steps.append(self._scale_step(gd, letters, beta))
+                    steps.append(("adj elementwise", self._scale_step(gd, letters, beta)))
             else:
                 raise ValueError("the intermediates program has a step that cannot be differentiated")
         self.prog_adj = steps
@@ -376,19 +390,28 @@ class TriplesResidualAdjoint:
         tr.residual()
         return tr.R
 
-    def backward(self):
+    def backward(self, timer=None):
         """(dt2, dG) for the adjoint in ``lam``; call after forward() on the same inputs (the
-        intermediates of the forward are read)."""
+        intermediates of the forward are read).  ``timer(name, "start"|"end")``, if given, is called
+        around every launch (to record events on the stream)."""
         h = self.stream.handle if self.stream is not None else None
         st = self.stream
+        tm = timer if timer is not None else (lambda name, what: None)
         for buf in [self.dG, self.dt2, self.dh] + list(self.d_imd.values()):
             buf.fill(0.0, st)
+        tm("dense Lambda and T3", "start")
         L.call("dcc_s3_transform_f64", self.Lam.vp, self.lam.vp, self.nvir, 0.0, self.lam_jobs.vp, self.njobs, h)
         L.call("dcc_s3_transform_f64", self.T.vp, self.tr.t3.vp, self.nvir, 0.0, self.t_jobs.vp, self.njobs, h)
-        for g in self.res_adj:
-            g.run(st)
-        for step in self.prog_adj:
+        tm("dense Lambda and T3", "end")
+        for name, steps in self.res_adj:
+            tm(name, "start")
+            for step in steps:
+                step(st)
+            tm(name, "end")
+        for name, step in self.prog_adj:
+            tm(name, "start")
             step(st)
+            tm(name, "end")
         return self.dt2, self.dG
 
     def integral_mask(self):

@@ -192,17 +192,28 @@ class BatchedRCCSDT(_Grouped):
     def set_amplitudes(self, s, t1, t2, t3):
         self.engines[s].set_amplitudes(t1, t2, t3)
 
-    def issue_step(self):
-        """One iteration of every system and then its energy, with no host synchronisation."""
+    def issue_step(self, timer=None):
+        """One iteration of every system and then its energy, with no host synchronisation.
+        ``timer(phase, "start"|"end")``, if given, is called around the four phases (not under capture)."""
         st = self.stream
+        tm = timer if timer is not None else (lambda name, what: None)
+        tm("lower order, T3 -> r1/r2/W, residual operands", "start")
         for e in self.engines:
             e.issue_lower()
+        tm("lower order, T3 -> r1/r2/W, residual operands", "end")
+        tm("R reset and Y gather", "start")
+        for e in self.engines:
             e.tr.R.fill(0.0, st)
             e.tr.materialize_y()
+        tm("R reset and Y gather", "end")
+        tm("grouped triples residual", "start")
         self._run_phases()
+        tm("grouped triples residual", "end")
+        tm("finalize and energy", "start")
         for e in self.engines:
             e.tr.finalize_launch()
             e.energy_launch()
+        tm("finalize and energy", "end")
 
     def energies(self):
         self.stream.synchronize()

@@ -37,19 +37,20 @@ SPLIT_K_MIN = 4096
 SPLIT_MIN_SLICES = 8
 
 
-def _split_k_letter(sub, A, B, C, ext, tile=64):
-    """For a contraction whose M x N would occupy a few workgroups against a long K (an r1, F_oo or
-    Foo1-type term: one occupied or virtual index per side, K = o v^2), the K letter to peel into
-    batch entries that write partial slices: the largest one that is not the contiguous index of A
-    or B (so every slice still loads coalesced).  None when the contraction fills the chip anyway or
-    C is not a contiguous array in its letter order."""
+def _split_k_letter(sub, A, B, C, ext, tile=64, min_tiles=64):
+    """For a contraction whose M x N would occupy fewer than ``min_tiles`` tiles of tile x tile against
+    a long K (an r1, F_oo or Foo1-type term: one occupied or virtual index per side, K = o v^2), the K
+    letter to peel into batch entries that write partial slices: the largest one that is not the
+    contiguous index of A or B (so every slice still loads coalesced).  None when the contraction
+    fills the chip anyway or C is not a contiguous array in its letter order."""
     sa, rest = sub.split(',')
     sb, sc = rest.split('->')
     kl = [x for x in sa if x in sb and x not in sc]
     m = int(np.prod([ext[x] for x in sc if x in sa]))
     n = int(np.prod([ext[x] for x in sc if x in sb]))
     k = int(np.prod([ext[x] for x in kl]))
-    if -(-m // tile) * -(-n // tile) >= 64 or k < SPLIT_K_MIN or C.shape is None or tuple(C.letters) != tuple(sc):
+    if (-(-m // tile) * -(-n // tile) >= min_tiles or k < SPLIT_K_MIN or C.shape is None
+            or tuple(C.letters) != tuple(sc)):
         return None
     shape = [ext[x] for x in sc]
     acc, want = 1, []
@@ -60,6 +61,45 @@ def _split_k_letter(sub, A, B, C, ext, tile=64):
         return None
     cand = [x for x in kl if A.strides[x] != 1 and B.strides[x] != 1 and SPLIT_MIN_SLICES <= ext[x] <= 1024]
     return max(cand, key=lambda x: ext[x]) if cand else None
+
+
+def split_k_partial_elems(sub, A, B, C, ext, tile=64, min_tiles=64):
+    """Elements of the partial slices plan_einsum_steps would use for this contraction (0: no split)."""
+    x = _split_k_letter(sub, A, B, C, ext, tile, min_tiles)
+    if x is None:
+        return 0
+    sc = sub.split('->')[1]
+    return int(ext[x] * np.prod([ext[c] for c in sc]))
+
+
+def plan_einsum_steps(name, sub, A, B, C, alpha, beta, ext, stream, tile=64, min_tiles=64, scratch=None):
+    """C = alpha * einsum(sub, A, B) + beta * C as device steps (callables of the stream) and the
+    buffers they hold: one gather-GEMM, or, when the output fills fewer than ``min_tiles`` tiles against
+    a long K, a split along one K letter x -- slice s fixes x = s and writes its own partial of C, and
+    the slices are then summed in a fixed order into C (deterministic, no atomics).  The partials go
+    to ``scratch`` (a DeviceArray of C's dtype, shared by steps that run one after another on one
+    stream) when it is large enough, else to a buffer of their own."""
+    x = _split_k_letter(sub, A, B, C, ext, tile, min_tiles)
+    if x is None:
+        g = plan_einsum(name, sub, A, B, C, alpha, beta, ext, [({}, {}, {})])
+        return [g.run], []
+    sa, rest = sub.split(',')
+    sb, sc = rest.split('->')
+    S = ext[x]
+    sl = next(c for c in "TUVWYZ" if c not in ext and c not in sub)
+    shape = (S,) + tuple(ext[c] for c in sc)
+    if scratch is not None and scratch.dtype == C.buf.dtype and scratch.size >= int(np.prod(shape)):
+        P = DeviceArray(shape, C.buf.dtype, ptr=scratch.ptr, owner=False)
+    else:
+        P = DeviceArray(shape, C.buf.dtype)
+    g = plan_einsum(name + " split-K", "%s,%s->%s" % (sa.replace(x, ""), sb.replace(x, ""), sc),
+                    A, B, Operand(P, sl + sc, P.shape), alpha, 0.0, dict(ext, **{sl: S}),
+                    [({x: s}, {x: s}, {sl: s}) for s in range(S)])
+    count = int(np.prod(P.shape[1:]))
+    fn = "dcc_sum_slices_" + ("f64" if C.buf.dtype == np.float64 else "f32")
+    dst = C.buf.offset_ptr(C.base)
+    return [g.run, lambda _s: L.call(fn, dst, P.vp, S, count, 1.0, float(beta),
+                                     stream.handle if stream is not None else None)], [P]
 
 
 class _Program:
@@ -74,30 +114,9 @@ class _Program:
 
     def ein(self, name, sub, A, B, C, alpha, beta):
         self.ops.append(("ein", name, sub, A, B, C, alpha, beta))
-        ext = self.eng.ext
-        x = _split_k_letter(sub, A, B, C, ext)
-        if x is None:
-            g = plan_einsum(name, sub, A, B, C, alpha, beta, ext, [({}, {}, {})])
-            self.steps.append(g.run)
-            return
-        # split-K: slice s fixes K letter x = s and writes its own partial of C; the slices are then
-        # summed in a fixed order into C (deterministic, no atomics)
-        sa, rest = sub.split(',')
-        sb, sc = rest.split('->')
-        S = ext[x]
-        sl = next(c for c in "TUVWYZ" if c not in ext and c not in sub)
-        P = DeviceArray((S,) + tuple(ext[c] for c in sc), C.buf.dtype)
-        self.keep.append(P)
-        g = plan_einsum(name + " split-K", "%s,%s->%s" % (sa.replace(x, ""), sb.replace(x, ""), sc),
-                        A, B, Operand(P, sl + sc, P.shape), alpha, 0.0, dict(ext, **{sl: S}),
-                        [({x: s}, {x: s}, {sl: s}) for s in range(S)])
-        count = int(np.prod(P.shape[1:]))
-        fn = "dcc_sum_slices_" + ("f64" if C.buf.dtype == np.float64 else "f32")
-        dst = C.buf.offset_ptr(C.base)
-        stream = self.eng.stream
-        self.steps.append(g.run)
-        self.steps.append(lambda _s: L.call(fn, dst, P.vp, S, count, 1.0, float(beta),
-                                            stream.handle if stream is not None else None))
+        steps, keep = plan_einsum_steps(name, sub, A, B, C, alpha, beta, self.eng.ext, self.eng.stream)
+        self.steps += steps
+        self.keep += keep
 
     def axpby(self, dst, src, letters, alpha, beta):
         self.ops.append(("axpby", dst, src, letters, alpha, beta))
