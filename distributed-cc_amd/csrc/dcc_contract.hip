@@ -110,7 +110,29 @@ template <int R, bool KFAST, int Q, int NT> struct Stage3 {
             return bf16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
         }
     }
+    // fragment of v_mfma_f32_32x32x16_bf16 for k values 16 ksub .. 16 ksub + 15 of the plane: lane l
+    // holds rows rbase + l % 32 at k = 16 ksub + 8 (l / 32) + 0..7
+    __device__ static bf16x8 frag32(const unsigned short *plane, int rbase, int lane, int ksub) {
+        if constexpr (KFAST) {
+            const int row = rbase + lane % 32, chunk = 2 * ksub + lane / 32;
+            return *reinterpret_cast<const bf16x8 *>(&plane[row * kBK3 + 8 * (chunk ^ ((row >> 1) & 3))]);
+        } else {
+            const int g = lane / 16, lrow = lane % 16;
+            const int kr = 16 * ksub + 8 * (g / 2) + lrow / 4, c = rbase + 16 * (g % 2) + 4 * (lrow % 4);
+            typedef __attribute__((address_space(3))) unsigned short lds_u16;
+            const lds_u16 *base = (const lds_u16 *)plane;
+            const s16x4 lo4 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(base + kr * (R + 16) + c));
+            const s16x4 hi4 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4 *)(base + (kr + 4) * (R + 16) + c));
+            const bf16x4 a = __builtin_bit_cast(bf16x4, lo4), b = __builtin_bit_cast(bf16x4, hi4);
+            return bf16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+        }
+    }
 };
+
+typedef float flt16 __attribute__((ext_vector_type(16)));
+// Accumulator of one MI x MI MFMA tile: 4 floats per lane (16 x 16) or 16 (32 x 32).
+template <int MI> struct AccOf { typedef flt4 t; static constexpr int n = 4; };
+template <> struct AccOf<32> { typedef flt16 t; static constexpr int n = 16; };
 
 // MODE 0: FP32 operands loaded in pairs; 1: packed hi/lo words instead of FP32 values (the pointers
 // of p are reinterpreted) loaded in pairs; 2: packed words loaded in quads (16-byte loads).
@@ -118,7 +140,9 @@ template <int R, bool KFAST, int Q, int NT> struct Stage3 {
 // one workgroup per CU), which loads a quarter fewer operand bytes per flop than two 128 x 128 tiles;
 // 256 x 256 on 8 waves as 4 x 2 (64 x 128 per wave) loads half the operand bytes per flop of
 // 128 x 128 and reads a quarter fewer fragment bytes from LDS per MFMA.
-template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int MODE, int NW>
+// MI: the MFMA shape, 16 (v_mfma_f32_16x16x32_bf16, one K step of 32 per instruction) or 32
+// (v_mfma_f32_32x32x16_bf16, two K steps of 16: half the instructions for the same work).
+template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int MODE, int NW, int MI = 16>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 void contract_bf16x3_kernel(const ContractArgs<float> p) {
     constexpr bool PACKED = MODE >= 1;
@@ -128,7 +152,9 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
     typedef Stage3<BN, !B_NFAST, Q, NT> SB;
     constexpr int WGM = NW == 4 ? 2 : (BM >= BN ? 4 : 2), WGN = NW / WGM;   // wave grid
     constexpr int WM = BM / WGM, WN = BN / WGN;
-    constexpr int FM = WM / 16, FN = WN / 16;
+    constexpr int FM = WM / MI, FN = WN / MI;
+    typedef typename AccOf<MI>::t acc_t;
+    constexpr int NACC = AccOf<MI>::n;
     constexpr int BUF = 2 * (SA::PLANE + SB::PLANE);            // one buffer: hi + lo planes, 16-bit units
     constexpr int MAIN_BYTES = 2 * BUF * 2;                      // two buffers
     // the M-contiguous epilogue stages at most 128 columns of C at a time (two passes for BN = 256)
@@ -304,12 +330,18 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
     };
 
     const int wm = wave / WGN, wn = wave % WGN;
-    const int lrow = lane % 16, lk = lane / 16;
-    flt4 acc[FM][FN];
+    acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FN; ++j) acc[i][j] = flt4{0, 0, 0, 0};
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{};
+    // row (of the workgroup tile) of accumulator element r of MFMA tile i, column of MFMA tile j:
+    // 16 x 16: D[4 (lane / 16) + r][lane % 16]; 32 x 32: D[8 (r / 4) + 4 (lane / 32) + r % 4][lane % 32]
+    auto acc_row = [&](int i, int r) {
+        return MI == 16 ? wm * WM + i * 16 + 4 * (lane / 16) + r
+                        : wm * WM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    };
+    auto acc_col = [&](int j) { return wn * WN + j * MI + lane % MI; };
 
     // Two LDS buffers and one register stage: step kt loads tile kt+1 into registers, multiplies
     // tile kt from buffer kt&1 and writes tile kt+1 into the other buffer (last read in step kt-1);
@@ -336,21 +368,44 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
         fill_table(kt & 1, (kt + 2) * kBK3);
         const unsigned short *As = lds + (kt & 1) * BUF;
         const unsigned short *Bs = As + 2 * SA::PLANE;
-        bf16x8 ah[FM], al[FM];
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
-            ah[i] = SA::frag(As, wm * WM + i * 16, lane);
-            al[i] = SA::frag(As + SA::PLANE, wm * WM + i * 16, lane);
-        }
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const bf16x8 bh = SB::frag(Bs, wn * WN + j * 16, lane);
-            const bf16x8 bl = SB::frag(Bs + SB::PLANE, wn * WN + j * 16, lane);
+        if constexpr (MI == 16) {
+            bf16x8 ah[FM], al[FM];
 #pragma unroll
             for (int i = 0; i < FM; ++i) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+                ah[i] = SA::frag(As, wm * WM + i * 16, lane);
+                al[i] = SA::frag(As + SA::PLANE, wm * WM + i * 16, lane);
+            }
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const bf16x8 bh = SB::frag(Bs, wn * WN + j * 16, lane);
+                const bf16x8 bl = SB::frag(Bs + SB::PLANE, wn * WN + j * 16, lane);
+#pragma unroll
+                for (int i = 0; i < FM; ++i) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                bf16x8 ah[FM], al[FM];
+#pragma unroll
+                for (int i = 0; i < FM; ++i) {
+                    ah[i] = SA::frag32(As, wm * WM + i * 32, lane, ks);
+                    al[i] = SA::frag32(As + SA::PLANE, wm * WM + i * 32, lane, ks);
+                }
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    const bf16x8 bh = SB::frag32(Bs, wn * WN + j * 32, lane, ks);
+                    const bf16x8 bl = SB::frag32(Bs + SB::PLANE, wn * WN + j * 32, lane, ks);
+#pragma unroll
+                    for (int i = 0; i < FM; ++i) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh, acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+                    }
+                }
             }
         }
         store_tiles(t1);
@@ -376,10 +431,10 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
             for (int i = 0; i < FM; ++i)
 #pragma unroll
                 for (int j = 0; j < FN; ++j) {
-                    const int col = wn * WN + j * 16 + lrow - pass * EBN;
+                    const int col = acc_col(j) - pass * EBN;
                     if (col < 0 || col >= EBN) continue;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) Cs[col * (BM + 1) + wm * WM + i * 16 + 4 * lk + r] = acc[i][j][r];
+                    for (int r = 0; r < NACC; ++r) Cs[col * (BM + 1) + acc_row(i, r)] = acc[i][j][r];
                 }
             __syncthreads();
             if (mok) {
@@ -399,39 +454,43 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
     uint32_t cok = 0;
 #pragma unroll
     for (int j = 0; j < FN; ++j) {
-        const int n = n0 + wn * WN + j * 16 + lrow;
+        const int n = n0 + acc_col(j);
         const bool ok = n < p.N;
         cok |= (ok ? 1u : 0u) << j;
         ccol[j] = ok ? map_off(p.cn, (uint32_t)n, batch) : 0;
     }
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
-        int64_t crow[4];
-        uint32_t rok = 0;
+        // four accumulator rows at a time (for 32 x 32 tiles: the four groups of four)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int64_t m = m0 + wm * WM + i * 16 + 4 * lk + r;
-            const bool ok = m < m_end;
-            rok |= (ok ? 1u : 0u) << r;
-            crow[r] = ok ? bt.c_off + map_off(p.cm, (uint32_t)m, batch) : 0;
-        }
-        double old[4][FN];
-        if (p.beta != 0.0) {
+        for (int r0 = 0; r0 < NACC; r0 += 4) {
+            int64_t crow[4];
+            uint32_t rok = 0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t m = m0 + acc_row(i, r0 + r);
+                const bool ok = m < m_end;
+                rok |= (ok ? 1u : 0u) << r;
+                crow[r] = ok ? bt.c_off + map_off(p.cm, (uint32_t)m, batch) : 0;
+            }
+            double old[4][FN];
+            if (p.beta != 0.0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j)
+                        old[r][j] = (((rok >> r) & (cok >> j)) & 1u) ? (double)p.C[crow[r] + ccol[j]] : 0.0;
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r)
 #pragma unroll
-                for (int j = 0; j < FN; ++j)
-                    old[r][j] = (((rok >> r) & (cok >> j)) & 1u) ? (double)p.C[crow[r] + ccol[j]] : 0.0;
+                for (int j = 0; j < FN; ++j) {
+                    if (!(((rok >> r) & (cok >> j)) & 1u)) continue;
+                    double v = p.alpha * (double)acc[i][j][r0 + r];
+                    if (p.beta != 0.0) v += p.beta * old[r][j];
+                    p.C[crow[r] + ccol[j]] = (float)v;
+                }
         }
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                if (!(((rok >> r) & (cok >> j)) & 1u)) continue;
-                double v = p.alpha * (double)acc[i][j][r];
-                if (p.beta != 0.0) v += p.beta * old[r][j];
-                p.C[crow[r] + ccol[j]] = (float)v;
-            }
     }
 }
 
@@ -539,24 +598,24 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
 }
 
-template <int BM, int BN, bool C_MFAST, int MODE, int NW>
+template <int BM, int BN, bool C_MFAST, int MODE, int NW, int MI>
 void launch_bf16x3(ContractArgs<float> &args, bool a_kfast, bool b_nfast, dim3 grid, dim3 block,
                    hipStream_t stream) {
     if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, MODE, NW>), grid, block, 0, stream,
-                           args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, MODE, NW, MI>), grid, block, 0,
+                           stream, args);
     else if (a_kfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, MODE, NW>), grid, block, 0,
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, MODE, NW, MI>), grid, block, 0,
                            stream, args);
     else if (b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, MODE, NW>), grid, block, 0,
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, MODE, NW, MI>), grid, block, 0,
                            stream, args);
     else
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, MODE, NW>), grid, block, 0,
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, MODE, NW, MI>), grid, block, 0,
                            stream, args);
 }
 
-template <int BM, int BN, int NW = 4>
+template <int BM, int BN, int NW = 4, int MI = 16>
 int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool a_kfast, bool b_nfast,
                bool c_mfast, int mode, hipStream_t s) {
     args.tiles_m = (int)((max_rows + BM - 1) / BM);
@@ -565,13 +624,13 @@ int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool
     DCC_CHECK(args.total_tiles < ((int64_t)1 << 31), "too many tiles");
     const dim3 grid((unsigned)args.total_tiles), block(NW * 64);
     if (c_mfast) {
-        if (mode == 2) launch_bf16x3<BM, BN, true, 2, NW>(args, a_kfast, b_nfast, grid, block, s);
-        else if (mode == 1) launch_bf16x3<BM, BN, true, 1, NW>(args, a_kfast, b_nfast, grid, block, s);
-        else launch_bf16x3<BM, BN, true, 0, NW>(args, a_kfast, b_nfast, grid, block, s);
+        if (mode == 2) launch_bf16x3<BM, BN, true, 2, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
+        else if (mode == 1) launch_bf16x3<BM, BN, true, 1, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
+        else launch_bf16x3<BM, BN, true, 0, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
     } else {
-        if (mode == 2) launch_bf16x3<BM, BN, false, 2, NW>(args, a_kfast, b_nfast, grid, block, s);
-        else if (mode == 1) launch_bf16x3<BM, BN, false, 1, NW>(args, a_kfast, b_nfast, grid, block, s);
-        else launch_bf16x3<BM, BN, false, 0, NW>(args, a_kfast, b_nfast, grid, block, s);
+        if (mode == 2) launch_bf16x3<BM, BN, false, 2, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
+        else if (mode == 1) launch_bf16x3<BM, BN, false, 1, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
+        else launch_bf16x3<BM, BN, false, 0, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
     }
     DCC_LAUNCH_CHECK();
     return 0;
@@ -588,7 +647,7 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
                   valid_map(cn),
               "invalid index map");
     DCC_CHECK(ak.table == nullptr && bk.table == nullptr, "K index maps must be mixed-radix maps");
-    DCC_CHECK(tile_shape >= -1 && tile_shape <= 6, "tile shape must be -1..6");
+    DCC_CHECK(tile_shape >= -1 && tile_shape <= 7, "tile shape must be -1..7");
     if (M == 0 || N == 0 || nbatch == 0 || max_rows == 0) return 0;
     DCC_CHECK(A != nullptr && B != nullptr && C != nullptr && batches != nullptr, "null pointer");
     ContractArgs<float> args;
@@ -628,6 +687,7 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
         case 4: return run_bf16x3<256, 128, 8>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         case 5: return run_bf16x3<128, 256, 8>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         case 6: return run_bf16x3<256, 256, 8>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
+        case 7: return run_bf16x3<256, 256, 8, 32>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
         default: return run_bf16x3<128, 128>(args, nbatch, max_rows, a_kfast, b_nfast, c_mfast, mode, s);
     }
 }

@@ -347,7 +347,7 @@ class TriplesResidualAdjoint:
                     ps, keep = plan_einsum_steps("adj %s %s" % (name, nm), rsub, X, Y, grad, alpha, 1.0, ext, st,
                                                  tile=128, min_tiles=256, scratch=self.split_scratch)
                     self.keep += keep
-                    steps += [(getattr(ps[0].__self__, "name", name), p) for p in ps]
+                    steps += [(getattr(ps[0], "name", name), p) for p in ps]
                 if beta != 1.0:
                     steps.append(("adj elementwise", self._scale_step(gC, sc, beta)))
             elif kind == "axpby":

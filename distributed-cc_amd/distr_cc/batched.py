@@ -7,7 +7,8 @@ GEMM launch (dcc_contract_grouped_f64) per term; the bandwidth passes (Y gather,
 permuted adds, fused finalize) stay per system.  ``capture()`` records a whole step -- the residual
 operands from the intermediates, R reset, Y gather, all grouped GEMMs, all finalizes -- into one HIP
 graph, so a step is one graph launch.  ``BatchedRCCSDT`` does the same for whole RCCSDT iterations
-(lower order, T3 -> r1/r2/W, triples residual, update and energy of every system).
+(lower order, T3 -> r1/r2/W, triples residual, update and energy of every system); there the
+contractions of the lower-order part are grouped the same way, operation by operation.
 ``autotune()`` times the eight tile variants of the grouped kernel for every term and keeps the fastest.
 """
 import ctypes
@@ -105,14 +106,16 @@ class _Grouped:
         else:
             self.issue_step()
 
+    def _tunable_phases(self):
+        return [ph for ph in self.phases if isinstance(ph, _GroupedPhase)]
+
     def autotune(self, reps=2):
         """Pick the tile shape of every grouped phase by timing; returns {phase: (shape, ms)}.
-        Run before a step: it accumulates into R and overwrites the W_oooo/F_oo scratch."""
+        Run before a step (and set the amplitudes afterwards): it accumulates into R, into the
+        intermediates and overwrites the W_oooo/F_oo scratch."""
         st = self.stream
         out = {}
-        for n, ph in enumerate(self.phases):
-            if not isinstance(ph, _GroupedPhase):
-                continue
+        for n, ph in enumerate(self._tunable_phases()):
             best = None
             for tile in TILE_SHAPES:
                 ph.prepare(tile, st)
@@ -135,6 +138,10 @@ class _Grouped:
         return sum(ph.flops if isinstance(ph, _GroupedPhase) else 0.0 for ph in self.phases)
 
     def close(self):
+        for ph in self._tunable_phases():
+            if ph.dev is not None:
+                ph.dev.free()
+                ph.dev = None
         if self.graph is not None:
             L.call("dcc_graph_destroy", self.graph)
             self.graph = None
@@ -180,7 +187,7 @@ class BatchedRCCSDT(_Grouped):
     step per iteration): the iteration is the Jacobi update of update_amps_rccsdt_tri_
     (distr_cc/rccsdt.py:890-977)."""
 
-    def __init__(self, systems, stream, level_shift=0.0, device=0):
+    def __init__(self, systems, stream, level_shift=0.0, device=0, group_lower=True):
         from distr_cc.rccsdt_engine import RCCSDTEngine
         self.systems = [(int(o), int(v)) for o, v, _, _, _ in systems]
         self.engines = [RCCSDTEngine(o, v, F, g, e, level_shift=level_shift, stream=stream, device=device)
@@ -188,6 +195,43 @@ class BatchedRCCSDT(_Grouped):
         for e in self.engines:
             e.tr.set_denominators(e.mo_energy, e.level_shift, e.nvir_diis)
         self._group([e.tr for e in self.engines], stream)
+        self._group_lower(group_lower)
+
+    def _group_lower(self, group):
+        """Operation q of every engine's issue_lower (RCCSDTEngine.lower_ops): when each engine's
+        operation starts with an FP64 contraction, those contractions run as one grouped persistent
+        launch, followed by the rest of every engine's operation (split-K sums, ...).  Operations
+        without a contraction run engine by engine.  If the engines' operation lists differ in length
+        (or ``group`` is False), every engine's lower-order work runs on its own."""
+        per = [e.lower_ops() for e in self.engines]
+        self.lower_phases = []
+        if not group or any(len(p) != len(per[0]) for p in per):
+            self.lower_plan = [(None, [[s for op in p for s in op] for p in per])]
+            return
+        self.lower_plan = []
+        for q in range(len(per[0])):
+            ops = [p[q] for p in per]
+            heads = [op[0] for op in ops]
+            if all(isinstance(h, GemmLaunch) and h.fn == "dcc_contract_f64" for h in heads):
+                ph = _GroupedPhase(heads)
+                ph.prepare(0, self.stream)
+                self.lower_phases.append(ph)
+                self.lower_plan.append((ph, [op[1:] for op in ops]))
+            else:
+                self.lower_plan.append((None, ops))
+
+    def _tunable_phases(self):
+        return super()._tunable_phases() + self.lower_phases
+
+    def issue_lower(self):
+        """issue_lower of every engine, with the contractions of each operation grouped."""
+        st = self.stream
+        for ph, lists in self.lower_plan:
+            if ph is not None:
+                ph.run(st)
+            for steps in lists:
+                for step in steps:
+                    step(st)
 
     def set_amplitudes(self, s, t1, t2, t3):
         self.engines[s].set_amplitudes(t1, t2, t3)
@@ -198,8 +242,7 @@ class BatchedRCCSDT(_Grouped):
         st = self.stream
         tm = timer if timer is not None else (lambda name, what: None)
         tm("lower order, T3 -> r1/r2/W, residual operands", "start")
-        for e in self.engines:
-            e.issue_lower()
+        self.issue_lower()
         tm("lower order, T3 -> r1/r2/W, residual operands", "end")
         tm("R reset and Y gather", "start")
         for e in self.engines:

@@ -73,16 +73,15 @@ def split_k_partial_elems(sub, A, B, C, ext, tile=64, min_tiles=64):
 
 
 def plan_einsum_steps(name, sub, A, B, C, alpha, beta, ext, stream, tile=64, min_tiles=64, scratch=None):
-    """C = alpha * einsum(sub, A, B) + beta * C as device steps (callables of the stream) and the
-    buffers they hold: one gather-GEMM, or, when the output fills fewer than ``min_tiles`` tiles against
+    """C = alpha * einsum(sub, A, B) + beta * C as device steps (callables of the stream, the first one
+    the GemmLaunch itself) and the buffers they hold: one gather-GEMM, or, when the output fills fewer than ``min_tiles`` tiles against
     a long K, a split along one K letter x -- slice s fixes x = s and writes its own partial of C, and
     the slices are then summed in a fixed order into C (deterministic, no atomics).  The partials go
     to ``scratch`` (a DeviceArray of C's dtype, shared by steps that run one after another on one
     stream) when it is large enough, else to a buffer of their own."""
     x = _split_k_letter(sub, A, B, C, ext, tile, min_tiles)
     if x is None:
-        g = plan_einsum(name, sub, A, B, C, alpha, beta, ext, [({}, {}, {})])
-        return [g.run], []
+        return [plan_einsum(name, sub, A, B, C, alpha, beta, ext, [({}, {}, {})])], []
     sa, rest = sub.split(',')
     sb, sc = rest.split('->')
     S = ext[x]
@@ -98,8 +97,8 @@ def plan_einsum_steps(name, sub, A, B, C, alpha, beta, ext, stream, tile=64, min
     count = int(np.prod(P.shape[1:]))
     fn = "dcc_sum_slices_" + ("f64" if C.buf.dtype == np.float64 else "f32")
     dst = C.buf.offset_ptr(C.base)
-    return [g.run, lambda _s: L.call(fn, dst, P.vp, S, count, 1.0, float(beta),
-                                     stream.handle if stream is not None else None)], [P]
+    return [g, lambda _s: L.call(fn, dst, P.vp, S, count, 1.0, float(beta),
+                                 stream.handle if stream is not None else None)], [P]
 
 
 class _Program:
@@ -111,11 +110,18 @@ class _Program:
         self.keep = []
         # the operations in order, (kind, ...), for programs that are differentiated (adjoint.py)
         self.ops = []
+        # the steps of every operation, one list per operation (a contraction's GemmLaunch first)
+        self.op_steps = []
+
+    def add(self, steps):
+        """An operation given as its list of steps."""
+        self.op_steps.append(list(steps))
+        self.steps += steps
 
     def ein(self, name, sub, A, B, C, alpha, beta):
         self.ops.append(("ein", name, sub, A, B, C, alpha, beta))
         steps, keep = plan_einsum_steps(name, sub, A, B, C, alpha, beta, self.eng.ext, self.eng.stream)
-        self.steps += steps
+        self.add(steps)
         self.keep += keep
 
     def axpby(self, dst, src, letters, alpha, beta):
@@ -123,11 +129,17 @@ class _Program:
         ext = self.eng.ext
         shape = tuple(ext[x] for x in letters)
         stream = self.eng.stream
-        self.steps.append(lambda _s: tensor_axpby(dst, src, letters, shape, alpha, beta, stream))
+        self.add([lambda _s: tensor_axpby(dst, src, letters, shape, alpha, beta, stream)])
 
     def call(self, fn):
         self.ops.append(("call", fn))
-        self.steps.append(lambda _s: fn())
+        self.add([lambda _s: fn()])
+
+    def extend(self, ops):
+        """Operations given as lists of steps (TriplesEngine.t3_to_r1r2_ops)."""
+        for op in ops:
+            self.ops.append(("steps", op))
+            self.add(op)
 
     def run(self, stream):
         for step in self.steps:
@@ -324,7 +336,7 @@ class RCCSDTEngine:
         p = _Program(self)
         p.call(lambda: tr.set_t1_dressed_device(self.G, self.Fd, n))
         lo_r1, lo_r2 = tr._lower["r1"], tr._lower["r2"]
-        p.call(lambda: tr.t3_to_r1r2(download=False))
+        p.extend(tr.t3_to_r1r2_ops())
         p.axpby(r1, Operand(lo_r1, "iaX", (o, v, 1)), "ia", 1.0, 1.0)
         p.axpby(r2, nv(lo_r2, "ijab"), "ijab", 1.0, 1.0)
         r2s = nv(self.r2s, "ijab")
@@ -386,10 +398,10 @@ class RCCSDTEngine:
             for k, buf in self.imd.items():
                 dst = getattr(tr, k)
                 letters = "ABCD"[:len(buf.shape)]
-                p.steps.append(lambda _s, d=dst, b=buf, l=letters: tensor_axpby(
-                    Operand(d, l, d.shape), Operand(b, l, b.shape), l, d.shape, 1.0, 0.0, self.stream))
-            p.steps.append(lambda _s: tensor_axpby(Operand(tr.t2, "ijab", (o, o, v, v)), t2v("ijab"), "ijab",
-                                                   (o, o, v, v), 1.0, 0.0, self.stream))
+                p.add([lambda _s, d=dst, b=buf, l=letters: tensor_axpby(
+                    Operand(d, l, d.shape), Operand(b, l, b.shape), l, d.shape, 1.0, 0.0, self.stream)])
+            p.add([lambda _s: tensor_axpby(Operand(tr.t2, "ijab", (o, o, v, v)), t2v("ijab"), "ijab",
+                                           (o, o, v, v), 1.0, 0.0, self.stream)])
         steps["imds3"] = p
         return steps
 
@@ -440,6 +452,16 @@ class RCCSDTEngine:
         """The energy of the last energy_launch (downloads two device scalars)."""
         sc = self.scal.download(stream=self.stream)
         return float(sc[2] + 2.0 * sc[3])
+
+    def lower_ops(self):
+        """The device work of issue_lower as a list of operations, each a list of steps (callables of
+        the stream; a contraction's GemmLaunch first), in issue order."""
+        ops = []
+        for name in ("dress", "r1r2", "update12", "imds3"):
+            ops += self.prog[name].op_steps
+        ops.append([lambda _s: self.tr.t3_to_w(reuse_z=True)])
+        ops.append([lambda _s: self.tr.prepare_residual()])
+        return ops
 
     def issue_lower(self):
         """Device work of update_amps before the triples residual, with no host synchronisation (so it

@@ -126,6 +126,9 @@ class GemmLaunch:
         self.keep = keep  # device tables referenced by maps
         self.tile = -1    # -1: library default; 0..7: dcc_contract_tiled_* tile_shape (autotuned)
 
+    def __call__(self, stream):
+        self.run(stream)
+
     def run(self, stream):
         s = stream.handle if stream is not None else None
         if self.fn == "dcc_contract_packed_bf16x3":
@@ -801,6 +804,29 @@ class TriplesEngine:
         return (lo['r1'].download(stream=self.stream).reshape(self.nocc, self.nvir).astype(np.float64),
                 lo['r2'].download(stream=self.stream).astype(np.float64))
 
+    def t3_to_r1r2_ops(self):
+        """The device work of t3_to_r1r2(download=False) as a list of operations, each a list of steps
+        (callables of the stream; the contractions are the GemmLaunch objects themselves), for callers
+        that interleave the operations of several engines."""
+        if self._lower is None:
+            self._setup_lower()
+        lo = self._lower
+        st = self.stream
+        s = lambda: st.handle if st is not None else None
+        ops = [[lambda _s, b=lo[buf]: b.fill(0.0, st)] for buf in ('r1', 'r2', 'dWoovo', 'dWovvv')]
+        for c, chunk in enumerate(lo['chunks']):
+            ops.append([lambda _s, c=c: self._build_z(c)])
+            for g in chunk[2] + chunk[3]:
+                if isinstance(g, tuple):    # ('sum', dst, dst_offset, partials, nslices, n, alpha)
+                    _, dst, off, part, ns, n, alpha = g
+                    ops.append([lambda _s, dst=dst, off=off, part=part, ns=ns, n=n, alpha=alpha:
+                                L.call("dcc_sum_slices_" + self.sfx, dst.offset_ptr(off), part.vp, ns, n, alpha, 1.0,
+                                       s())])
+                else:
+                    ops.append([g])
+        ops.append([lambda _s: lo.__setitem__('dw_valid', True)])
+        return ops
+
     def _add_dw(self):
         s = self.stream.handle if self.stream is not None else None
         for dst, src in ((self.W_oovo, self._lower['dWoovo']), (self.W_ovvv, self._lower['dWovvv'])):
@@ -935,28 +961,30 @@ class TriplesEngine:
 
     def set_tiles_by_name(self, tiles):
         """Apply a tiles_by_name() map; launches without an entry keep their tile.  A split-BF16x3
-        tile side of 256 (4: 256x128, 5: 128x256, 6: 256x256) is halved to 128 on a launch whose
+        tile side of 256 (4: 256x128, 5: 128x256, 6 and 7: 256x256) is halved to 128 on a launch whose
         extent on that side is at most 128 here (a chunked launch of a larger system), where it would
         be mostly padding."""
         for g in self._tunable():
             if isinstance(g, GemmLaunch) and g.name in tiles:
                 t = int(tiles[g.name])
                 if g.fn.endswith("bf16x3") and t >= 4:
-                    wide_m = t in (4, 6) and g.max_rows > 128
-                    wide_n = t in (5, 6) and g.args[1] > 128
-                    t = {(True, True): 6, (True, False): 4, (False, True): 5, (False, False): 3}[(wide_m, wide_n)]
+                    wide_m = t in (4, 6, 7) and g.max_rows > 128
+                    wide_n = t in (5, 6, 7) and g.args[1] > 128
+                    t = {(True, True): t if t >= 6 else 6, (True, False): 4, (False, True): 5,
+                         (False, False): 3}[(wide_m, wide_n)]
                 g.tile = t
 
     def autotune(self, reps=2, include_lower=True):
         """Time the workgroup tiles of every contraction launch and keep the fastest: eight for the
-        FP64/FP32 kernels (four shapes x one- or two-stage prefetch), seven shapes for split-BF16x3
-        (the four of 256 threads and 256x128, 128x256, 256x256 of 512 threads).
+        FP64/FP32 kernels (four shapes x one- or two-stage prefetch), eight for split-BF16x3
+        (the four of 256 threads and 256x128, 128x256, 256x256 of 512 threads on 16 x 16 x 32 MFMAs,
+        and 256x256 on 32 x 32 x 16 MFMAs).
 
         Run it before a residual (it accumulates into R and the scratch buffers); returns
         {launch name: (tile, ms)}."""
         from distr_cc.device import Event
         shapes = ("64x64", "128x64", "64x128", "128x128", "64x64s1", "128x64s1", "64x128s1", "128x128s1")
-        shapes3 = ("64x64", "128x64", "64x128", "128x128", "256x128", "128x256", "256x256")
+        shapes3 = ("64x64", "128x64", "64x128", "128x128", "256x128", "128x256", "256x256", "256x256m32")
         launches = self._tunable(include_lower)
         out = {}
         st = self.stream

@@ -151,8 +151,9 @@ int dcc_contract_packed_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, do
                                int64_t max_rows, int64_t tile_shape, dcc_stream_t stream);
 
 /* dcc_contract_f32_bf16x3 with an explicit workgroup tile: tile_shape 0 = 64x64, 1 = 128x64,
- * 2 = 64x128, 3 = 128x128 (256 threads), 4 = 256x128, 5 = 128x256, 6 = 256x256 (512 threads) (M x N); -1 picks
- * 128 on each side unless that side has at most 64 rows.
+ * 2 = 64x128, 3 = 128x128 (256 threads), 4 = 256x128, 5 = 128x256, 6 = 256x256 (512 threads) (M x N) on
+ * v_mfma_f32_16x16x32_bf16, 7 = 256x256 (512 threads) on v_mfma_f32_32x32x16_bf16; -1 picks 128 on each side
+ * unless that side has at most 64 rows.
  * No reference counterpart (as dcc_contract_f32_bf16x3). */
 int dcc_contract_tiled_f32_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
                                   dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn,

@@ -405,6 +405,8 @@ def test_batched_rccsdt_plan(monkeypatch):
     syss = [(o, v) + molecule_like(o, v, seed)[:3] for o, v, seed in shapes]
     amps = [random_amplitudes(o, v, seed + 1) for o, v, seed in shapes]
     bt = BatchedRCCSDT(syss, st)
+    # the lower-order contractions of the four systems run grouped, operation by operation
+    assert len(bt.lower_phases) >= 50 and all(len(ph.launches) == 4 for ph in bt.lower_phases)
     for n, a in enumerate(amps):
         bt.set_amplitudes(n, *a)
     bt.capture()

@@ -361,7 +361,7 @@ def test_contract_bf16x3_plain_gemm(dev, M, N, K):
 
 @pytest.mark.parametrize("vec", ["none", "pairs", "quads"])
 @pytest.mark.parametrize("packed", [False, True])
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7])
 @pytest.mark.parametrize("layout", ["kfast-nfast", "mfast-nfast", "kfast-kfast", "mfast-kfast-ct"])
 def test_contract_bf16x3_layouts(dev, layout, tile, packed, vec):
     """Every staging layout of the split-BF16x3 kernel (A contiguous along k or m, B along n or k),
