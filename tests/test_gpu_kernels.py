@@ -310,6 +310,53 @@ def test_contract_f32_plain_gemm(dev, M, N, K):
     assert err < 3e-6 * max(1.0, abs(ref).max()) * np.sqrt(K), err
 
 
+@pytest.mark.parametrize("tile", [8, 9, 10, 11])
+@pytest.mark.parametrize("c_mt", [False, True])
+@pytest.mark.parametrize("pad", [8, 6])
+def test_contract_bf16x3_dma_tiles(dev, tile, c_mt, pad):
+    """Tiles 8..11 (operands staged by direct global -> LDS requests, two or three stages in flight) on
+    packed, K-contiguous A and B with K a multiple of 32, partial M/N tiles, both C layouts; batch offsets
+    of 8 words take the 16-byte requests, 6 words the 4-byte ones.  Checked against the same split
+    evaluated exactly on the host, and against the register-staged kernel of the same tile."""
+    from abi_emulator import bf16_split
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    M, N, K = 300, 260, 160
+    _say("contract bf16x3 direct-to-LDS tile %d c_mt %d pad %d" % (tile, c_mt, pad))
+    rng = np.random.default_rng(70 + tile)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    C0 = rng.normal(size=(M, N)).astype(np.float32)
+    lin = lambda s: L.IdxMap(None, 0, 1, 1 << 20, 0, 0, s)
+    hA = np.concatenate([np.zeros(pad, np.float32), A.ravel()])
+    hB = np.concatenate([np.zeros(pad, np.float32), B.T.ravel()])        # K-contiguous B
+    hC = np.concatenate([np.zeros(pad, np.float32), (C0.T if c_mt else C0).ravel()])
+    am, ak, bk, bn = lin(K), lin(1), lin(1), lin(K)
+    cm, cn = (lin(1), lin(M)) if c_mt else (lin(N), lin(1))
+    bt = np.zeros(1, dtype=L.BATCH_DTYPE)
+    bt[0] = (pad, pad, pad, 0, M)
+    dbt = DeviceArray.from_host(bt)
+    dA, dB = DeviceArray.from_host(hA), DeviceArray.from_host(hB)
+    pA, pB = DeviceArray(hA.shape, np.uint32), DeviceArray(hB.shape, np.uint32)
+    L.call("dcc_pack_bf16x3", dA.vp, pA.vp, hA.size, None)
+    L.call("dcc_pack_bf16x3", dB.vp, pB.vp, hB.size, None)
+    outs = []
+    for t in (tile, {8: 6, 9: 4, 10: 5, 11: 7}[tile]):
+        dC = DeviceArray.from_host(hC)
+        L.call("dcc_contract_packed_bf16x3", M, N, K, 1.5, 0.5, pA.vp, am, ak, pB.vp, bk, bn, dC.vp, cm, cn,
+               dbt.vp, 1, M, t, None)
+        out = dC.download()
+        assert np.all(out[:pad] == 0)
+        out = out[pad:].astype(np.float64)
+        outs.append(out.reshape(N, M).T if c_mt else out.reshape(M, N))
+    ah, al = bf16_split(A)
+    bh, bl = bf16_split(B)
+    split_ref = 0.5 * C0.astype(np.float64) + 1.5 * (ah @ bh + ah @ bl + al @ bh)
+    scale = max(1.0, abs(split_ref).max())
+    assert abs(outs[0] - split_ref).max() < 3e-6 * scale * np.sqrt(K)
+    assert abs(outs[0] - outs[1]).max() < 3e-6 * scale * np.sqrt(K)
+
+
 @pytest.mark.parametrize("o,v", [(3, 9), (5, 17)])
 def test_residual_fp32_vs_oracle(dev, o, v):
     from distr_cc.synthetic import SyntheticSystem
