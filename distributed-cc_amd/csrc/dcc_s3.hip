@@ -65,12 +65,19 @@ __device__ __forceinline__ bool orbit_of(int64_t b, int64_t total, int nt, int n
     return true;
 }
 
-// Element h of this thread in orbit tile s: block coordinates g[3], offset in the v^3 block and
-// whether it lies inside the block.  Elements outside (partial edge tiles) get offset 0, the block's
-// first element, so that loads can be issued unconditionally: all of a thread's loads are then in
-// flight together (with a branch per element the compiler waited for each load before the next).
-__device__ __forceinline__ bool tile_elem(int nvir, const int R[3], int s, int h, int g[3], int64_t &off) {
-    const int e = threadIdx.x + h * kThreads;
+// Two element mappings of a tile to the 256 threads.  Scalar: thread t holds elements t and t + 256
+// (one element per load).  PAIR (v even, operands 2-element aligned): thread t holds elements 2t and
+// 2t + 1, adjacent along the contiguous axis, moved with one vector load or store (16 bytes for doubles,
+// 8 for floats); with v even a pair lies entirely inside or outside the block.
+template <bool PAIR> __device__ __forceinline__ int elem_of(int h) {
+    return PAIR ? 2 * (int)threadIdx.x + h : (int)threadIdx.x + h * kThreads;
+}
+
+// Local element e of orbit tile s: block coordinates g[3], offset in the v^3 block and whether it lies
+// inside the block.  Elements outside (partial edge tiles) get offset 0, the block's first element, so
+// that loads can be issued unconditionally: all of a thread's loads are then in flight together (with
+// a branch per element the compiler waited for each load before the next).
+__device__ __forceinline__ bool tile_elem(int nvir, const int R[3], int s, int e, int g[3], int64_t &off) {
     g[0] = R[kP[s][0]] * E + (e >> 6);
     g[1] = R[kP[s][1]] * E + ((e >> 3) & 7);
     g[2] = R[kP[s][2]] * E + (e & 7);
@@ -79,25 +86,41 @@ __device__ __forceinline__ bool tile_elem(int nvir, const int R[3], int s, int h
     return in;
 }
 
-// Load the six permuted tiles of orbit (R0,R1,R2) of block `src` into LDS: all loads first, then
-// the LDS stores.
-template <typename T>
-__device__ __forceinline__ void load_orbit(T (*tiles)[TE], const T *__restrict__ src, int nvir, const int R[3]) {
-    T val[6][kPer];
-    bool in[6][kPer];
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
+template <typename T> struct Vec2 { typedef T t __attribute__((ext_vector_type(2))); };
+
+// This thread's two elements of orbit tile s of a block, loaded together (zero outside the block).
+template <typename T, bool PAIR>
+__device__ __forceinline__ void load2(const T *__restrict__ src, int nvir, const int R[3], int s, T out[2]) {
+    if constexpr (PAIR) {
+        int g[3];
+        int64_t off;
+        const bool in = tile_elem(nvir, R, s, elem_of<true>(0), g, off);
+        const typename Vec2<T>::t x = *reinterpret_cast<const typename Vec2<T>::t *>(src + off);
+        out[0] = in ? x[0] : T(0);
+        out[1] = in ? x[1] : T(0);
+    } else {
 #pragma unroll
         for (int h = 0; h < kPer; ++h) {
             int g[3];
             int64_t off;
-            in[s][h] = tile_elem(nvir, R, s, h, g, off);
-            val[s][h] = src[off];
+            const bool in = tile_elem(nvir, R, s, elem_of<false>(h), g, off);
+            const T x = src[off];
+            out[h] = in ? x : T(0);
         }
+    }
+}
+
+// Load the six permuted tiles of orbit (R0,R1,R2) of block `src` into LDS: all loads first, then
+// the LDS stores.
+template <typename T, bool PAIR>
+__device__ __forceinline__ void load_orbit(T (*tiles)[TE], const T *__restrict__ src, int nvir, const int R[3]) {
+    T val[6][kPer];
+#pragma unroll
+    for (int s = 0; s < 6; ++s) load2<T, PAIR>(src, nvir, R, s, val[s]);
 #pragma unroll
     for (int s = 0; s < 6; ++s)
 #pragma unroll
-        for (int h = 0; h < kPer; ++h) tiles[s][threadIdx.x + h * kThreads] = in[s][h] ? val[s][h] : T(0);
+        for (int h = 0; h < kPer; ++h) tiles[s][elem_of<PAIR>(h)] = val[s][h];
 }
 
 // Is output tile s a repeat of an earlier tile of the same orbit?
@@ -110,17 +133,28 @@ __device__ __forceinline__ bool duplicate_tile(int s, const int R[3]) {
 
 // This thread's current values of the six output tiles of orbit R in block dst (loads issued
 // together; elements outside the block read the block's first element and are never stored).
-template <typename T>
+template <typename T, bool PAIR>
 __device__ __forceinline__ void prefetch_out(const T *dst, int nvir, const int R[3], T old[6][kPer]) {
 #pragma unroll
-    for (int s = 0; s < 6; ++s)
+    for (int s = 0; s < 6; ++s) load2<T, PAIR>(dst, nvir, R, s, old[s]);
+}
+
+// Store this thread's two values of output tile s (elements outside the block are skipped).
+template <typename T, bool PAIR>
+__device__ __forceinline__ void store2(T *dst, int nvir, const int R[3], int s, const T val[2]) {
+    if constexpr (PAIR) {
+        int g[3];
+        int64_t off;
+        if (tile_elem(nvir, R, s, elem_of<true>(0), g, off))
+            *reinterpret_cast<typename Vec2<T>::t *>(dst + off) = typename Vec2<T>::t{val[0], val[1]};
+    } else {
 #pragma unroll
         for (int h = 0; h < kPer; ++h) {
             int g[3];
             int64_t off;
-            tile_elem(nvir, R, s, h, g, off);
-            old[s][h] = dst[off];
+            if (tile_elem(nvir, R, s, elem_of<false>(h), g, off)) dst[off] = val[h];
         }
+    }
 }
 
 // Value of X.transpose(p)[x] for x = local element e of output tile s.
@@ -140,7 +174,37 @@ __device__ __forceinline__ double permuted(const T (*tiles)[TE], int s, int p, i
 
 // O: the output element type -- T, or uint32_t for FP32 input written as split-BF16x3 words
 // (dcc::pack_bf16x3 of the float result; beta must then be 0).
-template <typename T, typename O = T>
+template <typename T, typename O, bool PAIR>
+__device__ __forceinline__ void s3_transform_orbit(T (*tiles)[TE], O *__restrict__ dst, const T *__restrict__ src,
+                                                   int nvir, double beta, const dcc_s3_job &jb, const int R[3]) {
+    O old[6][kPer];
+    if constexpr (std::is_same<O, T>::value) {
+        if (beta != 0.0) prefetch_out<T, PAIR>(dst, nvir, R, old);
+    }
+    load_orbit<T, PAIR>(tiles, src, nvir, R);
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+        if (duplicate_tile(s, R)) continue;
+        O res[kPer];
+#pragma unroll
+        for (int h = 0; h < kPer; ++h) {
+            double val = 0.0;
+#pragma unroll
+            for (int p = 0; p < 6; ++p)
+                if (jb.coef[p] != 0.0) val += jb.coef[p] * permuted(tiles, s, p, elem_of<PAIR>(h));
+            if constexpr (std::is_same<O, T>::value) {
+                if (beta != 0.0) val += beta * (double)old[s][h];
+                res[h] = (T)val;
+            } else {
+                res[h] = dcc::pack_bf16x3((float)val);
+            }
+        }
+        store2<O, PAIR>(dst, nvir, R, s, res);
+    }
+}
+
+template <typename T, typename O = T, bool PAIR = false>
 __global__ __launch_bounds__(kThreads) void s3_transform_kernel(O *__restrict__ out, const T *__restrict__ src,
                                                                 int nvir, double beta,
                                                                 const dcc_s3_job *__restrict__ jobs, int nt,
@@ -150,49 +214,21 @@ __global__ __launch_bounds__(kThreads) void s3_transform_kernel(O *__restrict__ 
     int orbit, R[3];
     if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
     const dcc_s3_job jb = jobs[job];
-    O *dst = out + jb.dst;
-    O old[6][kPer];
-    if constexpr (std::is_same<O, T>::value) {
-        if (beta != 0.0) prefetch_out(dst, nvir, R, old);
-    }
-    load_orbit(tiles, src + jb.src, nvir, R);
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        if (duplicate_tile(s, R)) continue;
-#pragma unroll
-        for (int h = 0; h < kPer; ++h) {
-            int g[3];
-            int64_t off;
-            if (!tile_elem(nvir, R, s, h, g, off)) continue;
-            double val = 0.0;
-#pragma unroll
-            for (int p = 0; p < 6; ++p)
-                if (jb.coef[p] != 0.0) val += jb.coef[p] * permuted(tiles, s, p, threadIdx.x + h * kThreads);
-            if constexpr (std::is_same<O, T>::value) {
-                if (beta != 0.0) val += beta * (double)old[s][h];
-                dst[off] = (T)val;
-            } else {
-                dst[off] = dcc::pack_bf16x3((float)val);
-            }
-        }
-    }
+    // the paired form needs both blocks to start on an even element (uniform over the workgroup)
+    if (PAIR && ((jb.dst | jb.src) & 1) == 0)
+        s3_transform_orbit<T, O, true>(tiles, out + jb.dst, src + jb.src, nvir, beta, jb, R);
+    else
+        s3_transform_orbit<T, O, false>(tiles, out + jb.dst, src + jb.src, nvir, beta, jb, R);
 }
 
 // Multi-source S3 transform: out[x] = beta*out[x] + sum_p c[p] * src_p.transpose(p)[x], with its own
 // source block src_p per permutation (for example the six permuted base blocks X[pi(ijk)] of a
 // symmetrised term).  The orbit tiles of each source are staged in LDS in turn and the six output
 // tiles accumulate in registers, so the output block is read and written once.
-template <typename T>
-__global__ __launch_bounds__(kThreads) void s3_gather_kernel(T *__restrict__ out, const T *__restrict__ src,
-                                                             int nvir, double beta,
-                                                             const dcc_s3_gather_job *__restrict__ jobs,
-                                                             int nt, int nso) {
-    __shared__ T tiles[6][TE];
-    int64_t job;
-    int orbit, R[3];
-    if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
-    const dcc_s3_gather_job jb = jobs[job];
+template <typename T, bool PAIR>
+__device__ __forceinline__ void s3_gather_orbit(T (*tiles)[TE], T *__restrict__ out, const T *__restrict__ src,
+                                                int nvir, double beta, const dcc_s3_gather_job &jb,
+                                                const int R[3]) {
     double acc[6][kPer];
 #pragma unroll
     for (int s = 0; s < 6; ++s)
@@ -203,37 +239,52 @@ __global__ __launch_bounds__(kThreads) void s3_gather_kernel(T *__restrict__ out
         if (jb.coef[p] == 0.0) continue;   // uniform across the workgroup
         if (jb.src[p] != loaded) {
             __syncthreads();                // the previous source's tiles are no longer read
-            load_orbit(tiles, src + jb.src[p], nvir, R);
+            load_orbit<T, PAIR>(tiles, src + jb.src[p], nvir, R);
             __syncthreads();
             loaded = jb.src[p];
         }
 #pragma unroll
         for (int s = 0; s < 6; ++s)
 #pragma unroll
-            for (int h = 0; h < kPer; ++h)
-                acc[s][h] += jb.coef[p] * permuted(tiles, s, p, threadIdx.x + h * kThreads);
+            for (int h = 0; h < kPer; ++h) acc[s][h] += jb.coef[p] * permuted(tiles, s, p, elem_of<PAIR>(h));
     }
     T *dst = out + jb.dst;
     T old[6][kPer];
-    if (beta != 0.0) prefetch_out(dst, nvir, R, old);
+    if (beta != 0.0) prefetch_out<T, PAIR>(dst, nvir, R, old);
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
         if (duplicate_tile(s, R)) continue;
+        T res[kPer];
 #pragma unroll
         for (int h = 0; h < kPer; ++h) {
-            int g[3];
-            int64_t off;
-            if (!tile_elem(nvir, R, s, h, g, off)) continue;
             double val = acc[s][h];
             if (beta != 0.0) val += beta * (double)old[s][h];
-            dst[off] = (T)val;
+            res[h] = (T)val;
         }
+        store2<T, PAIR>(dst, nvir, R, s, res);
     }
 }
 
+template <typename T, bool PAIR = false>
+__global__ __launch_bounds__(kThreads) void s3_gather_kernel(T *__restrict__ out, const T *__restrict__ src,
+                                                             int nvir, double beta,
+                                                             const dcc_s3_gather_job *__restrict__ jobs,
+                                                             int nt, int nso) {
+    __shared__ T tiles[6][TE];
+    int64_t job;
+    int orbit, R[3];
+    if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
+    const dcc_s3_gather_job jb = jobs[job];
+    int64_t odd = jb.dst;
+    for (int p = 0; p < 6; ++p) odd |= jb.coef[p] != 0.0 ? jb.src[p] : 0;
+    if (PAIR && (odd & 1) == 0) s3_gather_orbit<T, true>(tiles, out, src, nvir, beta, jb, R);
+    else s3_gather_orbit<T, false>(tiles, out, src, nvir, beta, jb, R);
+}
+
 // Fused P3_full projection + denominator + t3 update + norms (one tile orbit).
-// Storage in T, arithmetic (projection, division, update, norms) in FP64.
-template <typename T>
+// Storage in T, arithmetic (projection, division, update, norms) in FP64.  PAIR: v even and the
+// three arrays aligned to two elements (checked on the host).
+template <typename T, bool PAIR = false>
 __global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const T *__restrict__ R, T *__restrict__ t3,
                                                                T *__restrict__ r3_out, int nvir,
                                                                const int *__restrict__ blk_ijk,
@@ -258,39 +309,39 @@ __global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const T *__restri
     const double *ew = w == 0 ? ei : (w == 1 ? ej : ek);
     const double eval = ew[has_e ? min(Rg[c] * E + i, nvir - 1) : 0];
     T old[6][kPer];
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
-#pragma unroll
-        for (int h = 0; h < kPer; ++h) {
-            int g[3];
-            int64_t off;
-            tile_elem(nvir, Rg, s, h, g, off);
-            old[s][h] = tb[off];
-        }
-    load_orbit(tiles, Rb, nvir, Rg);
+    prefetch_out<T, PAIR>(tb, nvir, Rg, old);
+    load_orbit<T, PAIR>(tiles, Rb, nvir, Rg);
     if (has_e) sden[w][c][i] = eval;
     __syncthreads();
     double nrm = 0.0, nact = 0.0;
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
         if (duplicate_tile(s, Rg)) continue;
+        T tnew[kPer], rout[kPer];
+        bool any_in = false;
 #pragma unroll
         for (int h = 0; h < kPer; ++h) {
+            const int e = elem_of<PAIR>(h);
             int g[3];
             int64_t off;
-            if (!tile_elem(nvir, Rg, s, h, g, off)) continue;
-            const int e = threadIdx.x + h * kThreads;
+            const bool in = tile_elem(nvir, Rg, s, e, g, off);
+            any_in |= in;
             double sym = 0.0;
 #pragma unroll
             for (int p = 0; p < 6; ++p) sym += permuted(tiles, s, p, e);
             const double own = permuted(tiles, s, 0, e);
             const double den = sden[0][kP[s][0]][e >> 6] + sden[1][kP[s][1]][(e >> 3) & 7] + sden[2][kP[s][2]][e & 7];
             const double r = (own - sym / 6.0) / den;
-            tb[off] = (T)((double)old[s][h] + r);
-            if (r3_out != nullptr) r3_out[blk * v3 + off] = (T)r;
-            nrm += r * r;
-            if (g[0] < nvir_diis && g[1] < nvir_diis && g[2] < nvir_diis) nact += r * r;
+            tnew[h] = (T)((double)old[s][h] + r);
+            rout[h] = (T)r;
+            if (in) {
+                nrm += r * r;
+                if (g[0] < nvir_diis && g[1] < nvir_diis && g[2] < nvir_diis) nact += r * r;
+            }
         }
+        if (!any_in) continue;
+        store2<T, PAIR>(tb, nvir, Rg, s, tnew);
+        if (r3_out != nullptr) store2<T, PAIR>(r3_out + blk * v3, nvir, Rg, s, rout);
     }
     // fixed-order reduction: wave shuffles, then the four wave sums in order
 #pragma unroll
@@ -343,6 +394,13 @@ __global__ __launch_bounds__(256) void reduce_pairs_kernel(const double *__restr
 
 namespace {
 
+// The paired element mapping needs v even (pairs never straddle a row end) and both arrays aligned
+// to two elements; per-job offsets are checked in the kernels.
+template <typename A, typename B> bool pair_ok(int64_t nvir, const A *a, const B *b) {
+    return nvir % 2 == 0 && reinterpret_cast<uintptr_t>(a) % (2 * sizeof(A)) == 0 &&
+           reinterpret_cast<uintptr_t>(b) % (2 * sizeof(B)) == 0;
+}
+
 template <typename T>
 int s3_gather(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_gather_job *jobs, int64_t njobs,
               dcc_stream_t stream) {
@@ -351,8 +409,12 @@ int s3_gather(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_gath
     DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
     const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
     DCC_CHECK(8 * nso * njobs < ((int64_t)1 << 31), "grid too large");
-    hipLaunchKernelGGL(s3_gather_kernel<T>, dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
-                       dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
+    if (pair_ok(nvir, out, src))
+        hipLaunchKernelGGL((s3_gather_kernel<T, true>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+                           dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
+    else
+        hipLaunchKernelGGL((s3_gather_kernel<T, false>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+                           dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -365,8 +427,12 @@ int s3_transform(O *out, const T *src, int64_t nvir, double beta, const dcc_s3_j
     DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
     const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
     DCC_CHECK(8 * nso * njobs < ((int64_t)1 << 31), "grid too large");
-    hipLaunchKernelGGL((s3_transform_kernel<T, O>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
-                       dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
+    if (pair_ok(nvir, out, src))
+        hipLaunchKernelGGL((s3_transform_kernel<T, O, true>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+                           dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
+    else
+        hipLaunchKernelGGL((s3_transform_kernel<T, O, false>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads),
+                           0, dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -381,9 +447,14 @@ int r3_finalize(const T *R, T *t3, T *r3_out, int64_t nblk, int64_t nvir, const 
     const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
     DCC_CHECK(8 * nso * nblk < ((int64_t)1 << 31), "grid too large");
     if (nvir_diis < 0 || nvir_diis > nvir) nvir_diis = nvir;
-    hipLaunchKernelGGL(r3_finalize_kernel<T>, dim3((unsigned)(8 * nso * nblk)), dim3(kThreads), 0,
-                       dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis, partials,
-                       (int)nt, (int)nso);
+    if (pair_ok(nvir, R, t3) && (r3_out == nullptr || pair_ok(nvir, r3_out, r3_out)))
+        hipLaunchKernelGGL((r3_finalize_kernel<T, true>), dim3((unsigned)(8 * nso * nblk)), dim3(kThreads), 0,
+                           dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis,
+                           partials, (int)nt, (int)nso);
+    else
+        hipLaunchKernelGGL((r3_finalize_kernel<T, false>), dim3((unsigned)(8 * nso * nblk)), dim3(kThreads), 0,
+                           dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis,
+                           partials, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
 }
