@@ -961,7 +961,7 @@ class TriplesEngine:
 
     def set_tiles_by_name(self, tiles):
         """Apply a tiles_by_name() map; launches without an entry keep their tile.  A split-BF16x3
-        tile side of 256 (4 and 9: 256x128, 5 and 10: 128x256, 6, 7, 8 and 11: 256x256) is halved to 128
+        tile side of 256 (4 and 9: 256x128, 5 and 10: 128x256, 6, 7, 8, 11, 12 and 13: 256x256) is halved to 128
         on a launch whose
         extent on that side is at most 128 here (a chunked launch of a larger system), where it would
         be mostly padding."""
@@ -969,9 +969,9 @@ class TriplesEngine:
             if isinstance(g, GemmLaunch) and g.name in tiles:
                 t = int(tiles[g.name])
                 if g.fn.endswith("bf16x3") and t >= 4:
-                    wide_m = t in (4, 6, 7, 8, 9, 11) and g.max_rows > 128
-                    wide_n = t in (5, 6, 7, 8, 10, 11) and g.args[1] > 128
-                    t = {(True, True): t if t in (6, 7, 8, 11) else 6, (True, False): 9 if t == 9 else 4,
+                    wide_m = t in (4, 6, 7, 8, 9, 11, 12, 13) and g.max_rows > 128
+                    wide_n = t in (5, 6, 7, 8, 10, 11, 12, 13) and g.args[1] > 128
+                    t = {(True, True): t if t in (6, 7, 8, 11, 12, 13) else 6, (True, False): 9 if t == 9 else 4,
                          (False, True): 10 if t == 10 else 5, (False, False): 3}[(wide_m, wide_n)]
                 g.tile = t
 
@@ -980,15 +980,16 @@ class TriplesEngine:
         FP64/FP32 kernels (four shapes x one- or two-stage prefetch), twelve for split-BF16x3
         (the four of 256 threads and 256x128, 128x256, 256x256 of 512 threads on 16 x 16 x 32 MFMAs,
         256x256 on 32 x 32 x 16 MFMAs, and four of those staged by direct global -> LDS requests: 256x256
-        with two stages, 256x128 and 128x256 with three, 256x256 on 32 x 32 x 16 with two; a launch
-        whose operands do not allow the direct requests runs the register-staged tile instead).
+        with two stages, 256x128 and 128x256 with three, 256x256 on 32 x 32 x 16 with two, and the
+        256x256 tile on four waves with AGPR accumulators on either MFMA shape; a launch whose operands
+        do not allow the direct requests runs the register-staged tile instead).
 
         Run it before a residual (it accumulates into R and the scratch buffers); returns
         {launch name: (tile, ms)}."""
         from distr_cc.device import Event
         shapes = ("64x64", "128x64", "64x128", "128x128", "64x64s1", "128x64s1", "64x128s1", "128x128s1")
         shapes3 = ("64x64", "128x64", "64x128", "128x128", "256x128", "128x256", "256x256", "256x256m32",
-                   "256x256lds2", "256x128lds3", "128x256lds3", "256x256m32lds2")
+                   "256x256lds2", "256x128lds3", "128x256lds3", "256x256m32lds2", "256x256w4", "256x256m32w4")
         launches = self._tunable(include_lower)
         out = {}
         st = self.stream

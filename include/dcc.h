@@ -153,10 +153,11 @@ int dcc_contract_packed_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, do
 /* dcc_contract_f32_bf16x3 with an explicit workgroup tile: tile_shape 0 = 64x64, 1 = 128x64,
  * 2 = 64x128, 3 = 128x128 (256 threads), 4 = 256x128, 5 = 128x256, 6 = 256x256 (512 threads) (M x N) on
  * v_mfma_f32_16x16x32_bf16, 7 = 256x256 (512 threads) on v_mfma_f32_32x32x16_bf16; 8 = 256x256, 9 = 256x128,
- * 10 = 128x256, 11 = 256x256 on 32x32x16, with operands staged by direct global -> LDS requests (two, three,
- * three and two stages) when both packed operands are K-contiguous in 16-byte quads and K is a positive
- * multiple of 32 (otherwise the register-staged tile 6, 4, 5 or 7 runs); -1 picks 128 on each side
- * unless that side has at most 64 rows.
+ * 10 = 128x256, 11 = 256x256 on 32x32x16, 12 and 13 = 256x256 on four waves (128x128 per wave, AGPR
+ * accumulators) on 16x16x32 and 32x32x16, with operands staged by direct global -> LDS requests (two, three,
+ * three, two, two and two stages) when both packed operands are K-contiguous in 16-byte quads and K is a
+ * positive multiple of 32 x stages (otherwise the register-staged tile 6, 4, 5, 7, 6 or 7 runs); -1 picks
+ * 128 on each side unless that side has at most 64 rows.
  * No reference counterpart (as dcc_contract_f32_bf16x3). */
 int dcc_contract_tiled_f32_bf16x3(int64_t M, int64_t N, int64_t K, double alpha, double beta, const float *A,
                                   dcc_idxmap am, dcc_idxmap ak, const float *B, dcc_idxmap bk, dcc_idxmap bn,

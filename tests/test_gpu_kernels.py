@@ -310,18 +310,19 @@ def test_contract_f32_plain_gemm(dev, M, N, K):
     assert err < 3e-6 * max(1.0, abs(ref).max()) * np.sqrt(K), err
 
 
-@pytest.mark.parametrize("tile", [8, 9, 10, 11])
+@pytest.mark.parametrize("tile", [8, 9, 10, 11, 12, 13])
 @pytest.mark.parametrize("c_mt", [False, True])
 @pytest.mark.parametrize("pad", [8, 6])
 def test_contract_bf16x3_dma_tiles(dev, tile, c_mt, pad):
-    """Tiles 8..11 (operands staged by direct global -> LDS requests, two or three stages in flight) on
-    packed, K-contiguous A and B with K a multiple of 32, partial M/N tiles, both C layouts; batch offsets
-    of 8 words take the 16-byte requests, 6 words the 4-byte ones.  Checked against the same split
-    evaluated exactly on the host, and against the register-staged kernel of the same tile."""
+    """Tiles 8..13 (operands staged by direct global -> LDS requests, two or three stages in flight; 12
+    and 13 on four waves with AGPR accumulators) on packed, K-contiguous A and B with K a multiple of
+    32 x stages, partial M/N tiles, both C layouts; batch offsets of 8 words take the 16-byte requests,
+    6 words the 4-byte ones.  Checked against the same split evaluated exactly on the host, and against
+    the register-staged kernel of the same tile."""
     from abi_emulator import bf16_split
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
-    M, N, K = 300, 260, 160
+    M, N, K = 300, 260, 192
     _say("contract bf16x3 direct-to-LDS tile %d c_mt %d pad %d" % (tile, c_mt, pad))
     rng = np.random.default_rng(70 + tile)
     A = rng.normal(size=(M, K)).astype(np.float32)
@@ -341,7 +342,7 @@ def test_contract_bf16x3_dma_tiles(dev, tile, c_mt, pad):
     L.call("dcc_pack_bf16x3", dA.vp, pA.vp, hA.size, None)
     L.call("dcc_pack_bf16x3", dB.vp, pB.vp, hB.size, None)
     outs = []
-    for t in (tile, {8: 6, 9: 4, 10: 5, 11: 7}[tile]):
+    for t in (tile, {8: 6, 9: 4, 10: 5, 11: 7, 12: 6, 13: 7}[tile]):
         dC = DeviceArray.from_host(hC)
         L.call("dcc_contract_packed_bf16x3", M, N, K, 1.5, 0.5, pA.vp, am, ak, pB.vp, bk, bn, dC.vp, cm, cn,
                dbt.vp, 1, M, t, None)
