@@ -477,8 +477,9 @@ def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT):
 def probe_residual(o, v, precision, tiles, seed=20261202):
     """One triples engine of the given precision on the synthetic system (nocc=o, nvir=v): tiles loaded
     from ``tiles`` if it exists (a tile list of this size or a {launch name: tile} map), else autotuned
-    (and saved there by launch name when a path is given); the residual is
-    run twice and the second one is timed, per launch.  Used to profile one precision's kernels."""
+    (and saved there by launch name when a path is given); the residual is run twice and the second one
+    is timed, per launch, followed by one fused projection/denominator/update.  Used to profile one
+    precision's kernels."""
     from distr_cc.device import Event, Stream
     from distr_cc.synthetic import SyntheticSystem
     from distr_cc.t3_engine import TriplesEngine
@@ -509,6 +510,12 @@ def probe_residual(o, v, precision, tiles, seed=20261202):
     a.record(st)
     eng.residual(timer=timer)
     b.record(st)
+    # then the fused projection/denominator/update of the same R (denominators from the orbital energies)
+    eng.set_denominators(sysm.mo_energy)
+    c, d = Event(), Event()
+    c.record(st)
+    eng.finalize_launch()
+    d.record(st)
     st.synchronize()
     launches = []
     for g in eng.launches:
@@ -518,7 +525,7 @@ def probe_residual(o, v, precision, tiles, seed=20261202):
         if starts and ends:
             launches.append({"name": g.name, "ms": starts[0].elapsed_ms(ends[0]), "flops": getattr(g, "flops", 0.0)})
     print(json.dumps({"probe": "triples residual", "precision": precision, "nocc": o, "nvir": v,
-                      "residual_ms": a.elapsed_ms(b), "launches": launches}))
+                      "residual_ms": a.elapsed_ms(b), "finalize_ms": c.elapsed_ms(d), "launches": launches}))
 
 
 def tune_tiles(o, v, precision, sigma_b, seed=20261202):

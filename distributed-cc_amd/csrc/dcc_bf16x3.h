@@ -50,7 +50,9 @@ template <> struct AccOf<32> { typedef flt16 t; static constexpr int n = 16; };
 // kernel), and no VGPRs hold staged operands.  Each stage is its own __shared__ array and the loop is
 // unrolled by S, so the compiler sees which stage every fragment read uses and does not wait for the
 // loads of the other stages; the waits are explicit (s_waitcnt vmcnt) before a raw s_barrier.
-// Stage layout: [row][32 packed words], the eight 16-byte chunks of a row XOR-swizzled by row & 7;
+// Stage layout: [row][32 packed words], the eight 16-byte chunks of a row XOR-swizzled by
+// swz(row) = (row & 7) ^ ((row >> 3) & 1), which keeps the 16-byte fragment reads of 8 (32 banks) or
+// 16 (64 banks) consecutive rows on distinct banks;
 // fragments are two 16-byte reads of packed words, split into hi and lo halves (v_perm_b32).
 // Requirements (checked on the host): A and B K-contiguous with 16-byte quads, K a positive multiple of
 // 32 S (whole passes of S steps).  A batch whose offsets are not multiples of four words is staged with 4-byte loads into the same
@@ -119,7 +121,7 @@ __device__ __forceinline__ void dma_tile(const ContractArgs<float> &p, int64_t b
         return n < p.N ? bt.b_off + map_off(p.bn, (uint32_t)n, batch) : 0;
     };
     // 16-byte path: request i of this wave covers rows (NW i + wave) 8 .. + 7, lane -> row lane / 8,
-    // LDS chunk lane % 8 holding global chunk (lane % 8) ^ (row & 7) = (lane % 8) ^ (lane / 8); the
+    // LDS chunk lane % 8 holding global chunk (lane % 8) ^ swz(row); the
     // row offsets are kept.  4-byte path (misaligned batches only): request i covers rows
     // (NW i + wave) 2 .. + 1, lane -> row lane / 32, word lane % 32; row offsets recomputed per request
     int64_t ra[QUAD ? DA : 1], rb[QUAD ? DB : 1];
@@ -129,7 +131,10 @@ __device__ __forceinline__ void dma_tile(const ContractArgs<float> &p, int64_t b
 #pragma unroll
         for (int i = 0; i < DB; ++i) rb[i] = brow((NW * i + wave) * 8 + lane / 8);
     }
-    const int gchunk = (lane % 8) ^ (lane / 8);                       // 16-byte path
+    auto swz = [](int row) __attribute__((always_inline)) { return (row & 7) ^ ((row >> 3) & 1); };
+    // 16-byte path: row = (NW i + wave) 8 + lane / 8 with NW even, so swz(row) = (lane / 8) ^ (wave & 1)
+    // for every request i
+    const int gchunk = (lane % 8) ^ (lane / 8) ^ (wave & 1);
     const int w4 = lane % 32, row4 = lane / 32;                       // 4-byte path: word, row in pair
 
     auto fill_table = [&](int t, int k0) __attribute__((always_inline)) {
@@ -150,12 +155,12 @@ __device__ __forceinline__ void dma_tile(const ContractArgs<float> &p, int64_t b
         } else {
             for (int i = 0; i < DA4; ++i) {
                 const int row = (NW * i + wave) * 2 + row4;
-                const int k = 4 * ((w4 / 4) ^ (row & 7)) + (w4 % 4);
+                const int k = 4 * ((w4 / 4) ^ swz(row)) + (w4 % 4);
                 dma4(Ap + arow(row) + ktab[slot][0][k], sa + (NW * i + wave) * 2 * kBK3);
             }
             for (int i = 0; i < DB4; ++i) {
                 const int row = (NW * i + wave) * 2 + row4;
-                const int k = 4 * ((w4 / 4) ^ (row & 7)) + (w4 % 4);
+                const int k = 4 * ((w4 / 4) ^ swz(row)) + (w4 % 4);
                 dma4(Bp + brow(row) + ktab[slot][1][k], sb + (NW * i + wave) * 2 * kBK3);
             }
         }
@@ -176,8 +181,8 @@ __device__ __forceinline__ void dma_tile(const ContractArgs<float> &p, int64_t b
     // fragment (hi, lo) of rows rbase.. of a stage for k chunks c, c + 1 (k = 4 c .. 4 c + 7)
     auto frag = [&](const uint32_t *st, int row, int c, bf16x8 &hi, bf16x8 &lo) __attribute__((always_inline)) {
         typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-        const u32x4 x = *reinterpret_cast<const u32x4 *>(st + row * kBK3 + 4 * (c ^ (row & 7)));
-        const u32x4 y = *reinterpret_cast<const u32x4 *>(st + row * kBK3 + 4 * ((c + 1) ^ (row & 7)));
+        const u32x4 x = *reinterpret_cast<const u32x4 *>(st + row * kBK3 + 4 * (c ^ swz(row)));
+        const u32x4 y = *reinterpret_cast<const u32x4 *>(st + row * kBK3 + 4 * ((c + 1) ^ swz(row)));
         uint32_t h[4], l[4];
         split_pair(x[0], x[1], h[0], l[0]);
         split_pair(x[2], x[3], h[1], l[1]);
