@@ -97,7 +97,7 @@ __device__ __forceinline__ void dma_tile(const ContractArgs<float> &p, int64_t b
     constexpr int FM = WM / MI, FN = WN / MI;
     typedef typename AccOf<MI>::t acc_t;
     constexpr int NACC = AccOf<MI>::n;
-    constexpr int AW = DmaSmem<BM, BN, S>::AW, BW = DmaSmem<BM, BN, S>::BW;
+    constexpr int AW = DmaSmem<BM, BN, S>::AW;
     // 16-byte loads: a wave request covers 8 rows of 128 bytes; per wave and tile DA + DB requests
     constexpr int DA = BM / (8 * NW), DB = BN / (8 * NW), DPT = DA + DB;
     static_assert(DA * 8 * NW == BM && DB * 8 * NW == BN, "rows per request");

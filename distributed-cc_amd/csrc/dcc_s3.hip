@@ -1,13 +1,14 @@
 // HBM-bound transforms of v^3 blocks under the symmetric group S3 of (a,b,c).
 //
-// Every permutation-type operation on a T3 block -- taking a permuted block,
-// spin summation, the P3_full projection -- reads element x of the output
-// from elements p(x) of the input for the six axis permutations p.  A
-// workgroup owns one *tile orbit*: an 8x8x8 tile with tile coordinates
-// (A,B,C) together with its five permuted tiles.  The six input tiles are
-// read into LDS along their contiguous axis (runs of 8 doubles), every
-// permuted access is then an LDS access, and the six output tiles are written
-// along their contiguous axis.  Each element is read and written exactly once.
+// Every permutation-type operation on a T3 block -- taking a permuted block, spin summation, the
+// P3_full projection -- reads element x of the output from elements p(x) of the input for the six
+// axis permutations p.  A workgroup computes one 16x16x16 output tile.  For every permutation p with
+// a nonzero coefficient it reads the source tile at the p-permuted tile coordinates, along that
+// tile's contiguous axis (rows of 16 elements: whole 128-byte lines in FP64), into LDS; the permuted
+// accesses are LDS accesses, and the next source tile is loaded while the current one is read.  The
+// output tile accumulates in registers and is written once along its contiguous axis.  The six tiles
+// of a tile orbit (a tile and its five permuted tiles) run as consecutive workgroups on one XCD, so
+// a source block that serves all six permutations is read from HBM once and re-read from L2.
 #include "dcc_common.h"
 #include <mutex>
 #include <type_traits>
@@ -15,19 +16,28 @@
 
 namespace {
 
-constexpr int E = 8;            // tile edge
-constexpr int TE = E * E * E;   // elements per tile
-constexpr int kThreads = 256;
-constexpr int kPer = TE / kThreads;  // elements per thread per tile
+constexpr int E = 16;                  // tile edge
+constexpr int TE = E * E * E;          // elements per tile
+constexpr int kThreads = 512;
+constexpr int kPer = TE / kThreads;    // elements per thread (8)
+// LDS strides (elements) of a staged tile, x slowest: rows padded to 17 and planes to 275 elements,
+// so that the lanes of a wave reading the tile along any of its three axes spread over the banks (in
+// an unpadded 16^3 tile a read along the slowest axis puts every lane of a row on one bank)
+constexpr int kSY = E + 1, kSX = E * kSY + 3, kLdsTile = E * kSX;
+// workgroups per 2x2x2 block of tile orbits: 8 orbit slots x 6 permuted tiles
+constexpr int kRun = 48;
 
 // The six permutations in the reference's tp_t3 order (src/rccsdt.c:32-39);
 // X.transpose(p)[x] = X[y] with y[p[n]] = x[n].
 __constant__ int kP[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
 
-__device__ __forceinline__ int perm_index(int a, int b, int c) {
-    // index in kP of the permutation (a,b,c)
-    return a == 0 ? (b == 1 ? 0 : 1) : (a == 1 ? (b == 0 ? 2 : 3) : (b == 0 ? 4 : 5));
-}
+// Tile coordinates (x slowest).  Three named values rather than an array: picking one by a run-time
+// axis index is then a select of values, not an indexed access that would place the array in
+// scratch memory.
+struct I3 {
+    int x, y, z;
+};
+__device__ __forceinline__ int pick3(int i, int a, int b, int c) { return i == 0 ? a : (i == 1 ? b : c); }
 
 // Orbit o -> tile triple (A >= B >= C) with o = A(A+1)(A+2)/6 + B(B+1)/2 + C.
 __device__ __forceinline__ void decode_orbit(int o, int &A, int &B, int &C) {
@@ -40,6 +50,256 @@ __device__ __forceinline__ void decode_orbit(int o, int &A, int &B, int &C) {
 }
 
 __host__ __device__ __forceinline__ int64_t n_orbits(int64_t nt) { return nt * (nt + 1) * (nt + 2) / 6; }
+
+// Workgroup b of a launch over jobs of nt^3 tiles -> (job, output tile coordinates Tc).  Workgroups
+// come in runs of 48 per 2x2x2 block of tile orbits (nso such blocks per job): 8 orbit slots (A >= B
+// >= C and A < nt, else exit) times the six permuted tiles s of the orbit, tile s having coordinates
+// (R[p_s[0]], R[p_s[1]], R[p_s[2]]) (a tile that repeats an earlier one of its orbit exits).
+// Consecutive workgroups run on one XCD (xcd_remap): the six tiles of an orbit read the same source
+// tiles while they are in that XCD's L2, and in FP32 (rows of 64 bytes, half a line) the tiles of a
+// 2x2x2 block read and write the two halves of their lines together.
+__device__ __forceinline__ bool tile_of(int64_t b, int64_t total, int nt, int nso, int64_t &job, I3 &Tc) {
+    const int64_t L = dcc_gemm::xcd_remap(b, total);
+    job = L / (kRun * (int64_t)nso);
+    const int r = (int)(L - job * kRun * (int64_t)nso);
+    const int slot = (r / 6) & 7, s = r % 6;
+    int SA, SB, SC;
+    decode_orbit(r / kRun, SA, SB, SC);
+    const int A = 2 * SA + ((slot >> 2) & 1), B = 2 * SB + ((slot >> 1) & 1), C = 2 * SC + (slot & 1);
+    if (!(A >= B && B >= C && A < nt)) return false;
+    Tc = I3{pick3(kP[s][0], A, B, C), pick3(kP[s][1], A, B, C), pick3(kP[s][2], A, B, C)};
+    for (int u = 0; u < s; ++u)
+        if (pick3(kP[u][0], A, B, C) == Tc.x && pick3(kP[u][1], A, B, C) == Tc.y && pick3(kP[u][2], A, B, C) == Tc.z)
+            return false;
+    return true;
+}
+
+// The elements of a tile a thread owns (tile-local x slowest, z contiguous).  PAIR (v even, blocks
+// aligned to two elements): z = 2 (t % 8) + h, h < 2, of row y = (t / 8) % 16 in the planes
+// x = t / 128 + 4u, u < 4; each pair is moved with one vector access, and a wave covers 8 whole rows.
+// Scalar: z = t % 16, y = (t / 16) % 16, x = t / 256 + 2k, k < 8.
+template <bool PAIR> struct Own {
+    int x0, y, z0;
+    __device__ __forceinline__ Own() {
+        const int t = threadIdx.x;
+        x0 = PAIR ? t >> 7 : t >> 8;
+        y = PAIR ? (t >> 3) & (E - 1) : (t >> 4) & (E - 1);
+        z0 = PAIR ? 2 * (t & 7) : t & (E - 1);
+    }
+    static constexpr int dx(int k) { return PAIR ? 4 * (k >> 1) : 2 * k; }
+    static constexpr int dz(int k) { return PAIR ? (k & 1) : 0; }
+};
+
+template <typename T> struct Vec2 { typedef T t __attribute__((ext_vector_type(2))); };
+
+// This thread's elements of the tile at tile coordinates Tc of the block at blk, zero outside the
+// block.  Elements outside read the block's first element instead, so that every load is issued
+// unconditionally and all of them are in flight together.
+template <typename T, bool PAIR>
+__device__ __forceinline__ void load_tile(const T *__restrict__ blk, int nvir, const I3 Tc, const Own<PAIR> &w,
+                                          T v[kPer]) {
+    const int gx0 = Tc.x * E + w.x0, gy = Tc.y * E + w.y, gz = Tc.z * E + w.z0;
+    const bool yz = gy < nvir && gz < nvir;   // PAIR: gz even and nvir even, so gz + 1 < nvir too
+    const int64_t row = (int64_t)gy * nvir + gz, plane = (int64_t)nvir * nvir;
+    if constexpr (PAIR) {
+#pragma unroll
+        for (int u = 0; u < kPer / 2; ++u) {
+            const int gx = gx0 + Own<PAIR>::dx(2 * u);
+            const bool in = yz && gx < nvir;
+            const typename Vec2<T>::t x =
+                *reinterpret_cast<const typename Vec2<T>::t *>(blk + (in ? gx * plane + row : 0));
+            v[2 * u] = in ? x[0] : T(0);
+            v[2 * u + 1] = in ? x[1] : T(0);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int gx = gx0 + Own<PAIR>::dx(k);
+            const bool in = yz && gx < nvir;
+            const T x = blk[in ? gx * plane + row : 0];
+            v[k] = in ? x : T(0);
+        }
+    }
+}
+
+// Store this thread's elements of tile Tc of the block at blk (elements outside the block skipped).
+template <typename O, bool PAIR>
+__device__ __forceinline__ void store_tile(O *blk, int nvir, const I3 Tc, const Own<PAIR> &w, const O v[kPer]) {
+    const int gx0 = Tc.x * E + w.x0, gy = Tc.y * E + w.y, gz = Tc.z * E + w.z0;
+    if (!(gy < nvir && gz < nvir)) return;
+    const int64_t row = (int64_t)gy * nvir + gz, plane = (int64_t)nvir * nvir;
+    if constexpr (PAIR) {
+#pragma unroll
+        for (int u = 0; u < kPer / 2; ++u) {
+            const int gx = gx0 + Own<PAIR>::dx(2 * u);
+            if (gx < nvir)
+                *reinterpret_cast<typename Vec2<O>::t *>(blk + gx * plane + row) =
+                    typename Vec2<O>::t{v[2 * u], v[2 * u + 1]};
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int gx = gx0 + Own<PAIR>::dx(k);
+            if (gx < nvir) blk[gx * plane + row] = v[k];
+        }
+    }
+}
+
+// Write this thread's elements of a source tile into the LDS tile (its own layout).
+template <typename T, bool PAIR>
+__device__ __forceinline__ void stage(T *lds, const Own<PAIR> &w, const T v[kPer]) {
+    const int base = w.x0 * kSX + w.y * kSY + w.z0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) lds[base + Own<PAIR>::dx(k) * kSX + Own<PAIR>::dz(k)] = v[k];
+}
+
+__device__ __forceinline__ int lds_stride(int axis) { return axis == 0 ? kSX : (axis == 1 ? kSY : 1); }
+
+// This thread's output elements from the staged source tile through permutation p: output coordinate
+// n is the staged tile's coordinate along its axis p[n].
+template <typename T, bool PAIR>
+__device__ __forceinline__ void read_permuted(const T *lds, const Own<PAIR> &w, int p, double x[kPer]) {
+    const int s0 = lds_stride(kP[p][0]), s1 = lds_stride(kP[p][1]), s2 = lds_stride(kP[p][2]);
+    const int base = w.x0 * s0 + w.y * s1 + w.z0 * s2;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) x[k] = (double)lds[base + Own<PAIR>::dx(k) * s0 + Own<PAIR>::dz(k) * s2];
+}
+
+// Tile coordinates of the source tile that output tile Tc reads through permutation p (source axis
+// p[n] carries output tile coordinate n).
+__device__ __forceinline__ I3 source_tile(const I3 Tc, int p) {
+    const int p0 = kP[p][0], p1 = kP[p][1];
+    return I3{p0 == 0 ? Tc.x : (p1 == 0 ? Tc.y : Tc.z), p0 == 1 ? Tc.x : (p1 == 1 ? Tc.y : Tc.z),
+              p0 == 2 ? Tc.x : (p1 == 2 ? Tc.y : Tc.z)};
+}
+
+// acc[k] += coef[p] * (block src_of(p)).transpose(p) at this thread's elements of output tile Tc,
+// over the permutations p with coef[p] != 0 in ascending order (coef and src_of uniform over the
+// workgroup).  Source tiles pass through the one LDS tile; the next is loaded while the current one
+// is read.
+template <typename T, bool PAIR, typename SrcOf>
+__device__ __forceinline__ void accumulate(T *lds, int nvir, const I3 Tc, const Own<PAIR> &w, const double *coef,
+                                           SrcOf src_of, double acc[kPer]) {
+    int p = 0;
+    while (p < 6 && coef[p] == 0.0) ++p;
+    T nxt[kPer];
+    if (p < 6) load_tile<T, PAIR>(src_of(p), nvir, source_tile(Tc, p), w, nxt);
+#pragma unroll 1
+    while (p < 6) {
+        int pn = p + 1;
+        while (pn < 6 && coef[pn] == 0.0) ++pn;
+        __syncthreads();   // every wave has finished reading the previous source tile
+        stage<T, PAIR>(lds, w, nxt);
+        __syncthreads();
+        if (pn < 6) load_tile<T, PAIR>(src_of(pn), nvir, source_tile(Tc, pn), w, nxt);
+        double x[kPer];
+        read_permuted<T, PAIR>(lds, w, p, x);
+        const double c = coef[p];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) acc[k] += c * x[k];
+        p = pn;
+    }
+}
+
+// O: the output element type -- T, or uint32_t for FP32 input written as split-BF16x3 words
+// (dcc::pack_bf16x3 of the float result; beta must then be 0).
+template <typename T, typename O, bool PAIR>
+__device__ __forceinline__ void s3_transform_tile(T *lds, O *__restrict__ dst, const T *__restrict__ src, int nvir,
+                                                  double beta, const dcc_s3_job *jb, const I3 Tc) {
+    constexpr bool same = std::is_same<O, T>::value;
+    const Own<PAIR> w;
+    const bool rmw = same && beta != 0.0;
+    T old[kPer] = {};
+    if constexpr (same) {
+        if (rmw) load_tile<T, PAIR>(dst, nvir, Tc, w, old);
+    }
+    double acc[kPer] = {};
+    accumulate<T, PAIR>(lds, nvir, Tc, w, jb->coef, [&](int) { return src; }, acc);
+    O res[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        double val = acc[k];
+        if constexpr (same) {
+            if (rmw) val += beta * (double)old[k];
+            res[k] = (T)val;
+        } else {
+            res[k] = dcc::pack_bf16x3((float)val);
+        }
+    }
+    store_tile<O, PAIR>(dst, nvir, Tc, w, res);
+}
+
+template <typename T, typename O = T, bool PAIR = false>
+__global__ __launch_bounds__(kThreads) void s3_transform_kernel(O *__restrict__ out, const T *__restrict__ src,
+                                                                int nvir, double beta,
+                                                                const dcc_s3_job *__restrict__ jobs, int nt,
+                                                                int nso) {
+    __shared__ T lds[kLdsTile];
+    int64_t job;
+    I3 Tc;
+    if (!tile_of(blockIdx.x, gridDim.x, nt, nso, job, Tc)) return;
+    const dcc_s3_job *jb = jobs + job;
+    // the paired form needs both blocks to start on an even element (uniform over the workgroup)
+    if (PAIR && ((jb->dst | jb->src) & 1) == 0)
+        s3_transform_tile<T, O, true>(lds, out + jb->dst, src + jb->src, nvir, beta, jb, Tc);
+    else
+        s3_transform_tile<T, O, false>(lds, out + jb->dst, src + jb->src, nvir, beta, jb, Tc);
+}
+
+// Multi-source S3 transform: out[x] = beta*out[x] + sum_p c[p] * src_p.transpose(p)[x], with its own
+// source block src_p per permutation (for example the six permuted base blocks X[pi(ijk)] of a
+// symmetrised term); the output tile is read and written once.
+template <typename T, bool PAIR>
+__device__ __forceinline__ void s3_gather_tile(T *lds, T *__restrict__ out, const T *__restrict__ src, int nvir,
+                                               double beta, const dcc_s3_gather_job *jb, const I3 Tc) {
+    const Own<PAIR> w;
+    T *dst = out + jb->dst;
+    T old[kPer] = {};
+    if (beta != 0.0) load_tile<T, PAIR>(dst, nvir, Tc, w, old);
+    double acc[kPer] = {};
+    accumulate<T, PAIR>(lds, nvir, Tc, w, jb->coef, [&](int p) { return src + jb->src[p]; }, acc);
+    T res[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        double val = acc[k];
+        if (beta != 0.0) val += beta * (double)old[k];
+        res[k] = (T)val;
+    }
+    store_tile<T, PAIR>(dst, nvir, Tc, w, res);
+}
+
+template <typename T, bool PAIR = false>
+__global__ __launch_bounds__(kThreads) void s3_gather_kernel(T *__restrict__ out, const T *__restrict__ src,
+                                                             int nvir, double beta,
+                                                             const dcc_s3_gather_job *__restrict__ jobs,
+                                                             int nt, int nso) {
+    __shared__ T lds[kLdsTile];
+    int64_t job;
+    I3 Tc;
+    if (!tile_of(blockIdx.x, gridDim.x, nt, nso, job, Tc)) return;
+    const dcc_s3_gather_job *jb = jobs + job;
+    int64_t odd = jb->dst;
+    for (int p = 0; p < 6; ++p) odd |= jb->coef[p] != 0.0 ? jb->src[p] : 0;
+    if (PAIR && (odd & 1) == 0) s3_gather_tile<T, true>(lds, out, src, nvir, beta, jb, Tc);
+    else s3_gather_tile<T, false>(lds, out, src, nvir, beta, jb, Tc);
+}
+
+// The fused projection/update keeps the 8x8x8 tile orbits: one workgroup reads the six tiles of an
+// orbit of R from HBM once and writes the six output tiles.  In the per-output-tile form above the
+// six tiles of an orbit each re-read all six source tiles through L2, one after another, and the
+// projection ran 1.3x longer (nocc=10, nvir=90, FP64) and 1.2x longer (nocc=16, nvir=160, FP32).
+namespace orbit8 {
+
+constexpr int E = 8;            // tile edge
+constexpr int TE = E * E * E;   // elements per tile
+constexpr int kThreads = 256;
+constexpr int kPer = TE / kThreads;  // elements per thread per tile
+
+__device__ __forceinline__ int perm_index(int a, int b, int c) {
+    // index in kP of the permutation (a,b,c)
+    return a == 0 ? (b == 1 ? 0 : 1) : (a == 1 ? (b == 0 ? 2 : 3) : (b == 0 ? 4 : 5));
+}
+
 
 // Workgroup b of a launch over njobs blocks -> (job, orbit index, tile triple R).  A tile row is 64
 // bytes of doubles (32 of floats), so a 128-byte line is shared with tiles whose coordinate along the
@@ -85,8 +345,6 @@ __device__ __forceinline__ bool tile_elem(int nvir, const int R[3], int s, int e
     off = in ? ((int64_t)g[0] * nvir + g[1]) * nvir + g[2] : 0;
     return in;
 }
-
-template <typename T> struct Vec2 { typedef T t __attribute__((ext_vector_type(2))); };
 
 // This thread's two elements of orbit tile s of a block, loaded together (zero outside the block).
 template <typename T, bool PAIR>
@@ -170,115 +428,6 @@ __device__ __forceinline__ double permuted(const T (*tiles)[TE], int s, int p, i
     }
     const int sp = perm_index(sy[0], sy[1], sy[2]);
     return (double)tiles[sp][(y[0] << 6) | (y[1] << 3) | y[2]];
-}
-
-// O: the output element type -- T, or uint32_t for FP32 input written as split-BF16x3 words
-// (dcc::pack_bf16x3 of the float result; beta must then be 0).
-template <typename T, typename O, bool PAIR>
-__device__ __forceinline__ void s3_transform_orbit(T (*tiles)[TE], O *__restrict__ dst, const T *__restrict__ src,
-                                                   int nvir, double beta, const dcc_s3_job &jb, const int R[3]) {
-    O old[6][kPer];
-    if constexpr (std::is_same<O, T>::value) {
-        if (beta != 0.0) prefetch_out<T, PAIR>(dst, nvir, R, old);
-    }
-    load_orbit<T, PAIR>(tiles, src, nvir, R);
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        if (duplicate_tile(s, R)) continue;
-        O res[kPer];
-#pragma unroll
-        for (int h = 0; h < kPer; ++h) {
-            double val = 0.0;
-#pragma unroll
-            for (int p = 0; p < 6; ++p)
-                if (jb.coef[p] != 0.0) val += jb.coef[p] * permuted(tiles, s, p, elem_of<PAIR>(h));
-            if constexpr (std::is_same<O, T>::value) {
-                if (beta != 0.0) val += beta * (double)old[s][h];
-                res[h] = (T)val;
-            } else {
-                res[h] = dcc::pack_bf16x3((float)val);
-            }
-        }
-        store2<O, PAIR>(dst, nvir, R, s, res);
-    }
-}
-
-template <typename T, typename O = T, bool PAIR = false>
-__global__ __launch_bounds__(kThreads) void s3_transform_kernel(O *__restrict__ out, const T *__restrict__ src,
-                                                                int nvir, double beta,
-                                                                const dcc_s3_job *__restrict__ jobs, int nt,
-                                                                int nso) {
-    __shared__ T tiles[6][TE];
-    int64_t job;
-    int orbit, R[3];
-    if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
-    const dcc_s3_job jb = jobs[job];
-    // the paired form needs both blocks to start on an even element (uniform over the workgroup)
-    if (PAIR && ((jb.dst | jb.src) & 1) == 0)
-        s3_transform_orbit<T, O, true>(tiles, out + jb.dst, src + jb.src, nvir, beta, jb, R);
-    else
-        s3_transform_orbit<T, O, false>(tiles, out + jb.dst, src + jb.src, nvir, beta, jb, R);
-}
-
-// Multi-source S3 transform: out[x] = beta*out[x] + sum_p c[p] * src_p.transpose(p)[x], with its own
-// source block src_p per permutation (for example the six permuted base blocks X[pi(ijk)] of a
-// symmetrised term).  The orbit tiles of each source are staged in LDS in turn and the six output
-// tiles accumulate in registers, so the output block is read and written once.
-template <typename T, bool PAIR>
-__device__ __forceinline__ void s3_gather_orbit(T (*tiles)[TE], T *__restrict__ out, const T *__restrict__ src,
-                                                int nvir, double beta, const dcc_s3_gather_job &jb,
-                                                const int R[3]) {
-    double acc[6][kPer];
-#pragma unroll
-    for (int s = 0; s < 6; ++s)
-#pragma unroll
-        for (int h = 0; h < kPer; ++h) acc[s][h] = 0.0;
-    int64_t loaded = -1;
-    for (int p = 0; p < 6; ++p) {
-        if (jb.coef[p] == 0.0) continue;   // uniform across the workgroup
-        if (jb.src[p] != loaded) {
-            __syncthreads();                // the previous source's tiles are no longer read
-            load_orbit<T, PAIR>(tiles, src + jb.src[p], nvir, R);
-            __syncthreads();
-            loaded = jb.src[p];
-        }
-#pragma unroll
-        for (int s = 0; s < 6; ++s)
-#pragma unroll
-            for (int h = 0; h < kPer; ++h) acc[s][h] += jb.coef[p] * permuted(tiles, s, p, elem_of<PAIR>(h));
-    }
-    T *dst = out + jb.dst;
-    T old[6][kPer];
-    if (beta != 0.0) prefetch_out<T, PAIR>(dst, nvir, R, old);
-#pragma unroll
-    for (int s = 0; s < 6; ++s) {
-        if (duplicate_tile(s, R)) continue;
-        T res[kPer];
-#pragma unroll
-        for (int h = 0; h < kPer; ++h) {
-            double val = acc[s][h];
-            if (beta != 0.0) val += beta * (double)old[s][h];
-            res[h] = (T)val;
-        }
-        store2<T, PAIR>(dst, nvir, R, s, res);
-    }
-}
-
-template <typename T, bool PAIR = false>
-__global__ __launch_bounds__(kThreads) void s3_gather_kernel(T *__restrict__ out, const T *__restrict__ src,
-                                                             int nvir, double beta,
-                                                             const dcc_s3_gather_job *__restrict__ jobs,
-                                                             int nt, int nso) {
-    __shared__ T tiles[6][TE];
-    int64_t job;
-    int orbit, R[3];
-    if (!orbit_of(blockIdx.x, gridDim.x, nt, nso, job, orbit, R)) return;
-    const dcc_s3_gather_job jb = jobs[job];
-    int64_t odd = jb.dst;
-    for (int p = 0; p < 6; ++p) odd |= jb.coef[p] != 0.0 ? jb.src[p] : 0;
-    if (PAIR && (odd & 1) == 0) s3_gather_orbit<T, true>(tiles, out, src, nvir, beta, jb, R);
-    else s3_gather_orbit<T, false>(tiles, out, src, nvir, beta, jb, R);
 }
 
 // Fused P3_full projection + denominator + t3 update + norms (one tile orbit).
@@ -366,6 +515,8 @@ __global__ __launch_bounds__(kThreads) void r3_finalize_kernel(const T *__restri
     }
 }
 
+}  // namespace orbit8
+
 __global__ __launch_bounds__(256) void reduce_pairs_kernel(const double *__restrict__ x, int64_t n,
                                                             double *__restrict__ out) {
     __shared__ double sa[256], sb[256];
@@ -408,12 +559,12 @@ int s3_gather(T *out, const T *src, int64_t nvir, double beta, const dcc_s3_gath
     if (nvir == 0 || njobs == 0) return 0;
     DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
     const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
-    DCC_CHECK(8 * nso * njobs < ((int64_t)1 << 31), "grid too large");
+    DCC_CHECK(kRun * nso * njobs < ((int64_t)1 << 31), "grid too large");
     if (pair_ok(nvir, out, src))
-        hipLaunchKernelGGL((s3_gather_kernel<T, true>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+        hipLaunchKernelGGL((s3_gather_kernel<T, true>), dim3((unsigned)(kRun * nso * njobs)), dim3(kThreads), 0,
                            dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     else
-        hipLaunchKernelGGL((s3_gather_kernel<T, false>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+        hipLaunchKernelGGL((s3_gather_kernel<T, false>), dim3((unsigned)(kRun * nso * njobs)), dim3(kThreads), 0,
                            dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
@@ -426,12 +577,12 @@ int s3_transform(O *out, const T *src, int64_t nvir, double beta, const dcc_s3_j
     if (nvir == 0 || njobs == 0) return 0;
     DCC_CHECK(out != nullptr && src != nullptr && jobs != nullptr, "null pointer");
     const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
-    DCC_CHECK(8 * nso * njobs < ((int64_t)1 << 31), "grid too large");
+    DCC_CHECK(kRun * nso * njobs < ((int64_t)1 << 31), "grid too large");
     if (pair_ok(nvir, out, src))
-        hipLaunchKernelGGL((s3_transform_kernel<T, O, true>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads), 0,
+        hipLaunchKernelGGL((s3_transform_kernel<T, O, true>), dim3((unsigned)(kRun * nso * njobs)), dim3(kThreads), 0,
                            dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     else
-        hipLaunchKernelGGL((s3_transform_kernel<T, O, false>), dim3((unsigned)(8 * nso * njobs)), dim3(kThreads),
+        hipLaunchKernelGGL((s3_transform_kernel<T, O, false>), dim3((unsigned)(kRun * nso * njobs)), dim3(kThreads),
                            0, dcc::as_stream(stream), out, src, (int)nvir, beta, jobs, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
@@ -444,17 +595,17 @@ int r3_finalize(const T *R, T *t3, T *r3_out, int64_t nblk, int64_t nvir, const 
     if (nblk == 0 || nvir == 0) return 0;
     DCC_CHECK(R != nullptr && t3 != nullptr && blk_ijk != nullptr && eia != nullptr && partials != nullptr,
               "null pointer");
-    const int64_t nt = (nvir + E - 1) / E, nso = n_orbits((nt + 1) / 2);
+    const int64_t nt = (nvir + orbit8::E - 1) / orbit8::E, nso = n_orbits((nt + 1) / 2);
     DCC_CHECK(8 * nso * nblk < ((int64_t)1 << 31), "grid too large");
     if (nvir_diis < 0 || nvir_diis > nvir) nvir_diis = nvir;
     if (pair_ok(nvir, R, t3) && (r3_out == nullptr || pair_ok(nvir, r3_out, r3_out)))
-        hipLaunchKernelGGL((r3_finalize_kernel<T, true>), dim3((unsigned)(8 * nso * nblk)), dim3(kThreads), 0,
-                           dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis,
-                           partials, (int)nt, (int)nso);
+        hipLaunchKernelGGL((orbit8::r3_finalize_kernel<T, true>), dim3((unsigned)(8 * nso * nblk)),
+                           dim3(orbit8::kThreads), 0, dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia,
+                           (int)nvir_diis, partials, (int)nt, (int)nso);
     else
-        hipLaunchKernelGGL((r3_finalize_kernel<T, false>), dim3((unsigned)(8 * nso * nblk)), dim3(kThreads), 0,
-                           dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia, (int)nvir_diis,
-                           partials, (int)nt, (int)nso);
+        hipLaunchKernelGGL((orbit8::r3_finalize_kernel<T, false>), dim3((unsigned)(8 * nso * nblk)),
+                           dim3(orbit8::kThreads), 0, dcc::as_stream(stream), R, t3, r3_out, (int)nvir, blk_ijk, eia,
+                           (int)nvir_diis, partials, (int)nt, (int)nso);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -513,7 +664,7 @@ int dcc_s3_gather_f32(float *out, const float *src, int64_t nvir, double beta, c
 }
 
 int64_t dcc_r3_finalize_tiles(int64_t nvir) {
-    const int64_t nt = (nvir + E - 1) / E;
+    const int64_t nt = (nvir + orbit8::E - 1) / orbit8::E;
     return n_orbits(nt);
 }
 

@@ -9,7 +9,9 @@ import os
 
 import numpy as np
 
-_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "build", "libdcc.so")
+# DCC_LIBRARY selects another build of the library (for example one compiled with different options)
+_LIB_PATH = os.environ.get("DCC_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)), "..",
+                                                          "build", "libdcc.so")
 
 if not os.path.exists(_LIB_PATH):
     raise RuntimeError(

@@ -180,9 +180,9 @@ typedef struct dcc_s3_job {
 } dcc_s3_job;
 
 /*
- * Apply njobs S3 transforms to v^3 blocks.  Each workgroup stages the six
- * permuted 8x8x8 tiles of one tile orbit in LDS so that every HBM read and
- * write is along the contiguous axis.  Replaces take_t3_ijk_single_
+ * Apply njobs S3 transforms to v^3 blocks.  Each workgroup computes one 16x16x16
+ * output tile from the permuted source tiles, staged one at a time in LDS, so
+ * that every HBM read and write is along the contiguous axis.  Replaces take_t3_ijk_single_
  * (src/rccsdt.c:119-129), t3_single_spin_summation_inplace_
  * (src/rccsdt.c:131-233), t3_spin_summation_triple_sym_ (src/rccsdt.c:245-310)
  * and t3_transpose_add_ (src/rccsdt.c:320-375); out and src must not overlap.

@@ -77,10 +77,10 @@ def test_contract_transposed_operands(dev, M, N, K):
 
 
 @pytest.mark.parametrize("dt", ["f64", "f32"])
-@pytest.mark.parametrize("v", [1, 5, 8, 13, 20, 33])
+@pytest.mark.parametrize("v", [1, 5, 8, 13, 20, 33, 48, 50])
 def test_s3_transform_all_perms(dev, v, dt):
     """dcc_s3_transform_*: every single permutation and a mixed combination, FP64 and FP32 storage
-    (partial 8^3 tiles and several tile orbits)."""
+    (partial 16^3 tiles and several tile orbits)."""
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
     from distr_cc.t3_engine import s3_jobs_array
@@ -105,7 +105,7 @@ def test_s3_transform_all_perms(dev, v, dt):
         assert abs(out[n] - ref).max() < (1e-13 if dt == "f64" else 1e-6 * max(1.0, abs(ref).max())), (n, v)
 
 
-@pytest.mark.parametrize("v", [5, 8, 20])
+@pytest.mark.parametrize("v", [5, 8, 20, 50])
 def test_s3_transform_pack_f32(dev, v):
     """dcc_s3_transform_pack_f32 writes exactly the words of dcc_s3_transform_f32 followed by
     dcc_pack_bf16x3."""
@@ -126,7 +126,7 @@ def test_s3_transform_pack_f32(dev, v):
 
 @pytest.mark.parametrize("shift", [0, 1])
 @pytest.mark.parametrize("dt", ["f64", "f32"])
-@pytest.mark.parametrize("v", [1, 5, 8, 13, 20, 33])
+@pytest.mark.parametrize("v", [1, 5, 8, 13, 20, 33, 48, 50])
 def test_s3_gather_multi_source(dev, v, dt, shift):
     """dcc_s3_gather_*: a different source block per permutation, repeated sources, zero
     coefficients and beta accumulation, against NumPy; ``shift`` = 1 moves every block one element
