@@ -856,7 +856,9 @@ def main():
                    "mixed_vs_fp64": mixed},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_MFMA_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "contract_bf16x3_kernel (the three W_vvvv x T3 launches)",
+                     "kernel": "contract_bf16x3_dma_kernel / contract_bf16x3_kernel (the three W_vvvv x T3 "
+                               "launches: two with both operands staged by direct global->LDS requests, one "
+                               "register-staged)",
                      "flops_per_launch": 3.0 * flops, "fp32_equivalent_flops_per_launch": flops,
                      "avg_launch_ms": avg_ms,
                      "note": "achieved counts the three BF16 products (hi*hi + hi*lo + lo*hi) of every FP32 "
