@@ -378,15 +378,16 @@ def _fp64_solve(eng, amps, tol=1e-11, max_iter=60):
     return e, it
 
 
-def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT):
+def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT, solves=True):
     """The batched ensemble: nsys synthetic systems, nocc ~ U{3..8}, nvir ~ U{20..64} (system s drawn
     from seed + s, B-factor sigma ``sigma_b``), whole FP64 RCCSDT iterations of all systems per step
     (lower order, T3 -> r1/r2/W, triples residual as grouped persistent launches with autotuned tiles,
     update, energy) captured in one HIP graph and replayed ``steps`` times from the MP2/T3 guess (the
     timed region).  Then every system is solved on its own (its engine's ungrouped launches with DIIS,
-    to |dE| < 1e-11) and the ensemble's energies after the timed steps are compared with those solves.
-    With sigma 0.05 the plain Jacobi steps diverge for every system (and the MP2 guess energies are
-    tens of Eh), so the ensemble runs at sigma 0.01, like the nocc=16, nvir=160 workload."""
+    to |dE| < 1e-11) and the ensemble's energies after the timed steps are compared with those solves
+    (skipped with ``solves=False``, for profiling the ensemble's own launches).  With sigma 0.05 the
+    plain Jacobi steps diverge for every system (and the MP2 guess energies are tens of Eh), so the
+    ensemble runs at sigma 0.01, like the nocc=16, nvir=160 workload."""
     from distr_cc.batched import BatchedRCCSDT
     from distr_cc.device import Event, Stream
     from distr_cc.synthetic import SyntheticSystem
@@ -447,9 +448,9 @@ def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT):
     last_de = [abs(x - y) for x, y in zip(e_batched, e_prev)]
     print("bench: ensemble %.1f ms per step; per-system FP64 DIIS solves" % ms, file=sys.stderr, flush=True)
     t1 = time.perf_counter()
-    solves = [_fp64_solve(e, amps[k]) for k, e in enumerate(bt.engines)]
+    sol = [_fp64_solve(e, amps[k]) for k, e in enumerate(bt.engines)] if solves else []
     solve_s = time.perf_counter() - t1
-    de = [abs(e_batched[k] - solves[k][0]) for k in range(nsys)]
+    de = [abs(e_batched[k] - sol[k][0]) for k in range(len(sol))] or [float("nan")]
     shapes = {}
     for k, (shape, _) in tuned.items():
         shapes[shape] = shapes.get(shape, 0) + 1
@@ -465,7 +466,7 @@ def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT):
            "max_abs_dE_last_step": float(max(last_de)),
            "max_abs_dE_vs_per_system_fp64_solves": float(max(de)),
            "median_abs_dE_vs_per_system_fp64_solves": float(np.median(de)),
-           "per_system_solve_iterations_min_max": [min(x[1] for x in solves), max(x[1] for x in solves)],
+           "per_system_solve_iterations_min_max": [min(x[1] for x in sol), max(x[1] for x in sol)] if sol else None,
            "per_system_solves_s": solve_s,
            "finite_energies": bool(np.all(np.isfinite(e_batched)))}
     bt.close()
@@ -770,6 +771,12 @@ def main():
     ap.add_argument("--probe-residual", choices=("fp64", "fp32", "bf16x3"), default=None,
                     help="only time one engine's residual at --nocc/--nvir (tiles via --tiles) and exit")
     ap.add_argument("--tiles", metavar="JSON", default=None, help="tile file of --probe-residual")
+    ap.add_argument("--probe-batched", type=int, default=0, metavar="NSYS",
+                    help="only the batched ensemble of NSYS systems (--steps replays, no per-system solves); "
+                         "prints its JSON (for profiling)")
+    ap.add_argument("--probe-q", type=int, default=0, metavar="BLKSIZE",
+                    help="only the (Q) timing at --nocc/--nvir with this block size; prints its JSON "
+                         "(for profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -790,6 +797,12 @@ def main():
         return
     if args.tune_tiles:
         tune_tiles(o, v, args.tune_tiles, args.sigma_b)
+        return
+    if args.probe_batched:
+        print(json.dumps(batched_ensemble(args.probe_batched, steps=args.steps, solves=False)))
+        return
+    if args.probe_q:
+        print(json.dumps(q_time(o, v, blksize=args.probe_q)))
         return
 
     t_start = time.perf_counter()
