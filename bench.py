@@ -378,14 +378,16 @@ def _fp64_solve(eng, amps, tol=1e-11, max_iter=60):
     return e, it
 
 
-def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT, solves=True):
+def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT, solves=True, graph=True):
     """The batched ensemble: nsys synthetic systems, nocc ~ U{3..8}, nvir ~ U{20..64} (system s drawn
     from seed + s, B-factor sigma ``sigma_b``), whole FP64 RCCSDT iterations of all systems per step
     (lower order, T3 -> r1/r2/W, triples residual as grouped persistent launches with autotuned tiles,
     update, energy) captured in one HIP graph and replayed ``steps`` times from the MP2/T3 guess (the
     timed region).  Then every system is solved on its own (its engine's ungrouped launches with DIIS,
     to |dE| < 1e-11) and the ensemble's energies after the timed steps are compared with those solves
-    (skipped with ``solves=False``, for profiling the ensemble's own launches).  With sigma 0.05 the
+    (skipped with ``solves=False``, for profiling the ensemble's own launches; ``graph=False`` issues the
+    same steps eagerly instead of replaying the graph, because rocprofv3's kernel tracing aborts the queue
+    on graph-launched kernels with a malformed-packet error).  With sigma 0.05 the
     plain Jacobi steps diverge for every system (and the MP2 guess energies are tens of Eh), so the
     ensemble runs at sigma 0.01, like the nocc=16, nvir=160 workload."""
     from distr_cc.batched import BatchedRCCSDT
@@ -428,7 +430,8 @@ def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT, sol
         else:
             phase_ms[name] = t0s.pop(name).elapsed_ms(e)
     restart()
-    bt.capture()
+    if graph:
+        bt.capture()
     bt.step()                          # warm-up replay
     st.synchronize()
     restart()
@@ -456,8 +459,8 @@ def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT, sol
         shapes[shape] = shapes.get(shape, 0) + 1
     out = {"workload": "%d synthetic systems, nocc~U{3..8}, nvir~U{20..64}, B-factor sigma %g, whole FP64 RCCSDT "
                        "iterations of every system per step (grouped persistent triples GEMMs with autotuned tiles, "
-                       "everything in one HIP graph per step), %d steps from the MP2/T3 guess"
-                       % (nsys, sigma_b, steps),
+                       "everything in one HIP graph per step), %d steps from the MP2/T3 guess%s"
+                       % (nsys, sigma_b, steps, "" if graph else " (steps issued eagerly, not replayed as a graph)"),
            "ms_per_step": ms, "system_iterations_per_s": nsys / (ms * 1e-3), "steps": steps,
            "triples_contraction_tflops": bt.flops() / (ms * 1e-3) / 1e12,
            "phase_ms_one_step_without_graph": phase_ms,
@@ -772,8 +775,8 @@ def main():
                     help="only time one engine's residual at --nocc/--nvir (tiles via --tiles) and exit")
     ap.add_argument("--tiles", metavar="JSON", default=None, help="tile file of --probe-residual")
     ap.add_argument("--probe-batched", type=int, default=0, metavar="NSYS",
-                    help="only the batched ensemble of NSYS systems (--steps replays, no per-system solves); "
-                         "prints its JSON (for profiling)")
+                    help="only the batched ensemble of NSYS systems (--steps steps issued eagerly, no "
+                         "per-system solves); prints its JSON (for profiling)")
     ap.add_argument("--probe-q", type=int, default=0, metavar="BLKSIZE",
                     help="only the (Q) timing at --nocc/--nvir with this block size; prints its JSON "
                          "(for profiling)")
@@ -799,7 +802,7 @@ def main():
         tune_tiles(o, v, args.tune_tiles, args.sigma_b)
         return
     if args.probe_batched:
-        print(json.dumps(batched_ensemble(args.probe_batched, steps=args.steps, solves=False)))
+        print(json.dumps(batched_ensemble(args.probe_batched, steps=args.steps, solves=False, graph=False)))
         return
     if args.probe_q:
         print(json.dumps(q_time(o, v, blksize=args.probe_q)))
