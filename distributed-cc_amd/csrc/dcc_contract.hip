@@ -115,7 +115,8 @@ template <int R, bool KFAST, int Q, int NT> struct Stage3 {
 // 128 x 128 and reads a quarter fewer fragment bytes from LDS per MFMA.
 // MI: the MFMA shape, 16 (v_mfma_f32_16x16x32_bf16, one K step of 32 per instruction) or 32
 // (v_mfma_f32_32x32x16_bf16, two K steps of 16: half the instructions for the same work).
-template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int MODE, int NW, int MI = 16>
+template <int BM, int BN, bool A_KFAST, bool B_NFAST, bool C_MFAST, int MODE, int NW, int MI = 16,
+          bool CVEC = false>
 __global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(2)))
 void contract_bf16x3_kernel(const ContractArgs<float> p) {
     constexpr bool PACKED = MODE >= 1;
@@ -423,6 +424,58 @@ void contract_bf16x3_kernel(const ContractArgs<float> p) {
         }
         return;
     }
+    if constexpr (MI == 16 && CVEC) {
+        // C contiguous along n with rows of 16-byte vectors (checked on the host; the batch offset
+        // here): a band of rows -- fragment i of every wave row -- is staged in LDS, then each wave
+        // writes whole rows with 16-byte stores (written directly from the accumulator layout, a store
+        // covers 64-byte pieces of four rows).  A separate instance: in the long-K kernels the extra
+        // code cost a spill reload per k step (the register-staged W_vvvv launch ran 8 % longer).
+        if ((bt.c_off & 3) == 0) {
+            constexpr int ER = 16 * WGM, LDC = BN + 16, NV = BN / 4;
+            static_assert(ER * LDC * 4 <= (int)sizeof(smem), "epilogue band fits the stage buffers");
+            float *Cs = reinterpret_cast<float *>(smem);
+            const int brow = wm * 16 + 4 * (lane / 16);   // band row of this lane's r = 0
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                __syncthreads();   // the previous band (first pass: the main loop's buffers) is read
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+#pragma unroll
+                    for (int r = 0; r < NACC; ++r) Cs[(brow + r) * LDC + acc_col(j)] = acc[i][j][r];
+                __syncthreads();
+                for (int g = tid; g < ER * NV; g += NT) {
+                    const int lr = g / NV, c4 = g % NV;
+                    const int64_t m = m0 + (lr / 16) * WM + i * 16 + lr % 16;
+                    const int n = n0 + 4 * c4;
+                    if (m >= m_end || n >= p.N) continue;
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+                    const f4 x = *reinterpret_cast<const f4 *>(Cs + lr * LDC + 4 * c4);
+                    const int64_t crow = bt.c_off + map_off(p.cm, (uint32_t)m, batch);
+                    float *c = p.C + crow + map_off(p.cn, (uint32_t)n, batch);
+                    if (n + 3 < p.N) {
+                        f4 old = {0.f, 0.f, 0.f, 0.f};
+                        if (p.beta != 0.0) old = *reinterpret_cast<const f4 *>(c);
+                        f4 out;
+#pragma unroll
+                        for (int e = 0; e < 4; ++e) {
+                            double v = p.alpha * (double)x[e];
+                            if (p.beta != 0.0) v += p.beta * (double)old[e];
+                            out[e] = (float)v;
+                        }
+                        *reinterpret_cast<f4 *>(c) = out;
+                    } else {
+                        for (int e = 0; n + e < p.N; ++e) {
+                            float *ce = p.C + crow + map_off(p.cn, (uint32_t)(n + e), batch);
+                            double v = p.alpha * (double)x[e];
+                            if (p.beta != 0.0) v += p.beta * (double)*ce;
+                            *ce = (float)v;
+                        }
+                    }
+                }
+            }
+            return;
+        }
+    }
     int64_t ccol[FN];
     uint32_t cok = 0;
 #pragma unroll
@@ -571,22 +624,26 @@ int contract_entry(int64_t M, int64_t N, int64_t K, double alpha, double beta, c
     return launch<T, 64, 64>(args, max_rows, a_kfast, b_nfast, c_mfast, stages, s);
 }
 
-template <int BM, int BN, bool C_MFAST, int MODE, int NW, int MI>
+template <int BM, int BN, bool C_MFAST, int MODE, int NW, int MI, bool CVEC = false>
 void launch_bf16x3(ContractArgs<float> &args, bool a_kfast, bool b_nfast, dim3 grid, dim3 block,
                    hipStream_t stream) {
     if (a_kfast && b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, MODE, NW, MI>), grid, block, 0,
-                           stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, true, C_MFAST, MODE, NW, MI, CVEC>), grid, block,
+                           0, stream, args);
     else if (a_kfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, MODE, NW, MI>), grid, block, 0,
-                           stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, true, false, C_MFAST, MODE, NW, MI, CVEC>), grid, block,
+                           0, stream, args);
     else if (b_nfast)
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, MODE, NW, MI>), grid, block, 0,
-                           stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, true, C_MFAST, MODE, NW, MI, CVEC>), grid, block,
+                           0, stream, args);
     else
-        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, MODE, NW, MI>), grid, block, 0,
-                           stream, args);
+        hipLaunchKernelGGL((contract_bf16x3_kernel<BM, BN, false, false, C_MFAST, MODE, NW, MI, CVEC>), grid,
+                           block, 0, stream, args);
 }
+
+// K up to which the register-staged tiles of a C contiguous along n write C through the vector epilogue
+// (short-K GEMMs, whose time the epilogue's stores dominate)
+constexpr int64_t kVecEpilogueMaxK = 4096;
 
 template <int BM, int BN, int NW = 4, int MI = 16>
 int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool a_kfast, bool b_nfast,
@@ -601,9 +658,20 @@ int run_bf16x3(ContractArgs<float> &args, int64_t nbatch, int64_t max_rows, bool
         else if (mode == 1) launch_bf16x3<BM, BN, true, 1, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
         else launch_bf16x3<BM, BN, true, 0, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
     } else {
-        if (mode == 2) launch_bf16x3<BM, BN, false, 2, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
-        else if (mode == 1) launch_bf16x3<BM, BN, false, 1, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
-        else launch_bf16x3<BM, BN, false, 0, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
+        bool cvec = false;
+        if constexpr (MI == 16) {
+            // packed operands with a short K: the vector epilogue instance
+            cvec = mode >= 1 && (args.vec & 4) && args.K <= kVecEpilogueMaxK;
+            if (cvec) {
+                if (mode == 2) launch_bf16x3<BM, BN, false, 2, NW, MI, true>(args, a_kfast, b_nfast, grid, block, s);
+                else launch_bf16x3<BM, BN, false, 1, NW, MI, true>(args, a_kfast, b_nfast, grid, block, s);
+            }
+        }
+        if (!cvec) {
+            if (mode == 2) launch_bf16x3<BM, BN, false, 2, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
+            else if (mode == 1) launch_bf16x3<BM, BN, false, 1, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
+            else launch_bf16x3<BM, BN, false, 0, NW, MI>(args, a_kfast, b_nfast, grid, block, s);
+        }
     }
     DCC_LAUNCH_CHECK();
     return 0;
@@ -649,6 +717,9 @@ int contract_bf16x3_entry(int64_t M, int64_t N, int64_t K, double alpha, double 
     args.vec = quads ? 3
                      : (pairs_contiguous(A, am, ak, a_kfast, 0, 8) ? 1 : 0) |
                            (pairs_contiguous(B, bn, bk, !b_nfast, N, 8) ? 2 : 0);
+    // bit 2: C contiguous along n in aligned runs of 16-byte vectors (rows staged through LDS and
+    // written as vectors by the register-staged tiles)
+    if (!c_mfast && vectors_contiguous(C, cm, cn, true, 0, 4, 16)) args.vec |= 4;
     hipStream_t s = dcc::as_stream(stream);
     // default: 128x128 unless one side of the problem is small
     int shape = (int)tile_shape;

@@ -465,6 +465,46 @@ def test_contract_bf16x3_layouts(dev, layout, tile, packed, vec):
     assert np.all(dC.download()[:pad] == 0)
 
 
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5, 6, 7])
+def test_contract_bf16x3_vector_epilogue_tail(dev, tile):
+    """Packed operands, a short K and C contiguous along n with rows of stride 84 (16-byte vector rows)
+    but N = 82: the row tail of the vector epilogue is written element by element, and the two padding
+    columns of every row, the elements before the batch offset and after the last row keep their
+    values."""
+    from abi_emulator import bf16_split
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    M, N, K, ld, pad = 150, 82, 76, 84, 8
+    _say("contract bf16x3 vector epilogue tail tile %d" % tile)
+    rng = np.random.default_rng(90 + tile)
+    A = rng.normal(size=(M, K)).astype(np.float32)
+    B = rng.normal(size=(K, N)).astype(np.float32)
+    C0 = rng.normal(size=(M, ld)).astype(np.float32)
+    lin = lambda s: L.IdxMap(None, 0, 1, 1 << 20, 0, 0, s)
+    hC = np.concatenate([np.full(pad, 7.0, np.float32), C0.ravel(), np.full(pad, 7.0, np.float32)])
+    hA = np.concatenate([np.zeros(pad, np.float32), A.ravel()])
+    hB = np.concatenate([np.zeros(pad, np.float32), B.ravel()])
+    pA, pB = DeviceArray(hA.shape, np.uint32), DeviceArray(hB.shape, np.uint32)
+    dA, dB = DeviceArray.from_host(hA), DeviceArray.from_host(hB)
+    L.call("dcc_pack_bf16x3", dA.vp, pA.vp, hA.size, None)
+    L.call("dcc_pack_bf16x3", dB.vp, pB.vp, hB.size, None)
+    dC = DeviceArray.from_host(hC)
+    bt = np.zeros(1, dtype=L.BATCH_DTYPE)
+    bt[0] = (pad, pad, pad, 0, M)
+    dbt = DeviceArray.from_host(bt)
+    L.call("dcc_contract_packed_bf16x3", M, N, K, 1.5, 0.5, pA.vp, lin(K), lin(1), pB.vp, lin(N), lin(1),
+           dC.vp, lin(ld), lin(1), dbt.vp, 1, M, tile, None)
+    out = dC.download()
+    got = out[pad:pad + M * ld].reshape(M, ld)
+    ah, al = bf16_split(A)
+    bh, bl = bf16_split(B)
+    split_ref = 0.5 * C0[:, :N].astype(np.float64) + 1.5 * (ah @ bh + ah @ bl + al @ bh)
+    scale = max(1.0, abs(split_ref).max())
+    assert abs(got[:, :N] - split_ref).max() < 3e-6 * scale * np.sqrt(K)
+    assert np.array_equal(got[:, N:], C0[:, N:])
+    assert np.all(out[:pad] == 7.0) and np.all(out[pad + M * ld:] == 7.0)
+
+
 @pytest.mark.parametrize("o,v", [(3, 9), (5, 17)])
 def test_residual_bf16x3_vs_oracle(dev, o, v):
     from distr_cc.synthetic import SyntheticSystem
