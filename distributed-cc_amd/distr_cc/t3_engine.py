@@ -512,7 +512,7 @@ class TriplesEngine:
         # W_oooo / F_oo (rccsdt.py:726-731).  The abc permutation of each of the six terms commutes
         # with the sum over source blocks p, so every term is the same natural GEMM
         # X_s[q] = sum_p pfs[s][p][q] T[p] (coalesced in and out) followed, for s > 0, by R[q] +=
-        # X_s[q] transposed by the term's permutation (an S3 orbit-tile pass).
+        # X_s[q] transposed by the term's permutation (an S3 gather pass).
         # s: 0 'pq,pabc->qabc', 1 'pq,pacb->qabc', 2 'pq,pbac->qabc', 3 'pq,pabc->qcab',
         #    4 'pq,pabc->qbca', 5 'pq,pabc->qcba'; R[q] += X_s[q].transpose(perm[s]).
         A = Operand(self.pfs, 'Spq', (6, n3, n3))
