@@ -270,38 +270,80 @@ class S3GatherLaunch:
                stream.handle if stream is not None else None)
 
 
-def plan_t2t3(o, v, R, t2, W_ovvv, W_oovo, scratch_bytes):
+class T2T3Operands:
+    """The operands of the fused T2 -> T3 base-term GEMM of one chunk of first occupied indices
+    i0 .. i0+n-1 (rccsdt.py:388-399), with K = (d, then l) of extent v + o:
+        Ap[i,b,j,a,K] = W_vvvo[a,b,K,j]        Bp[i,b,K,k,c] = t2[i0+i,k,K,c]      (K < v)
+        Ap[i,b,j,a,K] = -W_vooo[a,K-v,i0+i,j]   Bp[i,b,K,k,c] = t2[K-v,k,b,c]      (K >= v)
+    built by four strided copies (W_ovvv holds W_vvvo[a,b,d,j] as [j,a,b,d], W_oovo holds
+    W_vooo[a,l,i,j] as [i,j,a,l], t2 is [i,k,d,c]) and, for split-BF16x3 products, packed in place."""
+
+    def __init__(self, name, Ap, Bp, W_ovvv, W_oovo, t2, o, v, i0, n, pack):
+        self.name, self.flops = name, 0.0
+        self.Ap, self.Bp, self.W_ovvv, self.W_oovo, self.t2 = Ap, Bp, W_ovvv, W_oovo, t2
+        self.o, self.v, self.i0, self.n, self.pack = o, v, i0, n, pack
+
+    def run(self, stream):
+        o, v, i0, n = self.o, self.v, self.i0, self.n
+        K2 = v + o
+        sa = {'i': v * o * v * K2, 'b': o * v * K2, 'j': v * K2, 'a': K2}
+        sb = {'i': v * K2 * o * v, 'b': K2 * o * v, 'k': v, 'c': 1}
+        tensor_axpby(Operand.strided(self.Ap, dict(sa, d=1)),
+                     Operand.strided(self.W_ovvv, {'j': v ** 3, 'a': v * v, 'b': v, 'd': 1}),
+                     'ibjad', (n, v, o, v, v), 1.0, 0.0, stream)
+        tensor_axpby(Operand.strided(self.Ap, dict(sa, l=1), base=v),
+                     Operand.strided(self.W_oovo, {'i': o * v * o, 'j': v * o, 'a': o, 'l': 1}, base=i0 * o * v * o),
+                     'ibjal', (n, v, o, v, o), -1.0, 0.0, stream)
+        tensor_axpby(Operand.strided(self.Bp, dict(sb, d=o * v)),
+                     Operand.strided(self.t2, {'i': o * v * v, 'k': v * v, 'd': v, 'c': 1}, base=i0 * o * v * v),
+                     'ibdkc', (n, v, v, o, v), 1.0, 0.0, stream)
+        tensor_axpby(Operand.strided(self.Bp, dict(sb, l=o * v), base=v * o * v),
+                     Operand.strided(self.t2, {'l': o * v * v, 'k': v * v, 'b': v, 'c': 1}),
+                     'iblkc', (n, v, o, o, v), 1.0, 0.0, stream)
+        if self.pack:
+            s = stream.handle if stream is not None else None
+            for buf in (self.Ap, self.Bp):
+                L.call("dcc_pack_bf16x3", buf.vp, buf.vp, n * v * o * v * K2, s)
+
+
+def plan_t2t3(o, v, R, t2, W_ovvv, W_oovo, scratch_bytes, packed=False):
     """T2 -> T3 terms (rccsdt.py:388-399) added into the canonical blocks R; returns (Xt, launches).
 
     The twelve terms are the six simultaneous (ijk, abc) permutations of one base term
         X[i,j,k][a,b,c] = sum_d W_vvvo[a,b,d,j] t2[i,k,d,c] - sum_l W_vooo[a,l,i,j] t2[l,k,b,c],
-    r3[ijk][abc] += sum_pi X[pi(ijk)][pi(abc)].  X is built for every ordered (i,j,k) by two GEMMs
-    (M = (j,a,b), N = (i,k,c), K = d and M = (i,j,a), N = (k,b,c), K = l; no batch tables), and
-    one multi-source S3 gather pass adds, for every canonical target, its (up to six) permuted
-    source blocks in one read-modify-write.  Ordered triples are processed in chunks of the first
-    index whose X fits ``scratch_bytes``; Xt is that chunk buffer (ic, o, o, v, v, v).
-    W_ovvv holds W_vvvo[a,b,d,j] as [j,a,b,d] and W_oovo holds W_vooo[a,l,i,j] as [i,j,a,l]."""
+    r3[ijk][abc] += sum_pi X[pi(ijk)][pi(abc)].  With i and b fixed, both sums are one product over
+    K = (d, l): X[i,.,.][.,b,.] = Ap[i,b] Bp[i,b] with M = (j,a), N = (k,c) (T2T3Operands builds Ap and
+    Bp), so X is built for every ordered (i,j,k) by one batched GEMM over the (i,b) pairs that writes X
+    once (two GEMMs -- K = d, then K = l adding into X -- wrote it and then read and rewrote it), and
+    one multi-source S3 gather pass adds, for every canonical target, its (up to six) permuted source
+    blocks in one read-modify-write.  Ordered triples are processed in chunks of the first index whose
+    X fits ``scratch_bytes``; Xt is that chunk buffer (ic, o, o, v, v, v).  ``packed``: the GEMM runs
+    as split-BF16x3 products on operands packed by T2T3Operands (when K = v + o > 16; a shorter K stays
+    on FP32 like every such GEMM of the split-BF16x3 engine)."""
     v3 = v ** 3
     dt = R.dtype
     ic = int(max(1, min(o, scratch_bytes // max(1, o * o * v3 * dt.itemsize))))
     Xt = DeviceArray((ic, o, o, v, v, v), dt)
-    ext = {x: v for x in 'abcd'}
-    ext.update({x: o for x in 'ijkl'})
-    one = [({}, {}, {})]
+    K2 = v + o
+    packed = packed and K2 > 16
+    Ap = DeviceArray((ic, v, o, v, K2), dt)
+    Bp = DeviceArray((ic, v, K2, o, v), dt)
+    ext = {'j': o, 'a': v, 'K': K2, 'k': o, 'c': v}
     # source-position permutations of (i,j,k); the abc permutation of the S3 job is its inverse
     pis = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
     triples = canonical_triples(o)
     launches = []
     for i0 in range(0, o, ic):
         n = min(ic, o - i0)
-        ext['i'] = n
-        X = Operand(Xt, 'ijkabc', (n, o, o, v, v, v))
-        A1 = Operand(W_ovvv, 'jabd', (o, v, v, v))
-        B1 = Operand(t2, 'ikdc', (n, o, v, v), base=i0 * o * v * v)
-        launches.append(plan_einsum('T2T3: X W_vvvo', 'jabd,ikdc->ijkabc', A1, B1, X, 1.0, 0.0, ext, one))
-        A2 = Operand(W_oovo, 'ijal', (n, o, v, o), base=i0 * o * v * o)
-        B2 = Operand(t2, 'lkbc', (o, o, v, v))
-        launches.append(plan_einsum('T2T3: X W_vooo', 'ijal,lkbc->ijkabc', A2, B2, X, -1.0, 1.0, ext, one))
+        launches.append(T2T3Operands('T2T3: operands', Ap, Bp, W_ovvv, W_oovo, t2, o, v, i0, n, packed))
+        A = Operand(Ap, 'ibjaK', (ic, v, o, v, K2))
+        B = Operand(Bp, 'ibKkc', (ic, v, K2, o, v))
+        X = Operand(Xt, 'ijkabc', (ic, o, o, v, v, v))
+        fixed = [{'i': i, 'b': b} for i in range(n) for b in range(v)]
+        g = plan_einsum('T2T3: X', 'jaK,Kkc->jkac', A, B, X, 1.0, 0.0, ext, [(f, f, f) for f in fixed])
+        if packed:
+            g.fn = "dcc_contract_packed_bf16x3"
+        launches.append(g)
         # one gather pass: target q takes sum_pi X[pi(ijk)]^{pi^-1} over the sources in this chunk
         jobs = []
         for q, t in enumerate(triples):
@@ -487,7 +529,7 @@ class TriplesEngine:
         padding) stay on FP32 MFMA, which is also the more accurate of the two."""
         if self.contract == "bf16x3":
             for g in launches:
-                if isinstance(g, GemmLaunch) and g.args[2] > 16:
+                if isinstance(g, GemmLaunch) and g.args[2] > 16 and g.fn != "dcc_contract_packed_bf16x3":
                     g.fn = "dcc_contract_f32_bf16x3"
 
     # ------------------------------------------------------------------ plan
@@ -544,7 +586,7 @@ class TriplesEngine:
     def _plan_t2t3(self):
         """T2 -> T3 terms (plan_t2t3), with the base-term scratch bounded by ``ring_g_bytes``."""
         self.Xt, launches = plan_t2t3(self.nocc, self.nvir, self.R, self.t2, self.W_ovvv, self.W_oovo,
-                                      self.ring_g_bytes)
+                                      self.ring_g_bytes, packed=self.contract == "bf16x3")
         return launches
 
     def _plan_ring(self):

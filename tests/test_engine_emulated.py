@@ -259,17 +259,18 @@ def test_mixed_iteration_plan(monkeypatch):
     assert abs(a3 - b3).max() <= 1e-5 * abs(b3).max()
 
 
-@pytest.mark.parametrize("o,v", [(3, 5), (3, 7)])
+@pytest.mark.parametrize("o,v", [(3, 5), (3, 7), (3, 14)])
 def test_bf16x3_residual_plan(monkeypatch, o, v):
     """Split-BF16x3 residual plan; at nvir=5 an FP32 GEMM (K = o v <= 16) reads Y/Z, which then get a
-    packed copy, at nvir=7 Y/Z are written as packed words by the transform itself."""
+    packed copy, at nvir=7 and 14 Y/Z are written as packed words by the transform itself; at nvir=14
+    the fused T2 -> T3 GEMM (K = v + o = 17) runs on operands its operand pass packs."""
     abi_emulator.install(monkeypatch)
     from distr_cc.synthetic import SyntheticSystem
     from distr_cc.t3_engine import TriplesEngine
     s = SyntheticSystem(o, v, naux=20, seed=31)
     im, t2, t3 = s.bare_intermediates(), s.t2(), s.t3_tri()
     eng = TriplesEngine(o, v, dtype=np.float32, contract="bf16x3")
-    assert eng._yz_direct == (v == 7)
+    assert eng._yz_direct == (o * v > 16)
     # every residual GEMM with K > 16 reads packed (pre-split) operands, the others stay FP32
     gemms = [g for g in eng.launches if hasattr(g, "args")]
     assert all(g.fn == ("dcc_contract_packed_bf16x3" if g.args[2] > 16 else "dcc_contract_f32") for g in gemms)
