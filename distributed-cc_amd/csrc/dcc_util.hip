@@ -2,6 +2,7 @@
 // strided tensor axpby (block views, transposes, packing), deterministic dot
 // products and the singles/doubles denominators.
 #include "dcc_common.h"
+#include <type_traits>
 
 namespace {
 
@@ -109,6 +110,58 @@ inline unsigned grid_for(int64_t total) {
     return (unsigned)blocks;
 }
 
+// Operands of the fused T2 -> T3 base-term GEMM for the first occupied indices i0 .. i0+n-1, K = (d, then l):
+//   Ap[i,b,j,a,K] = W_ovvv[j,a,b,K] (K < v), -W_oovo[i0+i,j,a,K-v] (K >= v)
+//   Bp[i,b,K,k,c] = t2[i0+i,k,K,c]   (K < v),  t2[K-v,k,b,c]      (K >= v)
+// One element per thread, consecutive threads along the contiguous index of both outputs (K for Ap,
+// (k,c) for Bp), so every write and most reads coalesce; O = uint32_t writes split-BF16x3 words.
+template <typename T, typename O>
+__global__ __launch_bounds__(256) void t2t3_operands_kernel(O *__restrict__ Ap, O *__restrict__ Bp,
+                                                            const T *__restrict__ W_ovvv,
+                                                            const T *__restrict__ W_oovo,
+                                                            const T *__restrict__ t2, int o, int v, int i0,
+                                                            int64_t na, int64_t total) {
+    const int K2 = v + o, ov = o * v;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        T x;
+        O *dst;
+        if (t < na) {                      // Ap: row (i,b,j,a), column K
+            const int row = (int)(t / K2), K = (int)(t - (int64_t)row * K2);
+            const int a = row % v, r1 = row / v, j = r1 % o, r2 = r1 / o, b = r2 % v, i = r2 / v;
+            x = K < v ? W_ovvv[(((int64_t)j * v + a) * v + b) * v + K]
+                      : -W_oovo[(((int64_t)(i0 + i) * o + j) * v + a) * o + (K - v)];
+            dst = Ap + t;
+        } else {                           // Bp: row (i,b,K), column (k,c)
+            const int64_t u = t - na;
+            const int row = (int)(u / ov), col = (int)(u - (int64_t)row * ov);
+            const int c = col % v, k = col / v, K = row % K2, r1 = row / K2, b = r1 % v, i = r1 / v;
+            x = K < v ? t2[(((int64_t)(i0 + i) * o + k) * v + K) * v + c]
+                      : t2[(((int64_t)(K - v) * o + k) * v + b) * v + c];
+            dst = Bp + u;
+        }
+        if constexpr (std::is_same<O, uint32_t>::value) *dst = dcc::pack_bf16x3((float)x);
+        else *dst = (O)x;
+    }
+}
+
+template <typename T, typename O>
+int t2t3_operands(O *Ap, O *Bp, const T *W_ovvv, const T *W_oovo, const T *t2, int64_t nocc, int64_t nvir,
+                  int64_t i0, int64_t n, dcc_stream_t stream) {
+    DCC_CHECK(nocc >= 0 && nvir >= 0 && i0 >= 0 && n >= 0 && i0 + n <= nocc, "bad occupied range");
+    if (n == 0 || nvir == 0 || nocc == 0) return 0;
+    DCC_CHECK(Ap != nullptr && Bp != nullptr && W_ovvv != nullptr && W_oovo != nullptr && t2 != nullptr,
+              "null pointer");
+    const int64_t rows_a = n * nvir * nocc * nvir, rows_b = n * nvir * (nvir + nocc);
+    DCC_CHECK(rows_a < ((int64_t)1 << 31) && rows_b < ((int64_t)1 << 31) && nocc * nvir < ((int64_t)1 << 31),
+              "extent exceeds 2^31");
+    const int64_t na = rows_a * (nvir + nocc), total = na + rows_b * nocc * nvir;
+    hipLaunchKernelGGL((t2t3_operands_kernel<T, O>), dim3(grid_for(total)), dim3(256), 0, dcc::as_stream(stream),
+                       Ap, Bp, W_ovvv, W_oovo, t2, (int)nocc, (int)nvir, (int)i0, na, total);
+    DCC_LAUNCH_CHECK();
+    return 0;
+}
+
 template <typename TD, typename TS>
 int tensor_axpby(TD *dst, const TS *src, const dcc_strided *layout, double alpha, double beta, dcc_stream_t stream) {
     DCC_CHECK(layout != nullptr, "null layout");
@@ -181,6 +234,22 @@ int dcc_pack_bf16x3(const float *src, uint32_t *dst, int64_t n, dcc_stream_t str
     hipLaunchKernelGGL(pack_bf16x3_kernel, dim3(grid_for(n)), dim3(256), 0, dcc::as_stream(stream), src, dst, n);
     DCC_LAUNCH_CHECK();
     return 0;
+}
+
+int dcc_t2t3_operands_f64(double *Ap, double *Bp, const double *W_ovvv, const double *W_oovo, const double *t2,
+                          int64_t nocc, int64_t nvir, int64_t i0, int64_t n, dcc_stream_t stream) {
+    return t2t3_operands<double, double>(Ap, Bp, W_ovvv, W_oovo, t2, nocc, nvir, i0, n, stream);
+}
+
+int dcc_t2t3_operands_f32(float *Ap, float *Bp, const float *W_ovvv, const float *W_oovo, const float *t2,
+                          int64_t nocc, int64_t nvir, int64_t i0, int64_t n, dcc_stream_t stream) {
+    return t2t3_operands<float, float>(Ap, Bp, W_ovvv, W_oovo, t2, nocc, nvir, i0, n, stream);
+}
+
+int dcc_t2t3_operands_packed_f32(uint32_t *Ap, uint32_t *Bp, const float *W_ovvv, const float *W_oovo,
+                                 const float *t2, int64_t nocc, int64_t nvir, int64_t i0, int64_t n,
+                                 dcc_stream_t stream) {
+    return t2t3_operands<float, uint32_t>(Ap, Bp, W_ovvv, W_oovo, t2, nocc, nvir, i0, n, stream);
 }
 
 int dcc_sum_slices_f64(double *dst, const double *src, int64_t nslices, int64_t n, double alpha, double beta,

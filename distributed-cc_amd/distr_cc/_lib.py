@@ -91,6 +91,9 @@ _SIGS = {
     "dcc_contract_f32_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap, IdxMap,
                                  c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_pack_bf16x3": ([c_vp, c_vp, c_i64, c_vp], ctypes.c_int),
+    "dcc_t2t3_operands_f64": ([c_vp] * 5 + [c_i64] * 4 + [c_vp], ctypes.c_int),
+    "dcc_t2t3_operands_f32": ([c_vp] * 5 + [c_i64] * 4 + [c_vp], ctypes.c_int),
+    "dcc_t2t3_operands_packed_f32": ([c_vp] * 5 + [c_i64] * 4 + [c_vp], ctypes.c_int),
     "dcc_contract_packed_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap,
                                     IdxMap, c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_contract_tiled_f32_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap,

@@ -275,8 +275,8 @@ class T2T3Operands:
     i0 .. i0+n-1 (rccsdt.py:388-399), with K = (d, then l) of extent v + o:
         Ap[i,b,j,a,K] = W_vvvo[a,b,K,j]        Bp[i,b,K,k,c] = t2[i0+i,k,K,c]      (K < v)
         Ap[i,b,j,a,K] = -W_vooo[a,K-v,i0+i,j]   Bp[i,b,K,k,c] = t2[K-v,k,b,c]      (K >= v)
-    built by four strided copies (W_ovvv holds W_vvvo[a,b,d,j] as [j,a,b,d], W_oovo holds
-    W_vooo[a,l,i,j] as [i,j,a,l], t2 is [i,k,d,c]) and, for split-BF16x3 products, packed in place."""
+    written by one dcc_t2t3_operands call (W_ovvv holds W_vvvo[a,b,d,j] as [j,a,b,d], W_oovo holds
+    W_vooo[a,l,i,j] as [i,j,a,l], t2 is [i,k,d,c]), as split-BF16x3 words for split-BF16x3 products."""
 
     def __init__(self, name, Ap, Bp, W_ovvv, W_oovo, t2, o, v, i0, n, pack):
         self.name, self.flops = name, 0.0
@@ -284,26 +284,9 @@ class T2T3Operands:
         self.o, self.v, self.i0, self.n, self.pack = o, v, i0, n, pack
 
     def run(self, stream):
-        o, v, i0, n = self.o, self.v, self.i0, self.n
-        K2 = v + o
-        sa = {'i': v * o * v * K2, 'b': o * v * K2, 'j': v * K2, 'a': K2}
-        sb = {'i': v * K2 * o * v, 'b': K2 * o * v, 'k': v, 'c': 1}
-        tensor_axpby(Operand.strided(self.Ap, dict(sa, d=1)),
-                     Operand.strided(self.W_ovvv, {'j': v ** 3, 'a': v * v, 'b': v, 'd': 1}),
-                     'ibjad', (n, v, o, v, v), 1.0, 0.0, stream)
-        tensor_axpby(Operand.strided(self.Ap, dict(sa, l=1), base=v),
-                     Operand.strided(self.W_oovo, {'i': o * v * o, 'j': v * o, 'a': o, 'l': 1}, base=i0 * o * v * o),
-                     'ibjal', (n, v, o, v, o), -1.0, 0.0, stream)
-        tensor_axpby(Operand.strided(self.Bp, dict(sb, d=o * v)),
-                     Operand.strided(self.t2, {'i': o * v * v, 'k': v * v, 'd': v, 'c': 1}, base=i0 * o * v * v),
-                     'ibdkc', (n, v, v, o, v), 1.0, 0.0, stream)
-        tensor_axpby(Operand.strided(self.Bp, dict(sb, l=o * v), base=v * o * v),
-                     Operand.strided(self.t2, {'l': o * v * v, 'k': v * v, 'b': v, 'c': 1}),
-                     'iblkc', (n, v, o, o, v), 1.0, 0.0, stream)
-        if self.pack:
-            s = stream.handle if stream is not None else None
-            for buf in (self.Ap, self.Bp):
-                L.call("dcc_pack_bf16x3", buf.vp, buf.vp, n * v * o * v * K2, s)
+        fn = "dcc_t2t3_operands_" + ("packed_f32" if self.pack else _suffix(self.Ap.dtype))
+        L.call(fn, self.Ap.vp, self.Bp.vp, self.W_ovvv.vp, self.W_oovo.vp, self.t2.vp, self.o, self.v, self.i0,
+               self.n, stream.handle if stream is not None else None)
 
 
 def plan_t2t3(o, v, R, t2, W_ovvv, W_oovo, scratch_bytes, packed=False):

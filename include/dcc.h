@@ -351,6 +351,27 @@ int dcc_tensor_axpby_f64_f32(double *dst, const float *src, const dcc_strided *l
                              dcc_stream_t stream);
 
 /*
+ * Operands of the T2 -> T3 base term (distr_cc/rccsdt.py:388-399) as one product over K = (d, l) of
+ * extent nvir + nocc, for the first occupied indices i0 .. i0+n-1 (i < n below):
+ *   Ap[i,b,j,a,K] = W_ovvv[j,a,b,K] (K < nvir), -W_oovo[i0+i,j,a,K-nvir] (K >= nvir)
+ *   Bp[i,b,K,k,c] = t2[i0+i,k,K,c]   (K < nvir), t2[K-nvir,k,b,c]        (K >= nvir)
+ * with W_ovvv = W_vvvo[a,b,d,j] stored as [j,a,b,d] (nocc, nvir, nvir, nvir), W_oovo = W_vooo[a,l,i,j]
+ * stored as [i,j,a,l] (nocc, nocc, nvir, nocc) and t2 (nocc, nocc, nvir, nvir); Ap holds
+ * n*nvir*nocc*nvir*(nvir+nocc) and Bp n*nvir*(nvir+nocc)*nocc*nvir elements.  Then
+ * X[i0+i,j,k,a,b,c] = sum_K Ap[i,b,j,a,K] Bp[i,b,K,k,c] is the base term of rccsdt.py:388-399 for every
+ * ordered (i,j,k), one batched GEMM over (i,b).  The _packed_f32 form writes split-BF16x3 words
+ * (dcc_pack_bf16x3 of each float).  No reference counterpart (the reference evaluates the two einsums
+ * of the base term separately).
+ */
+int dcc_t2t3_operands_f64(double *Ap, double *Bp, const double *W_ovvv, const double *W_oovo, const double *t2,
+                          int64_t nocc, int64_t nvir, int64_t i0, int64_t n, dcc_stream_t stream);
+int dcc_t2t3_operands_f32(float *Ap, float *Bp, const float *W_ovvv, const float *W_oovo, const float *t2,
+                          int64_t nocc, int64_t nvir, int64_t i0, int64_t n, dcc_stream_t stream);
+int dcc_t2t3_operands_packed_f32(uint32_t *Ap, uint32_t *Bp, const float *W_ovvv, const float *W_oovo,
+                                 const float *t2, int64_t nocc, int64_t nvir, int64_t i0, int64_t n,
+                                 dcc_stream_t stream);
+
+/*
  * dst[i] = beta * dst[i] + alpha * sum_{s < nslices} src[s * n + i] (FP64 sum in slice order): the
  * reduction of split-K partial products of the T3 -> r1/r2/W contractions (distr_cc/rccsdt.py:253-366).
  */

@@ -216,6 +216,36 @@ class Emulator:
         self.dev.view(_ptr(dst), np.uint32, n)[:] = w
         return 0
 
+    def _t2t3_operands(self, dt, packed, Ap, Bp, W_ovvv, W_oovo, t2, nocc, nvir, i0, n, stream):
+        o, v, i0, n = int(nocc), int(nvir), int(i0), int(n)
+        dev = self.dev
+        Wv = dev.view(_ptr(W_ovvv), dt, o * v ** 3).reshape(o, v, v, v)          # [j,a,b,d]
+        Wo = dev.view(_ptr(W_oovo), dt, o * o * v * o).reshape(o, o, v, o)      # [i,j,a,l]
+        T2 = dev.view(_ptr(t2), dt, o * o * v * v).reshape(o, o, v, v)          # [i,k,d,c]
+        A = np.empty((n, v, o, v, v + o), dt)
+        A[..., :v] = Wv.transpose(2, 0, 1, 3)[None]                              # [b,j,a,d]
+        A[..., v:] = -Wo[i0:i0 + n][:, None]                                     # [i,j,a,l] -> [i,b,j,a,l]
+        B = np.empty((n, v, v + o, o, v), dt)
+        B[:, :, :v] = T2[i0:i0 + n].transpose(0, 2, 1, 3)[:, None]               # [i,d,k,c]
+        B[:, :, v:] = T2.transpose(2, 0, 1, 3)[None]                             # [b,l,k,c]
+        for ptr, x in ((Ap, A), (Bp, B)):
+            if packed:
+                hi, lo = bf16_split(x.astype(np.float32).ravel())
+                dev.view(_ptr(ptr), np.uint32, x.size)[:] = ((hi.astype(np.float32).view(np.uint32) >> 16) |
+                                                             (lo.astype(np.float32).view(np.uint32) & 0xFFFF0000))
+            else:
+                dev.view(_ptr(ptr), dt, x.size)[:] = x.ravel()
+        return 0
+
+    def dcc_t2t3_operands_f64(self, *a):
+        return self._t2t3_operands(np.float64, False, *a)
+
+    def dcc_t2t3_operands_f32(self, *a):
+        return self._t2t3_operands(np.float32, False, *a)
+
+    def dcc_t2t3_operands_packed_f32(self, *a):
+        return self._t2t3_operands(np.float32, True, *a)
+
     def _contract(self, dt, M, N, K, alpha, beta, A, am, ak, B, bk, bn, C, cm, cn, batches, nbatch, max_rows,
                   stream):
         split, packed = dt in ("bf16x3", "packed"), dt == "packed"

@@ -505,6 +505,39 @@ def test_contract_bf16x3_vector_epilogue_tail(dev, tile):
     assert np.all(out[:pad] == 7.0) and np.all(out[pad + M * ld:] == 7.0)
 
 
+@pytest.mark.parametrize("dt", ["f64", "f32", "packed_f32"])
+@pytest.mark.parametrize("o,v,i0,n", [(3, 7, 1, 2), (4, 9, 0, 4), (2, 20, 1, 1)])
+def test_t2t3_operands(dev, dt, o, v, i0, n):
+    """dcc_t2t3_operands_*: the K = (d, l) operands of the fused T2 -> T3 base term, element for element
+    against NumPy (the packed form against dcc_pack_bf16x3 of the FP32 operands)."""
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    _say("t2t3 operands %s o=%d v=%d i0=%d n=%d" % (dt, o, v, i0, n))
+    rng = np.random.default_rng(7 * o + v)
+    typ = np.float64 if dt == "f64" else np.float32
+    Wv = rng.normal(size=(o, v, v, v)).astype(typ)     # [j,a,b,d]
+    Wo = rng.normal(size=(o, o, v, o)).astype(typ)     # [i,j,a,l]
+    t2 = rng.normal(size=(o, o, v, v)).astype(typ)     # [i,k,d,c]
+    A = np.empty((n, v, o, v, v + o), typ)
+    A[..., :v] = Wv.transpose(2, 0, 1, 3)[None]
+    A[..., v:] = -Wo[i0:i0 + n][:, None]
+    B = np.empty((n, v, v + o, o, v), typ)
+    B[:, :, :v] = t2[i0:i0 + n].transpose(0, 2, 1, 3)[:, None]
+    B[:, :, v:] = t2.transpose(2, 0, 1, 3)[None]
+    dWv, dWo, dt2 = DeviceArray.from_host(Wv), DeviceArray.from_host(Wo), DeviceArray.from_host(t2)
+    out_t = np.uint32 if dt == "packed_f32" else typ
+    dA, dB = DeviceArray(A.shape, out_t), DeviceArray(B.shape, out_t)
+    L.call("dcc_t2t3_operands_" + dt, dA.vp, dB.vp, dWv.vp, dWo.vp, dt2.vp, o, v, i0, n, None)
+    for got, ref in ((dA, A), (dB, B)):
+        if dt == "packed_f32":
+            r = DeviceArray.from_host(ref)
+            w = DeviceArray(ref.shape, np.uint32)
+            L.call("dcc_pack_bf16x3", r.vp, w.vp, ref.size, None)
+            assert np.array_equal(got.download(), w.download())
+        else:
+            assert np.array_equal(got.download(), ref)
+
+
 @pytest.mark.parametrize("o,v", [(3, 9), (5, 17)])
 def test_residual_bf16x3_vs_oracle(dev, o, v):
     from distr_cc.synthetic import SyntheticSystem
