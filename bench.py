@@ -478,7 +478,7 @@ def batched_ensemble(nsys, steps=20, seed=20261202, sigma_b=SIGMA_B_DEFAULT, sol
     return out
 
 
-def probe_residual(o, v, precision, tiles, seed=20261202):
+def probe_residual(o, v, precision, tiles, seed=20261202, vvvv_pairs=False):
     """One triples engine of the given precision on the synthetic system (nocc=o, nvir=v): tiles loaded
     from ``tiles`` if it exists (a tile list of this size or a {launch name: tile} map), else autotuned
     (and saved there by launch name when a path is given); the residual is run twice and the second one
@@ -491,7 +491,7 @@ def probe_residual(o, v, precision, tiles, seed=20261202):
                     "bf16x3": (np.float32, "bf16x3")}[precision]
     sysm = SyntheticSystem(o, v, naux=300, seed=seed)
     st = Stream()
-    eng = TriplesEngine(o, v, stream=st, dtype=dt, contract=contract)
+    eng = TriplesEngine(o, v, stream=st, dtype=dt, contract=contract, vvvv_pairs=vvvv_pairs)
     eng.set_t3(sysm.t3_tri().astype(dt))
     eng.set_intermediates(sysm.bare_intermediates(), sysm.t2())
     if tiles and os.path.exists(tiles):
@@ -598,7 +598,7 @@ def plain_fp64_residual(o, v, steps, warmup, autotune=True, tiles=None, breakdow
     for dst, src in zip(saved, (eng.W_oovo, eng.W_ovvv, eng.t3)):
         L.call("dcc_memcpy_dtod", dst.vp, src.vp, src.nbytes, stream.handle)
     stream.synchronize()
-    dom = [g for g in eng.launches if g.name.startswith("W_vvvv")]
+    dom = [g for g in eng.launches if g.name.startswith("W_vvvv") and g.flops > 0]
     ev = {}
 
     def timer(g, what):
@@ -774,6 +774,8 @@ def main():
     ap.add_argument("--probe-residual", choices=("fp64", "fp32", "bf16x3"), default=None,
                     help="only time one engine's residual at --nocc/--nvir (tiles via --tiles) and exit")
     ap.add_argument("--tiles", metavar="JSON", default=None, help="tile file of --probe-residual")
+    ap.add_argument("--vvvv-pairs", type=int, default=1,
+                    help="pair-symmetric W_vvvv products in the triples engines (0 = plain GEMMs)")
     ap.add_argument("--probe-batched", type=int, default=0, metavar="NSYS",
                     help="only the batched ensemble of NSYS systems (--steps steps issued eagerly, no "
                          "per-system solves); prints its JSON (for profiling)")
@@ -796,7 +798,7 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.probe_residual:
-        probe_residual(o, v, args.probe_residual, args.tiles)
+        probe_residual(o, v, args.probe_residual, args.tiles, vvvv_pairs=bool(args.vvvv_pairs))
         return
     if args.tune_tiles:
         tune_tiles(o, v, args.tune_tiles, args.sigma_b)
@@ -840,7 +842,10 @@ def main():
                 mixed["speedup_" + name] = mixed["fp64_ms_per_iteration"] / mixed[name + "_ms_per_iteration"]
         mixed["fp64_converged_iteration_dE_1e-6"] = _converged_at(e64, 1e-6)
     # dominant kernel of the headline: the three split-BF16x3 W_vvvv x T3 launches per iteration
-    lw = head["launches"]
+    # (the GEMMs only: with the pair-symmetric products the split / unfold passes carry no flops and are
+    # reported beside them)
+    lw = [x for x in head["launches"] if x[1] > 0]
+    pair_pass_ms = sum(x[2] for x in head["launches"] if x[1] == 0) / max(1, len(head["ms"]))
     avg_ms = float(np.mean([x[2] for x in lw]))
     flops = float(np.mean([x[1] for x in lw]))
     achieved = 3.0 * flops / (avg_ms * 1e-3) / 1e12
@@ -872,14 +877,22 @@ def main():
                    "mixed_vs_fp64": mixed},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_BF16_MFMA_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
-                     "kernel": "contract_bf16x3_dma_kernel / contract_bf16x3_kernel (the three W_vvvv x T3 "
+                     "kernel": "contract_bf16x3_dma_kernel / contract_bf16x3_kernel: the W_vvvv x T3 GEMMs, "
+                               "as symmetric + antisymmetric products over triangular index pairs (one launch "
+                               "with two batch entries per term and chunk of blocks; the two terms whose "
+                               "operands are both K-contiguous staged by direct global->LDS requests, the "
+                               "first register-staged)" if pair_pass_ms > 0 else
+                               "contract_bf16x3_dma_kernel / contract_bf16x3_kernel (the three W_vvvv x T3 "
                                "launches: two with both operands staged by direct global->LDS requests, one "
                                "register-staged)",
                      "flops_per_launch": 3.0 * flops, "fp32_equivalent_flops_per_launch": flops,
-                     "avg_launch_ms": avg_ms,
+                     "avg_launch_ms": avg_ms, "launches_per_iteration": len(lw) / max(1, len(head["ms"])),
+                     "pair_split_unfold_ms_per_iteration": pair_pass_ms,
                      "note": "achieved counts the three BF16 products (hi*hi + hi*lo + lo*hi) of every FP32 "
-                             "multiply-add: 3 x 2 v^5 per canonical block and launch; the FP32-equivalent rate "
-                             "is achieved/3 = %.0f TFLOP/s (%.2f x the FP32 MFMA peak)"
+                             "multiply-add the launch executes (pair form: 2 (nS + nA) kpad v per block, "
+                             "nS = v(v+1)/2, nA = v(v-1)/2, kpad = nS rounded up to 64: about half the "
+                             "2 v^5 of the plain product); the FP32-equivalent rate is achieved/3 = %.0f "
+                             "TFLOP/s (%.2f x the FP32 MFMA peak)"
                              % (achieved / 3, achieved / 3 / PEAK_FP32_MFMA_TFLOPS)},
     }
     if not args.no_cpu_baseline:

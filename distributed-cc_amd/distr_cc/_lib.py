@@ -94,6 +94,14 @@ _SIGS = {
     "dcc_t2t3_operands_f64": ([c_vp] * 5 + [c_i64] * 4 + [c_vp], ctypes.c_int),
     "dcc_t2t3_operands_f32": ([c_vp] * 5 + [c_i64] * 4 + [c_vp], ctypes.c_int),
     "dcc_t2t3_operands_packed_f32": ([c_vp] * 5 + [c_i64] * 4 + [c_vp], ctypes.c_int),
+    "dcc_vvvv_pair_weights_f64": ([c_vp, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
+    "dcc_vvvv_pair_weights_f32": ([c_vp, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
+    "dcc_vvvv_pair_weights_packed_f32": ([c_vp, c_i64, c_i64, c_vp, c_vp], ctypes.c_int),
+    "dcc_vvvv_pair_split_f64": ([c_vp, c_i64, c_i64, ctypes.c_int32, c_i64, c_vp, c_vp], ctypes.c_int),
+    "dcc_vvvv_pair_split_f32": ([c_vp, c_i64, c_i64, ctypes.c_int32, c_i64, c_vp, c_vp], ctypes.c_int),
+    "dcc_vvvv_pair_split_packed_f32": ([c_vp, c_i64, c_i64, ctypes.c_int32, c_i64, c_vp, c_vp], ctypes.c_int),
+    "dcc_vvvv_pair_unfold_f64": ([c_vp, c_i64, c_i64, ctypes.c_int32, c_vp, c_vp], ctypes.c_int),
+    "dcc_vvvv_pair_unfold_f32": ([c_vp, c_i64, c_i64, ctypes.c_int32, c_vp, c_vp], ctypes.c_int),
     "dcc_contract_packed_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap,
                                     IdxMap, c_vp, IdxMap, IdxMap, c_vp, c_i64, c_i64, c_i64, c_vp], ctypes.c_int),
     "dcc_contract_tiled_f32_bf16x3": ([c_i64, c_i64, c_i64, c_dbl, c_dbl, c_vp, IdxMap, IdxMap, c_vp, IdxMap,

@@ -231,7 +231,7 @@ class TriplesResidualAdjoint:
         n = o + v
         self.nocc, self.nvir, self.nmo, self.stream = o, v, n, stream
         fock = np.asarray(fock, dtype=np.float64)
-        self.eng = eng = RCCSDTEngine(o, v, fock, pppp, np.diag(fock).copy(), stream=stream)
+        self.eng = eng = RCCSDTEngine(o, v, fock, pppp, np.diag(fock).copy(), stream=stream, vvvv_pairs=False)
         self.tr = tr = eng.tr
         self.n3 = tr.n3
         tr.set_t3(t3_local)

@@ -190,8 +190,9 @@ class BatchedRCCSDT(_Grouped):
     def __init__(self, systems, stream, level_shift=0.0, device=0, group_lower=True):
         from distr_cc.rccsdt_engine import RCCSDTEngine
         self.systems = [(int(o), int(v)) for o, v, _, _, _ in systems]
-        self.engines = [RCCSDTEngine(o, v, F, g, e, level_shift=level_shift, stream=stream, device=device)
-                        for o, v, F, g, e in systems]
+        # plain W_vvvv GEMMs: the phases are grouped launch by launch across the systems
+        self.engines = [RCCSDTEngine(o, v, F, g, e, level_shift=level_shift, stream=stream, device=device,
+                                     vvvv_pairs=False) for o, v, F, g, e in systems]
         for e in self.engines:
             e.tr.set_denominators(e.mo_energy, e.level_shift, e.nvir_diis)
         self._group([e.tr for e in self.engines], stream)

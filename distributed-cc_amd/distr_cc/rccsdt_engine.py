@@ -150,11 +150,14 @@ class RCCSDTEngine:
     """Device state of the RCCSDT amplitude iteration for one closed-shell system."""
 
     def __init__(self, nocc, nvir, fock, pppp, mo_energy, level_shift=0.0, nvir_diis=None, stream=None,
-                 device=0, t3_dtype=np.float64, triples_contract="native"):
+                 device=0, t3_dtype=np.float64, triples_contract="native", vvvv_pairs="auto"):
         """``t3_dtype`` float32 selects the mixed-precision iteration: T3, the triples residual and its
         operands in FP32 with FP32-MFMA contractions, everything else (integrals, T1/T2, lower-order
         residual, denominators, projection arithmetic, norms, energy, DIIS) in FP64;
-        ``triples_contract="bf16x3"`` (with float32) runs those contractions as split-BF16x3."""
+        ``triples_contract="bf16x3"`` (with float32) runs those contractions as split-BF16x3.
+        ``vvvv_pairs`` is the triples engine's option: the W_vvvv built here (the T1-dressed <ab|de>
+        plus <lm|de> t2[l,m,a,b]) is pair-symmetric, so "auto" uses the half-flop pair products
+        whenever the device has room for them."""
         o, v = int(nocc), int(nvir)
         n = o + v
         fock = np.asarray(fock, dtype=np.float64)
@@ -170,7 +173,7 @@ class RCCSDTEngine:
             raise ValueError("nvir_diis must be a positive integer")
         self.nvir_diis = min(self.nvir_diis, v)
         self.tr = tr = TriplesEngine(o, v, stream=stream, device=device, dtype=t3_dtype,
-                                     contract=triples_contract)
+                                     contract=triples_contract, vvvv_pairs=vvvv_pairs)
         self.mixed = tr.dtype != np.float64
         self.ext = {x: o for x in OCC}
         self.ext.update({x: v for x in VIR})

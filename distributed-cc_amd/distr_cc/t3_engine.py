@@ -270,6 +270,49 @@ class S3GatherLaunch:
                stream.handle if stream is not None else None)
 
 
+class PairLaunch:
+    """One dcc_vvvv_pair_split_* (kpad given) or dcc_vvvv_pair_unfold_* call over the canonical blocks
+    b0 .. b0+nb-1 of ``blocks`` (T3 for a split, R for an unfold); ``buf`` is the pair buffer."""
+
+    def __init__(self, name, fn, blocks, buf, b0, nb, nvir, axes, kpad=None):
+        self.name, self.flops, self.fn = name, 0.0, fn
+        v3 = int(nvir) ** 3
+        self.blocks = blocks   # keeps the array behind the pointer alive
+        self.blk = ctypes.c_void_p(blocks.ptr + int(b0) * v3 * blocks.dtype.itemsize)
+        self.buf, self.nb, self.nvir, self.axes, self.kpad = buf, int(nb), int(nvir), int(axes), kpad
+
+    def run(self, stream):
+        s = stream.handle if stream is not None else None
+        if self.kpad is not None:
+            L.call(self.fn, self.blk, self.nb, self.nvir, self.axes, int(self.kpad), self.buf.vp, s)
+        else:
+            L.call(self.fn, self.buf.vp, self.nb, self.nvir, self.axes, self.blk, s)
+
+
+def pair_dims(nvir):
+    """(nS, nA, kpad) of the pair-symmetric W_vvvv products: symmetric and strict pair counts and
+    their common K, padded to a multiple of 64 (the two-stage direct-to-LDS GEMM tiles step K by 64)."""
+    v = int(nvir)
+    nS, nA = v * (v + 1) // 2, v * (v - 1) // 2
+    return nS, nA, -(-nS // 64) * 64
+
+
+def check_pair_symmetric(W, samples=4096, rtol=1e-8):
+    """Raise ValueError unless W[a,b,d,e] == W[b,a,e,d] (to rtol of the sampled magnitude) on a fixed
+    random sample of index tuples: the pair-symmetric W_vvvv products assume it."""
+    W = np.asarray(W)
+    v = W.shape[0]
+    if v == 0:
+        return
+    a, b, d, e = np.random.default_rng(12345).integers(0, v, size=(4, samples))
+    x, y = W[a, b, d, e], W[b, a, e, d]
+    scale = max(float(np.abs(x).max()), np.finfo(np.float64).tiny)
+    dev = float(np.abs(x - y).max())
+    if dev > rtol * scale:
+        raise ValueError("W_vvvv is not pair-symmetric (max |W[a,b,d,e] - W[b,a,e,d]| = %.3g of %.3g); "
+                         "build the engine with vvvv_pairs=False" % (dev, scale))
+
+
 class T2T3Operands:
     """The operands of the fused T2 -> T3 base-term GEMM of one chunk of first occupied indices
     i0 .. i0+n-1 (rccsdt.py:388-399), with K = (d, then l) of extent v + o:
@@ -356,7 +399,7 @@ class TriplesEngine:
     """Holds the device state of one RCCSDT triples update for (nocc, nvir)."""
 
     def __init__(self, nocc, nvir, stream=None, device=0, dtype=np.float64, contract="native", ring_fused=None,
-                 ring_g_bytes=8 << 30):
+                 ring_g_bytes=8 << 30, vvvv_pairs=False, pair_bytes=None):
         """``dtype`` float64 is the plain FP64 path; float32 stores T3, R, Y and the
         intermediates in FP32 and contracts on FP32 MFMA, keeping the denominator,
         the projection arithmetic and the norms in FP64.  ``contract="bf16x3"`` (FP32 storage
@@ -366,7 +409,13 @@ class TriplesEngine:
         Z[(p,q)][l][d,b,c] = Y[l][b,d,c] + Y[l][c,b,d] next to Y, so the two terms that share the
         W_ovov + W_ovvo weight become one, fused along K with the W_ovvo term: three ring
         GEMMs per part instead of five, and 4/5 of the ring flops.  ``ring_g_bytes`` bounds the
-        scratch buffer of the ring operator G (pairs are processed in chunks that fit)."""
+        scratch buffer of the ring operator G (pairs are processed in chunks that fit).
+
+        ``vvvv_pairs`` runs the three W_vvvv terms as symmetric and antisymmetric products over
+        triangular index pairs (half the flops; dcc_vvvv_pair_* in the library), which requires a
+        pair-symmetric W_vvvv, W[a,b,d,e] = W[b,a,e,d], as every T1/T2-dressed one is; set_intermediates
+        checks a sample of it.  "auto" turns it on when the device has room for its buffers.  ``pair_bytes`` (default ``ring_g_bytes``) bounds their scratch buffers
+        (blocks are processed in chunks that fit)."""
         device_init(device)
         o, v = int(nocc), int(nvir)
         self.nocc, self.nvir = o, v
@@ -451,6 +500,24 @@ class TriplesEngine:
                         coef[TP_T3.index((a[b[0]], a[b[1]], a[b[2]]))] += 1.0
                     jobs.append(((pq * o + l) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
             self.z_jobs = DeviceArray.from_host(s3_jobs_array(jobs))
+        # --- pair-symmetric W_vvvv: weights [nS + nA][kpad], split T3 and products for a chunk of blocks
+        nS, nA, kp = pair_dims(v)
+        per_blk = 2 * v * (kp + nS) * dt.itemsize
+        pb = int(ring_g_bytes if pair_bytes is None else pair_bytes)
+        nbc = int(max(1, min(n3, pb // max(1, per_blk))))
+        nbc = -(-n3 // -(-n3 // nbc))              # equal chunks: no short remainder
+        if isinstance(vvvv_pairs, str):
+            if vvvv_pairs != "auto":
+                raise ValueError("vvvv_pairs must be True, False or 'auto'")
+            from distr_cc.device import mem_get_info
+            need = ((nS + nA) * kp + nbc * per_blk) * dt.itemsize
+            vvvv_pairs = mem_get_info()[0] > need + (24 << 30)
+        self.vvvv_pairs = bool(vvvv_pairs)
+        if self.vvvv_pairs:
+            self.pair_dims = (nS, nA, kp)
+            self.Wp = DeviceArray((nS + nA, kp), dt)
+            self.Tp = DeviceArray((2, nbc, v, kp), dt)
+            self.Up = DeviceArray((2, nbc, v, nS), dt)
         if contract not in ("native", "bf16x3") or (contract == "bf16x3" and dt != np.float32):
             raise ValueError("contract must be 'native', or 'bf16x3' with float32 storage")
         self.contract = contract
@@ -469,6 +536,9 @@ class TriplesEngine:
         else), by writing them as packed words in the first place; the others get packed copies.
         Returns the (source, packed, count) passes to run."""
         sources = [self.t3, self.Yz, self.W1, self.W_ovvv, self.W_oovo, self.t2, self.WR1, self.WR3, self.pfs]
+        # the pair-symmetric W_vvvv operands are written as packed words by their own kernels
+        in_place = [self.Wp, self.Tp] if self.vvvv_pairs else []
+        sources += in_place
         packed = {}
         # Y/Z are packed in place unless a GEMM kept on FP32 (K <= 16, tiny systems) reads them
         yz = self.Yz.ptr, self.Yz.ptr + self.Yz.nbytes
@@ -479,7 +549,8 @@ class TriplesEngine:
             for src in sources:
                 if src.ptr <= ptr < src.ptr + src.nbytes:
                     if id(src) not in packed:
-                        dst = src if (src is self.Yz and yz_in_place) else DeviceArray(src.shape, np.uint32)
+                        direct = (src is self.Yz and yz_in_place) or any(src is b for b in in_place)
+                        dst = src if direct else DeviceArray(src.shape, np.uint32)
                         packed[id(src)] = (src, dst)
                     return packed[id(src)]
             return None
@@ -498,7 +569,7 @@ class TriplesEngine:
         # Y/Z packed in place are written as packed words by materialize_y itself
         self._yz_direct = yz_in_place and id(self.Yz) in packed
         return [(src, dst, int(np.prod(src.shape))) for src, dst in packed.values()
-                if not (self._yz_direct and src is self.Yz)]
+                if not (self._yz_direct and src is self.Yz) and not any(src is b for b in in_place)]
 
     def pack_operands(self):
         """Run the split-BF16x3 packing passes (after T3, Y/Z and the intermediates are final)."""
@@ -529,10 +600,13 @@ class TriplesEngine:
         # W_vvvv x T3 with the block index in N (rccsdt.py:424-426); the F_vv terms (rccsdt.py:400-402)
         # ride along in W1 (see __init__)
         one = [dict()]
-        for sub, tl in [('abde,Bdec->Babc', 'Bdec'), ('acde,Bdbe->Babc', 'Bdbe'), ('bcde,Bade->Babc', 'Bade')]:
-            A = Operand(self.W1, sub[:4], (v, v, v, v))
-            B = Operand(self.t3, tl, (n3, v, v, v))
-            launches.append(plan_term('W_vvvv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
+        if self.vvvv_pairs:
+            launches += self._plan_vvvv_pairs()
+        else:
+            for sub, tl in [('abde,Bdec->Babc', 'Bdec'), ('acde,Bdbe->Babc', 'Bdbe'), ('bcde,Bade->Babc', 'Bade')]:
+                A = Operand(self.W1, sub[:4], (v, v, v, v))
+                B = Operand(self.t3, tl, (n3, v, v, v))
+                launches.append(plan_term('W_vvvv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
         launches += self._plan_ring()
         # W_oooo / F_oo (rccsdt.py:726-731).  The abc permutation of each of the six terms commutes
         # with the sum over source blocks p, so every term is the same natural GEMM
@@ -564,6 +638,38 @@ class TriplesEngine:
                     srcs[p], coef[p] = ((s - 1) * qc + q - q0) * v3, 1.0
                 jobs.append((q * v3, srcs, coef))
             launches.append(S3GatherLaunch('W_oooo/F_oo add', self.R, self.Xp, v, 1.0, jobs))
+        return launches
+
+    def _plan_vvvv_pairs(self):
+        """W_vvvv x T3 (rccsdt.py:424-426) as pair-symmetric products.  Term t has its output pair and
+        T3 pair at block positions (0,1), (0,2), (1,2) (pair_axes t); per chunk of blocks: one split of
+        T3 into TS/TA, one GEMM with two batch entries (WS x TS over the nS symmetric pairs, WA x TA over
+        the nA strict ones, common K = kpad) into US/UA, one unfold adding (US +- UA)/2 into R.  Axes 0
+        keeps the shared index innermost ([pair][z] rows: B is N-contiguous, as T3 is); axes 1 and 2 put
+        the pairs innermost (B K-contiguous, for the direct-to-LDS tiles)."""
+        v, n3 = self.nvir, self.n3
+        nS, nA, kp = self.pair_dims
+        nbc = self.Tp.shape[1]
+        packed = self.contract == "bf16x3"
+        split_fn = "dcc_vvvv_pair_split_" + ("packed_f32" if packed else self.sfx)
+        unfold_fn = "dcc_vvvv_pair_unfold_" + self.sfx
+        launches = []
+        for axes, sub in enumerate(('abde,Bdec->Babc', 'acde,Bdbe->Babc', 'bcde,Bade->Babc')):
+            for b0 in range(0, n3, nbc):
+                nb = min(nbc, n3 - b0)
+                launches.append(PairLaunch('W_vvvv pairs split', split_fn, self.t3, self.Tp, b0, nb, v, axes, kp))
+                am, ak = L.IdxMap(None, 0, 1, nS, 0, 0, kp), L.IdxMap(None, 0, 1, kp, 0, 0, 1)
+                if axes == 0:     # Tp [2][nb][kp][v], Up [2][nb][nS][v]
+                    bk, bn = L.IdxMap(None, 0, 1, kp, 0, 0, v), L.IdxMap(None, 0, nb, v, 0, kp * v, 1)
+                    cm, cn = L.IdxMap(None, 0, 1, nS, 0, 0, v), L.IdxMap(None, 0, nb, v, 0, nS * v, 1)
+                else:             # Tp [2][nb][v][kp], Up [2][nb][v][nS]
+                    bk, bn = L.IdxMap(None, 0, 1, kp, 0, 0, 1), L.IdxMap(None, 0, nb, v, 0, v * kp, kp)
+                    cm, cn = L.IdxMap(None, 0, 1, nS, 0, 0, 1), L.IdxMap(None, 0, nb, v, 0, v * nS, nS)
+                batches = [(0, 0, 0, 0, nS), (nS * kp, nb * v * kp, nb * v * nS, 0, nA)]
+                launches.append(GemmLaunch('W_vvvv pairs ' + sub, nS, nb * v, kp, 1.0, 0.0, self.Wp, am, ak,
+                                           self.Tp, bk, bn, self.Up, cm, cn, batches, nS,
+                                           2.0 * (nS + nA) * nb * v * kp))
+                launches.append(PairLaunch('W_vvvv pairs unfold', unfold_fn, self.R, self.Up, b0, nb, v, axes))
         return launches
 
     def _plan_t2t3(self):
@@ -791,6 +897,9 @@ class TriplesEngine:
                      Operand.strided(F, {'a': v, 'd': 1}), 'abd', (v, v, v), 0.5, 1.0, self.stream)
         tensor_axpby(Operand.strided(self.W1, {'a': v3 + v, 'b': v2, 'e': 1}),
                      Operand.strided(F, {'b': v, 'e': 1}), 'abe', (v, v, v), 0.5, 1.0, self.stream)
+        if self.vvvv_pairs:
+            fn = "dcc_vvvv_pair_weights_" + ("packed_f32" if self.contract == "bf16x3" else self.sfx)
+            L.call(fn, self.W1.vp, v, self.pair_dims[2], self.Wp.vp, s)
         L.call("dcc_ring_weights_" + self.sfx, self.W_ovov.vp, self.W_ovvo.vp, o, v, self.WR1.vp, self.WR3a.vp,
                self.WR3b.vp, s)
         L.call("dcc_oooo_prefactors_" + self.sfx, self.W_oooo.vp, self.F_oo.vp, o, self.blk_ijk.vp, self.n3,
@@ -890,6 +999,8 @@ class TriplesEngine:
 
     def set_intermediates(self, imds, t2):
         o, v = self.nocc, self.nvir
+        if self.vvvv_pairs:
+            check_pair_symmetric(imds['W_vvvv'])
         for name in ('F_oo', 'F_vv', 'W_oooo', 'W_ovvo', 'W_ovov', 'W_vooo', 'W_vvvo', 'W_vvvv'):
             getattr(self, name).upload(np.ascontiguousarray(imds[name], dtype=self.dtype), self.stream)
         self.t2.upload(np.ascontiguousarray(t2, dtype=self.dtype), self.stream)

@@ -372,6 +372,41 @@ int dcc_t2t3_operands_packed_f32(uint32_t *Ap, uint32_t *Bp, const float *W_ovvv
                                  dcc_stream_t stream);
 
 /*
+ * Pair-symmetric W_vvvv x T3 (distr_cc/rccsdt.py:424-426).  Each of the three terms is
+ * r[..x..y..] += sum_{d,e} W1[x,y,d,e] T[..d..e..] with the output pair (x, y) and the T3 pair (d, e) at
+ * the same two positions of a v^3 block (pair_axes 0 = positions (0,1), 1 = (0,2), 2 = (1,2)) and the
+ * third index z shared.  For a pair-symmetric W1 (W1[x,y,d,e] = W1[y,x,e,d], as every dressed W_vvvv
+ * is) the term is two products over triangular pairs, of half the flops of the plain one:
+ *   US[p(x<=y)] = sum_q WS[p,q] TS[q],  UA[p(x<y)] = sum_q WA[p,q] TA[q],
+ *   r[xy] += (US + UA)/2, r[yx] += (US - UA)/2, r[xx] += US/2,
+ * with TS[q(d<=e)] = T[de] + T[ed], TA[q(d<e)] = T[de] - T[ed], WS[p,q(d<e)] = W1[p,de] + W1[p,ed],
+ * WS[p,q(d=d)] = W1[p,dd], WA[p,q] = W1[p,de] - W1[p,ed].  Pairs are numbered row-major over the upper
+ * triangle: q(d<=e) = d v - d(d-1)/2 + e - d (nS = v(v+1)/2 pairs), q(d<e) = d(v-1) - d(d-1)/2 + e-d-1
+ * (nA = v(v-1)/2).  kpad >= nS is the padded K of both products.  Layouts:
+ *   Wp [nS + nA][kpad]                          WS rows then WA rows, zero past nS / nA columns
+ *   Tp [2][nblk][kpad][v] (pair_axes 0), [2][nblk][v][kpad] (1, 2)   TS then TA, zero padded
+ *   Up [2][nblk][nS][v]   (pair_axes 0), [2][nblk][v][nS]   (1, 2)   US then UA (UA in the first nA)
+ * dcc_vvvv_pair_weights_* builds Wp from W1 (nvir^4); dcc_vvvv_pair_split_* builds Tp from nblk
+ * consecutive blocks of T3; the caller forms Up with one batched GEMM (two entries: WS x TS, WA x TA);
+ * dcc_vvvv_pair_unfold_* adds it into nblk consecutive blocks of R.  The _packed_f32 forms write
+ * split-BF16x3 words.  No reference counterpart (the reference runs the plain v^2 x v^2 einsums).
+ */
+int dcc_vvvv_pair_weights_f64(const double *W1, int64_t nvir, int64_t kpad, double *Wp, dcc_stream_t stream);
+int dcc_vvvv_pair_weights_f32(const float *W1, int64_t nvir, int64_t kpad, float *Wp, dcc_stream_t stream);
+int dcc_vvvv_pair_weights_packed_f32(const float *W1, int64_t nvir, int64_t kpad, uint32_t *Wp,
+                                     dcc_stream_t stream);
+int dcc_vvvv_pair_split_f64(const double *t3, int64_t nblk, int64_t nvir, int32_t pair_axes, int64_t kpad,
+                            double *Tp, dcc_stream_t stream);
+int dcc_vvvv_pair_split_f32(const float *t3, int64_t nblk, int64_t nvir, int32_t pair_axes, int64_t kpad,
+                            float *Tp, dcc_stream_t stream);
+int dcc_vvvv_pair_split_packed_f32(const float *t3, int64_t nblk, int64_t nvir, int32_t pair_axes, int64_t kpad,
+                                   uint32_t *Tp, dcc_stream_t stream);
+int dcc_vvvv_pair_unfold_f64(const double *Up, int64_t nblk, int64_t nvir, int32_t pair_axes, double *R,
+                             dcc_stream_t stream);
+int dcc_vvvv_pair_unfold_f32(const float *Up, int64_t nblk, int64_t nvir, int32_t pair_axes, float *R,
+                             dcc_stream_t stream);
+
+/*
  * dst[i] = beta * dst[i] + alpha * sum_{s < nslices} src[s * n + i] (FP64 sum in slice order): the
  * reduction of split-K partial products of the T3 -> r1/r2/W contractions (distr_cc/rccsdt.py:253-366).
  */

@@ -237,6 +237,93 @@ class Emulator:
                 dev.view(_ptr(ptr), dt, x.size)[:] = x.ravel()
         return 0
 
+    def _store(self, ptr, x, dt, packed):
+        """Write x at ptr as dt values, or as split-BF16x3 words when packed."""
+        if packed:
+            hi, lo = bf16_split(x.astype(np.float32).ravel())
+            self.dev.view(_ptr(ptr), np.uint32, x.size)[:] = ((hi.astype(np.float32).view(np.uint32) >> 16) |
+                                                              (lo.astype(np.float32).view(np.uint32) & 0xFFFF0000))
+        else:
+            self.dev.view(_ptr(ptr), dt, x.size)[:] = x.ravel()
+
+    def _pair_weights(self, dt, packed, W1, nvir, kpad, Wp, stream):
+        v, kp = int(nvir), int(kpad)
+        if kp < v * (v + 1) // 2:
+            return 1
+        W = self.dev.view(_ptr(W1), dt, v ** 4).reshape(v, v, v, v)
+        dS, eS = np.triu_indices(v)
+        dA, eA = np.triu_indices(v, 1)
+        out = np.zeros((len(dS) + len(dA), kp), dt)
+        WS = W[dS, eS]                                   # (nS, v, v) rows x <= y
+        out[:len(dS), :len(dS)] = WS[:, dS, eS] + WS[:, eS, dS] * (dS != eS)[None, :]
+        WA = W[dA, eA]
+        out[len(dS):, :len(dA)] = WA[:, dA, eA] - WA[:, eA, dA]
+        self._store(Wp, out, dt, packed)
+        return 0
+
+    @staticmethod
+    def _pair_plane(axes):
+        """(nb, v, v, v) block view -> [b, z, d, e] with the pair on the last two axes."""
+        return {0: (0, 3, 1, 2), 1: (0, 2, 1, 3), 2: (0, 1, 2, 3)}[axes]
+
+    def _pair_split(self, dt, packed, t3, nblk, nvir, axes, kpad, Tp, stream):
+        nb, v, kp, axes = int(nblk), int(nvir), int(kpad), int(axes)
+        if kp < v * (v + 1) // 2 or axes not in (0, 1, 2):
+            return 1
+        P = self.dev.view(_ptr(t3), dt, nb * v ** 3).reshape(nb, v, v, v).transpose(self._pair_plane(axes))
+        dS, eS = np.triu_indices(v)
+        dA, eA = np.triu_indices(v, 1)
+        out = np.zeros((2, nb, v, kp), dt)
+        out[0, :, :, :len(dS)] = P[:, :, dS, eS] + P[:, :, eS, dS]
+        out[1, :, :, :len(dA)] = P[:, :, dA, eA] - P[:, :, eA, dA]
+        if axes == 0:
+            out = out.transpose(0, 1, 3, 2)               # [2][nb][kp][v]
+        self._store(Tp, np.ascontiguousarray(out), dt, packed)
+        return 0
+
+    def _pair_unfold(self, dt, Up, nblk, nvir, axes, R, stream):
+        nb, v, axes = int(nblk), int(nvir), int(axes)
+        if axes not in (0, 1, 2):
+            return 1
+        dS, eS = np.triu_indices(v)
+        dA, eA = np.triu_indices(v, 1)
+        mp = len(dS)
+        U = self.dev.view(_ptr(Up), dt, 2 * nb * mp * v)
+        U = U.reshape(2, nb, mp, v).transpose(0, 1, 3, 2) if axes == 0 else U.reshape(2, nb, v, mp)
+        US, UA = U[0], U[1][:, :, :len(dA)]
+        qS = dA * v - dA * (dA - 1) // 2 + (eA - dA)          # symmetric index of each strict pair
+        full = np.zeros((nb, v, v, v), dt)                     # [b, z, x, y]
+        full[:, :, dS, eS] = 0.5 * US
+        full[:, :, dA, eA] = 0.5 * (US[:, :, qS] + UA)
+        full[:, :, eA, dA] = 0.5 * (US[:, :, qS] - UA)
+        Rv = self.dev.view(_ptr(R), dt, nb * v ** 3).reshape(nb, v, v, v).transpose(self._pair_plane(axes))
+        Rv += full
+        return 0
+
+    def dcc_vvvv_pair_weights_f64(self, *a):
+        return self._pair_weights(np.float64, False, *a)
+
+    def dcc_vvvv_pair_weights_f32(self, *a):
+        return self._pair_weights(np.float32, False, *a)
+
+    def dcc_vvvv_pair_weights_packed_f32(self, *a):
+        return self._pair_weights(np.float32, True, *a)
+
+    def dcc_vvvv_pair_split_f64(self, *a):
+        return self._pair_split(np.float64, False, *a)
+
+    def dcc_vvvv_pair_split_f32(self, *a):
+        return self._pair_split(np.float32, False, *a)
+
+    def dcc_vvvv_pair_split_packed_f32(self, *a):
+        return self._pair_split(np.float32, True, *a)
+
+    def dcc_vvvv_pair_unfold_f64(self, *a):
+        return self._pair_unfold(np.float64, *a)
+
+    def dcc_vvvv_pair_unfold_f32(self, *a):
+        return self._pair_unfold(np.float32, *a)
+
     def dcc_t2t3_operands_f64(self, *a):
         return self._t2t3_operands(np.float64, False, *a)
 

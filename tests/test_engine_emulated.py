@@ -51,6 +51,61 @@ def test_engine_ring_chunked(monkeypatch, o, v, pairs, fused):
     assert abs(eng.R.download() - Ro).max() <= 1e-12 * abs(Ro).max()
 
 
+def pair_symmetric_noise(im, rng, scale=1e-3):
+    """Independent noise on every intermediate except W_vvvv, whose noise keeps its pair symmetry
+    W[a,b,d,e] = W[b,a,e,d] (and nothing more: it is not symmetric in (a,b) or (d,e) alone)."""
+    out = {k: x + scale * rng.normal(size=x.shape) for k, x in im.items()}
+    n = scale * rng.normal(size=im['W_vvvv'].shape)
+    out['W_vvvv'] = im['W_vvvv'] + n + n.transpose(1, 0, 3, 2)
+    return out
+
+
+@pytest.mark.parametrize("o,v,prec,chunk", [(3, 5, "fp64", None), (4, 4, "fp64", 7), (3, 6, "fp32", 5),
+                                            (3, 7, "bf16x3", None), (2, 9, "bf16x3", 2)])
+def test_engine_vvvv_pairs(monkeypatch, o, v, prec, chunk):
+    """W_vvvv x T3 as symmetric / antisymmetric products over triangular pairs (split, one two-entry
+    GEMM, unfold per term and chunk of blocks) against the oracle's plain einsums."""
+    abi_emulator.install(monkeypatch)
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import GemmLaunch, TriplesEngine, pair_dims
+    s = SyntheticSystem(o, v, naux=20, seed=41 + v)
+    im = pair_symmetric_noise(s.bare_intermediates(), np.random.default_rng(6))
+    t2, t3 = s.t2(), s.t3_tri()
+    kw = {"fp64": {}, "fp32": dict(dtype=np.float32),
+          "bf16x3": dict(dtype=np.float32, contract="bf16x3")}[prec]
+    nS, nA, kp = pair_dims(v)
+    assert (nS, nA) == (v * (v + 1) // 2, v * (v - 1) // 2) and kp % 64 == 0 and kp >= nS
+    isz = 8 if prec == "fp64" else 4
+    eng = TriplesEngine(o, v, vvvv_pairs=True,
+                        pair_bytes=None if chunk is None else chunk * 2 * v * (kp + nS) * isz, **kw)
+    nchunk = 1 if chunk is None else -(-eng.n3 // chunk)
+    pairs = [g for g in eng.launches if isinstance(g, GemmLaunch) and g.name.startswith("W_vvvv pairs")]
+    assert len(pairs) == 3 * nchunk and not any(g.name.startswith("W_vvvv a") for g in eng.launches)
+    if prec == "bf16x3":
+        assert all(g.fn == "dcc_contract_packed_bf16x3" for g in pairs)
+    eng.set_t3(t3)
+    eng.set_intermediates(im, t2)
+    eng.residual()
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3)
+    tol = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 3e-4}[prec]
+    assert abs(eng.R.download().astype(np.float64) - Ro).max() <= tol * abs(Ro).max()
+
+
+def test_engine_vvvv_pairs_rejects_asymmetric(monkeypatch):
+    """An engine with pair-symmetric W_vvvv products refuses intermediates whose W_vvvv lacks the
+    symmetry, instead of computing the symmetrised term."""
+    abi_emulator.install(monkeypatch)
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine
+    s = SyntheticSystem(3, 4, naux=20, seed=5)
+    rng = np.random.default_rng(8)
+    im = {k: x + 1e-3 * rng.normal(size=x.shape) for k, x in s.bare_intermediates().items()}
+    eng = TriplesEngine(3, 4, vvvv_pairs=True)
+    with pytest.raises(ValueError, match="pair-symmetric"):
+        eng.set_intermediates(im, s.t2())
+    eng.set_intermediates(pair_symmetric_noise(s.bare_intermediates(), rng), s.t2())
+
+
 def test_engine_oooo_chunk_rounding(monkeypatch):
     """W_oooo/F_oo target chunks: a bound of 52 targets is rounded up to one 64-row chunk (within
     5/4 of the bound), leaving a partial second chunk of 20 of the 84 targets."""

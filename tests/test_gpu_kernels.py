@@ -378,6 +378,111 @@ def test_residual_fp32_vs_oracle(dev, o, v):
     assert abs(n2 - n2o) <= 1e-4 * n2o
 
 
+_PAIR_PLANE = {0: (0, 3, 1, 2), 1: (0, 2, 1, 3), 2: (0, 1, 2, 3)}   # block -> [b, z, d, e]
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32", "packed_f32"])
+@pytest.mark.parametrize("v,nb", [(5, 3), (33, 2), (40, 2), (64, 1)])
+def test_vvvv_pair_kernels(dev, dt, v, nb):
+    """dcc_vvvv_pair_weights / _split / _unfold element for element against NumPy for all three pair
+    axes (packed words against dcc_pack_bf16x3 of the FP32 values); outputs start as NaN / junk words so
+    padding the kernels leave unwritten shows."""
+    from distr_cc import _lib as L
+    from distr_cc.device import DeviceArray
+    from distr_cc.t3_engine import pair_dims
+    _say("vvvv pair kernels %s v=%d nb=%d" % (dt, v, nb))
+    rng = np.random.default_rng(v * 10 + nb)
+    typ = np.float64 if dt == "f64" else np.float32
+    nS, nA, kp = pair_dims(v)
+    dS, eS = np.triu_indices(v)
+    dA, eA = np.triu_indices(v, 1)
+
+    def junk(shape):
+        if dt == "packed_f32":
+            return DeviceArray.from_host(np.full(shape, 0xDEADBEEF, np.uint32))
+        return DeviceArray.from_host(np.full(shape, np.nan, typ))
+
+    def check(got, ref):
+        if dt == "packed_f32":
+            r = DeviceArray.from_host(np.ascontiguousarray(ref, np.float32))
+            w = DeviceArray(ref.shape, np.uint32)
+            L.call("dcc_pack_bf16x3", r.vp, w.vp, ref.size, None)
+            assert np.array_equal(got.download(), w.download())
+        else:
+            assert np.array_equal(got.download(), ref)
+
+    W1 = rng.normal(size=(v, v, v, v)).astype(typ)
+    ref = np.zeros((nS + nA, kp), typ)
+    WS, WA = W1[dS, eS], W1[dA, eA]
+    ref[:nS, :nS] = np.where(dS == eS, WS[:, dS, eS], WS[:, dS, eS] + WS[:, eS, dS])
+    ref[nS:, :nA] = WA[:, dA, eA] - WA[:, eA, dA]
+    got, dW1 = junk(ref.shape), DeviceArray.from_host(W1)   # named: a temporary would be freed before the launch
+    L.call("dcc_vvvv_pair_weights_" + dt, dW1.vp, v, kp, got.vp, None)
+    check(got, ref)
+    T = rng.normal(size=(nb, v, v, v)).astype(typ)
+    dT = DeviceArray.from_host(T)
+    for axes in range(3):
+        P = T.transpose(_PAIR_PLANE[axes])
+        ref = np.zeros((2, nb, v, kp), typ)
+        ref[0, :, :, :nS] = P[:, :, dS, eS] + P[:, :, eS, dS]
+        ref[1, :, :, :nA] = P[:, :, dA, eA] - P[:, :, eA, dA]
+        if axes == 0:
+            ref = np.ascontiguousarray(ref.transpose(0, 1, 3, 2))
+        got = junk(ref.shape)
+        L.call("dcc_vvvv_pair_split_" + dt, dT.vp, nb, v, axes, kp, got.vp, None)
+        check(got, ref)
+        if dt == "packed_f32":
+            continue
+        U = rng.normal(size=(2, nb, v, nS)).astype(typ)       # [part][b][z][p]
+        R0 = rng.normal(size=(nb, v, v, v)).astype(typ)
+        qS = dA * v - dA * (dA - 1) // 2 + (eA - dA)
+        full = np.zeros((nb, v, v, v), typ)                   # [b, z, x, y]
+        full[:, :, dS, eS] = typ(0.5) * U[0]
+        full[:, :, dA, eA] = typ(0.5) * (U[0][:, :, qS] + U[1][:, :, :nA])
+        full[:, :, eA, dA] = typ(0.5) * (U[0][:, :, qS] - U[1][:, :, :nA])
+        want = R0.copy()
+        want.transpose(_PAIR_PLANE[axes])[...] += full
+        dU = DeviceArray.from_host(np.ascontiguousarray(U.transpose(0, 1, 3, 2)) if axes == 0 else U)
+        dR = DeviceArray.from_host(R0)
+        L.call("dcc_vvvv_pair_unfold_" + dt, dU.vp, nb, v, axes, dR.vp, None)
+        assert np.array_equal(dR.download(), want), axes
+
+
+@pytest.mark.parametrize("prec,o,v,chunk", [("fp64", 3, 33, None), ("fp64", 4, 20, 7), ("fp32", 3, 40, None),
+                                            ("bf16x3", 3, 33, None), ("bf16x3", 4, 50, 9)])
+def test_residual_vvvv_pairs_vs_oracle(dev, prec, o, v, chunk):
+    """The residual with pair-symmetric W_vvvv products (chunks of blocks where given) against the oracle
+    and against the same engine with the plain W_vvvv GEMMs, on intermediates with independent noise
+    except a pair-symmetric one on W_vvvv."""
+    from distr_cc.synthetic import SyntheticSystem
+    from distr_cc.t3_engine import TriplesEngine, pair_dims
+    from oracle import rccsdt_tri as tri
+    _say("residual vvvv pairs %s o=%d v=%d chunk=%s" % (prec, o, v, chunk))
+    s = SyntheticSystem(o, v, naux=30, seed=900 + o * v)
+    rng = np.random.default_rng(901)
+    im = {k: x + 1e-3 * rng.normal(size=x.shape) for k, x in s.bare_intermediates().items()}
+    n = 1e-3 * rng.normal(size=im['W_vvvv'].shape)
+    im['W_vvvv'] = s.bare_intermediates()['W_vvvv'] + n + n.transpose(1, 0, 3, 2)
+    t2, t3 = s.t2(), s.t3_tri()
+    kw = {"fp64": {}, "fp32": dict(dtype=np.float32), "bf16x3": dict(dtype=np.float32, contract="bf16x3")}[prec]
+    nS, nA, kp = pair_dims(v)
+    isz = 8 if prec == "fp64" else 4
+    eng = TriplesEngine(o, v, vvvv_pairs=True,
+                        pair_bytes=None if chunk is None else chunk * 2 * v * (kp + nS) * isz, **kw)
+    # chunks of at most `chunk` blocks, made equal: 20 blocks with a bound of 9 are 3 chunks of 7
+    assert eng.Tp.shape[1] == (eng.n3 if chunk is None else -(-eng.n3 // -(-eng.n3 // chunk)))
+    R = []
+    for e in (eng, TriplesEngine(o, v, **kw)):
+        e.set_t3(t3)
+        e.set_intermediates(im, t2)
+        e.residual()
+        R.append(e.R.download().astype(np.float64))
+    Ro = tri.compute_r3_tri(o, v, im, t2, t3)
+    tol = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 3e-4}[prec]
+    assert abs(R[0] - Ro).max() <= tol * abs(Ro).max(), abs(R[0] - Ro).max() / abs(Ro).max()
+    assert abs(R[0] - R[1]).max() <= tol * abs(Ro).max()
+
+
 @pytest.mark.parametrize("M,N,K", [(16, 16, 32), (37, 53, 29), (130, 70, 300), (64, 64, 64)])
 def test_contract_bf16x3_plain_gemm(dev, M, N, K):
     """Split-BF16x3 path: pins the v_mfma_f32_16x16x32_bf16 operand/accumulator layouts and the split
