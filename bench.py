@@ -114,7 +114,7 @@ def _tiles_from_small(dt, contract, seed, sigma_b, o=10, v=90):
     sysm = SyntheticSystem(o, v, naux=300, seed=seed, sigma_b=sigma_b)
     st = Stream()
     eng = RCCSDTEngine(o, v, sysm.fock, sysm.pppp(), sysm.mo_energy, stream=st, t3_dtype=dt,
-                       triples_contract=contract)
+                       triples_contract=contract, vvvv_pairs=VVVV_PAIRS["mode"])
     eng.set_amplitudes(np.zeros((o, v)), sysm.t2(), sysm.t3_tri())
     eng.update_amps()
     eng.tr.autotune()
@@ -127,6 +127,9 @@ def _tiles_from_small(dt, contract, seed, sigma_b, o=10, v=90):
 
 PRECISIONS = {"fp64": (np.float64, "native"), "fp32": (np.float32, "native"), "bf16x3": (np.float32, "bf16x3")}
 TILE_DIR = os.path.join(ROOT, "distributed-cc_amd", "distr_cc", "tiles")
+# W_vvvv terms as pair-symmetric products ("auto": when the device has room for their weights) or as the
+# plain GEMMs (False); set from --vvvv-pairs
+VVVV_PAIRS = {"mode": "auto"}
 
 
 def tuned_tiles(precision, o, v):
@@ -165,7 +168,7 @@ def iterate(inp, precision, iters, warmup, nvir_diis, tiles=None, launch_prefix=
     dt, contract = PRECISIONS[precision]
     st = Stream()
     eng = RCCSDTEngine(o, v, inp.sysm.fock, inp.pppp, inp.sysm.mo_energy, stream=st, t3_dtype=dt,
-                       triples_contract=contract, nvir_diis=nvir_diis)
+                       triples_contract=contract, nvir_diis=nvir_diis, vvvv_pairs=VVVV_PAIRS["mode"])
     if tiles:
         eng.tr.set_tiles_by_name(tiles)
     eng.set_amplitudes(np.zeros((o, v)), inp.t2, inp.t3)
@@ -541,7 +544,7 @@ def tune_tiles(o, v, precision, sigma_b, seed=20261202):
     dt, contract = PRECISIONS[precision]
     st = Stream()
     eng = RCCSDTEngine(o, v, inp.sysm.fock, inp.pppp, inp.sysm.mo_energy, stream=st, t3_dtype=dt,
-                       triples_contract=contract)
+                       triples_contract=contract, vvvv_pairs=VVVV_PAIRS["mode"])
     eng.set_amplitudes(np.zeros((o, v)), inp.t2, inp.t3)
     eng.update_amps()
     print("bench: autotuning %s tiles at nocc=%d nvir=%d" % (precision, o, v), file=sys.stderr, flush=True)
@@ -585,7 +588,7 @@ def plain_fp64_residual(o, v, steps, warmup, autotune=True, tiles=None, breakdow
     im = sysm.bare_intermediates()
     t2, t3 = sysm.t2(), sysm.t3_tri()
     stream = Stream()
-    eng = TriplesEngine(o, v, stream=stream)
+    eng = TriplesEngine(o, v, stream=stream, vvvv_pairs=VVVV_PAIRS["mode"])
     eng.set_t3(t3)
     eng.set_intermediates(im, t2)
     eng.set_denominators(sysm.mo_energy)
@@ -775,7 +778,8 @@ def main():
                     help="only time one engine's residual at --nocc/--nvir (tiles via --tiles) and exit")
     ap.add_argument("--tiles", metavar="JSON", default=None, help="tile file of --probe-residual")
     ap.add_argument("--vvvv-pairs", type=int, default=1,
-                    help="pair-symmetric W_vvvv products in the triples engines (0 = plain GEMMs)")
+                    help="pair-symmetric W_vvvv products in the triples engines when the device has room "
+                         "(0 = plain GEMMs everywhere)")
     ap.add_argument("--probe-batched", type=int, default=0, metavar="NSYS",
                     help="only the batched ensemble of NSYS systems (--steps steps issued eagerly, no "
                          "per-system solves); prints its JSON (for profiling)")
@@ -785,6 +789,7 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
+    VVVV_PAIRS["mode"] = "auto" if args.vvvv_pairs else False
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs the triples path on one GPU; --gpus %d is not supported "
               "(the canonical i<=j<=k blocks would shard over GPUs as DistributedT3IJK distributes them; "
@@ -798,7 +803,7 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.probe_residual:
-        probe_residual(o, v, args.probe_residual, args.tiles, vvvv_pairs=bool(args.vvvv_pairs))
+        probe_residual(o, v, args.probe_residual, args.tiles, vvvv_pairs=VVVV_PAIRS["mode"])
         return
     if args.tune_tiles:
         tune_tiles(o, v, args.tune_tiles, args.sigma_b)
@@ -850,7 +855,7 @@ def main():
     flops = float(np.mean([x[1] for x in lw]))
     achieved = 3.0 * flops / (avg_ms * 1e-3) / 1e12
     if "fp64" in runs and runs["fp64"]["launches"]:
-        l64 = runs["fp64"]["launches"]
+        l64 = [x for x in runs["fp64"]["launches"] if x[1] > 0]
         a64 = float(np.mean([x[2] for x in l64]))
         mixed["fp64_w_vvvv"] = {"avg_ms": a64, "tflops": float(np.mean([x[1] for x in l64])) / (a64 * 1e-3) / 1e12}
         mixed["fp64_w_vvvv"]["frac_fp64_peak"] = mixed["fp64_w_vvvv"]["tflops"] / PEAK_FP64_MFMA_TFLOPS

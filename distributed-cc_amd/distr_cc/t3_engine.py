@@ -414,8 +414,9 @@ class TriplesEngine:
         ``vvvv_pairs`` runs the three W_vvvv terms as symmetric and antisymmetric products over
         triangular index pairs (half the flops; dcc_vvvv_pair_* in the library), which requires a
         pair-symmetric W_vvvv, W[a,b,d,e] = W[b,a,e,d], as every T1/T2-dressed one is; set_intermediates
-        checks a sample of it.  "auto" turns it on when the device has room for its buffers.  ``pair_bytes`` (default ``ring_g_bytes``) bounds their scratch buffers
-        (blocks are processed in chunks that fit)."""
+        checks a sample of it.  "auto" turns it on when the device has room for its weights (its
+        other buffers are views of the T2 -> T3 and ring scratch).  ``pair_bytes`` (optional) bounds
+        that scratch further (blocks are processed in chunks that fit)."""
         device_init(device)
         o, v = int(nocc), int(nvir)
         self.nocc, self.nvir = o, v
@@ -501,23 +502,19 @@ class TriplesEngine:
                     jobs.append(((pq * o + l) * v3, ijk_to_linear(ci, cj, ck, o) * v3, coef))
             self.z_jobs = DeviceArray.from_host(s3_jobs_array(jobs))
         # --- pair-symmetric W_vvvv: weights [nS + nA][kpad], split T3 and products for a chunk of blocks
+        # (the split T3 and the products of a chunk of blocks live in the T2 -> T3 and ring scratch
+        # buffers, idle during the W_vvvv terms: only the weights take memory of their own)
         nS, nA, kp = pair_dims(v)
-        per_blk = 2 * v * (kp + nS) * dt.itemsize
-        pb = int(ring_g_bytes if pair_bytes is None else pair_bytes)
-        nbc = int(max(1, min(n3, pb // max(1, per_blk))))
-        nbc = -(-n3 // -(-n3 // nbc))              # equal chunks: no short remainder
         if isinstance(vvvv_pairs, str):
             if vvvv_pairs != "auto":
                 raise ValueError("vvvv_pairs must be True, False or 'auto'")
             from distr_cc.device import mem_get_info
-            need = ((nS + nA) * kp + nbc * per_blk) * dt.itemsize
-            vvvv_pairs = mem_get_info()[0] > need + (24 << 30)
+            vvvv_pairs = mem_get_info()[0] > (nS + nA) * kp * dt.itemsize + (24 << 30)
         self.vvvv_pairs = bool(vvvv_pairs)
+        self.pair_bytes = None if pair_bytes is None else int(pair_bytes)
         if self.vvvv_pairs:
             self.pair_dims = (nS, nA, kp)
             self.Wp = DeviceArray((nS + nA, kp), dt)
-            self.Tp = DeviceArray((2, nbc, v, kp), dt)
-            self.Up = DeviceArray((2, nbc, v, nS), dt)
         if contract not in ("native", "bf16x3") or (contract == "bf16x3" and dt != np.float32):
             raise ValueError("contract must be 'native', or 'bf16x3' with float32 storage")
         self.contract = contract
@@ -597,17 +594,19 @@ class TriplesEngine:
         # T2 -> T3 (rccsdt.py:388-399)
         Rb = Operand(self.R, 'Babc', (n3, v, v, v))
         launches += self._plan_t2t3()
+        ring = self._plan_ring()
         # W_vvvv x T3 with the block index in N (rccsdt.py:424-426); the F_vv terms (rccsdt.py:400-402)
         # ride along in W1 (see __init__)
         one = [dict()]
         if self.vvvv_pairs:
+            self._pair_scratch()
             launches += self._plan_vvvv_pairs()
         else:
             for sub, tl in [('abde,Bdec->Babc', 'Bdec'), ('acde,Bdbe->Babc', 'Bdbe'), ('bcde,Bade->Babc', 'Bade')]:
                 A = Operand(self.W1, sub[:4], (v, v, v, v))
                 B = Operand(self.t3, tl, (n3, v, v, v))
                 launches.append(plan_term('W_vvvv ' + sub, sub, A, B, Rb, 1.0, 1.0, one, ext))
-        launches += self._plan_ring()
+        launches += ring
         # W_oooo / F_oo (rccsdt.py:726-731).  The abc permutation of each of the six terms commutes
         # with the sum over source blocks p, so every term is the same natural GEMM
         # X_s[q] = sum_p pfs[s][p][q] T[p] (coalesced in and out) followed, for s > 0, by R[q] +=
@@ -639,6 +638,25 @@ class TriplesEngine:
                 jobs.append((q * v3, srcs, coef))
             launches.append(S3GatherLaunch('W_oooo/F_oo add', self.R, self.Xp, v, 1.0, jobs))
         return launches
+
+    def _pair_scratch(self):
+        """Tp (split T3) and Up (products) of the pair-symmetric W_vvvv terms for a chunk of blocks, as
+        views of the T2 -> T3 base-term buffer Xt and the ring scratch Gs: the W_vvvv launches run after
+        the T2 -> T3 ones have consumed Xt and before the ring writes Gs (one stream).  Chunks hold as
+        many blocks as both buffers (and ``pair_bytes``, if given) allow, made equal in size."""
+        v, n3, dt = self.nvir, self.n3, self.dtype
+        nS, nA, kp = self.pair_dims
+        t_blk, u_blk = 2 * v * kp, 2 * v * nS          # elements per block
+        cap = min(int(np.prod(self.Xt.shape)) // t_blk, int(np.prod(self.Gs.shape)) // u_blk, n3)
+        if self.pair_bytes is not None:
+            cap = min(cap, self.pair_bytes // ((t_blk + u_blk) * dt.itemsize))
+        if cap >= 1:
+            nbc = -(-n3 // -(-n3 // cap))               # equal chunks: no short remainder
+            self.Tp = DeviceArray((2, nbc, v, kp), dt, ptr=self.Xt.ptr, owner=False)
+            self.Up = DeviceArray((2, nbc, v, nS), dt, ptr=self.Gs.ptr, owner=False)
+        else:                                           # scratch bounds below one block: own buffers
+            self.Tp = DeviceArray((2, 1, v, kp), dt)
+            self.Up = DeviceArray((2, 1, v, nS), dt)
 
     def _plan_vvvv_pairs(self):
         """W_vvvv x T3 (rccsdt.py:424-426) as pair-symmetric products.  Term t has its output pair and

@@ -78,7 +78,11 @@ def test_engine_vvvv_pairs(monkeypatch, o, v, prec, chunk):
     isz = 8 if prec == "fp64" else 4
     eng = TriplesEngine(o, v, vvvv_pairs=True,
                         pair_bytes=None if chunk is None else chunk * 2 * v * (kp + nS) * isz, **kw)
-    nchunk = 1 if chunk is None else -(-eng.n3 // chunk)
+    # chunks of equal size within the bound (and within the T2 -> T3 / ring scratch they borrow)
+    nbc = eng.Tp.shape[1]
+    nchunk = -(-eng.n3 // nbc)
+    assert nbc <= (chunk or eng.n3) and nbc == -(-eng.n3 // nchunk)
+    assert eng.Tp.nbytes <= eng.Xt.nbytes and eng.Up.nbytes <= eng.Gs.nbytes
     pairs = [g for g in eng.launches if isinstance(g, GemmLaunch) and g.name.startswith("W_vvvv pairs")]
     assert len(pairs) == 3 * nchunk and not any(g.name.startswith("W_vvvv a") for g in eng.launches)
     if prec == "bf16x3":

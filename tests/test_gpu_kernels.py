@@ -469,8 +469,12 @@ def test_residual_vvvv_pairs_vs_oracle(dev, prec, o, v, chunk):
     isz = 8 if prec == "fp64" else 4
     eng = TriplesEngine(o, v, vvvv_pairs=True,
                         pair_bytes=None if chunk is None else chunk * 2 * v * (kp + nS) * isz, **kw)
-    # chunks of at most `chunk` blocks, made equal: 20 blocks with a bound of 9 are 3 chunks of 7
-    assert eng.Tp.shape[1] == (eng.n3 if chunk is None else -(-eng.n3 // -(-eng.n3 // chunk)))
+    # chunks of at most `chunk` blocks, made equal (20 blocks with a bound of 9: 3 chunks of 7), in the
+    # T2 -> T3 and ring scratch buffers
+    nbc = eng.Tp.shape[1]
+    assert nbc <= (chunk or eng.n3) and nbc == -(-eng.n3 // -(-eng.n3 // nbc))
+    assert chunk is None or nbc == 7 or eng.n3 != 20
+    assert eng.Tp.ptr == eng.Xt.ptr and eng.Up.ptr == eng.Gs.ptr
     R = []
     for e in (eng, TriplesEngine(o, v, **kw)):
         e.set_t3(t3)
