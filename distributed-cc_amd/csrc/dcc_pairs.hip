@@ -27,18 +27,23 @@ namespace {
 
 constexpr int kPT = 32;        // plane tile edge
 constexpr int kPThreads = 256;
+// flat element loops run on 32-bit indices below this extent (the loop index plus one grid stride stays
+// below 2^32)
+constexpr int64_t kIndex32 = (int64_t)1 << 31;
 
 __device__ __forceinline__ int64_t row_start(int64_t d, int v, bool strict) {
     return strict ? d * (2 * (int64_t)v - d - 1) / 2 : d * (2 * (int64_t)v - d + 1) / 2;
 }
 
-// (d, e) of pair index q (d <= e, or d < e when strict); q must be below the pair count.
+// (d, e) of pair index q (d <= e, or d < e when strict); q must be below the pair count.  A single-precision
+// estimate of the row, corrected by whole steps against the exact integer row starts (one step at most
+// for any nvir this library accepts, where the estimate is off by far less than a row).
 __device__ __forceinline__ void decode_pair(int64_t q, int v, bool strict, int &d, int &e) {
-    const double w = strict ? 2.0 * v - 1.0 : 2.0 * v + 1.0;
-    int x = (int)((w - sqrt(fmax(w * w - 8.0 * (double)q, 0.0))) * 0.5);
+    const float w = strict ? 2.0f * v - 1.0f : 2.0f * v + 1.0f;
+    int x = (int)((w - sqrtf(fmaxf(w * w - 8.0f * (float)q, 0.0f))) * 0.5f);
     x = x < 0 ? 0 : (x >= v ? v - 1 : x);
-    if (row_start(x, v, strict) > q) --x;
-    else if (row_start(x + 1, v, strict) <= q) ++x;
+    while (x > 0 && row_start(x, v, strict) > q) --x;
+    while (row_start(x + 1, v, strict) <= q) ++x;
     d = x;
     e = (int)(q - row_start(x, v, strict)) + x + (strict ? 1 : 0);
 }
@@ -57,13 +62,15 @@ inline unsigned grid_for(int64_t total) {
 }
 
 // Wp[r][q], r < nS: WS, else WA (row r - nS); one element per thread, q fastest.
-template <typename T, typename O>
+// I: the index type of the flat element loop (uint32_t when the extent allows: cheap divisions).
+template <typename T, typename O, typename I>
 __global__ __launch_bounds__(kPThreads) void pair_weights_kernel(const T *__restrict__ W1, int v, int64_t nS,
-                                                                int64_t nA, int64_t kp, O *__restrict__ Wp,
-                                                                int64_t total) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
-        const int64_t r = t / kp, q = t - r * kp;
+                                                                int64_t nA, I kp, O *__restrict__ Wp,
+                                                                I total) {
+    const I stride = (I)gridDim.x * blockDim.x;
+    for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const I r_ = t / kp;
+        const int64_t r = r_, q = t - r_ * kp;
         const bool anti = r >= nS;
         T x = (T)0;
         if (q < (anti ? nA : nS)) {
@@ -80,15 +87,17 @@ __global__ __launch_bounds__(kPThreads) void pair_weights_kernel(const T *__rest
 }
 
 // Pair axes (0, 1): Tp[part][b][q][z] = T[b,d,e,z] +- T[b,e,d,z], z innermost on both sides.
-template <typename T, typename O>
+template <typename T, typename O, typename I>
 __global__ __launch_bounds__(kPThreads) void pair_split_row_kernel(const T *__restrict__ t3, int v, int64_t nS,
-                                                                  int64_t nA, int64_t kp, int64_t half,
+                                                                  int64_t nA, I kp, I half,
                                                                   O *__restrict__ Tp) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x, v3 = (int64_t)v * v * v;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < 2 * half; t += stride) {
+    const int64_t v3 = (int64_t)v * v * v;
+    const I stride = (I)gridDim.x * blockDim.x, vv = (I)v;
+    for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < 2 * half; t += stride) {
         const bool anti = t >= half;
-        const int64_t u = anti ? t - half : t;
-        const int64_t row = u / v, z = u - row * v, b = row / kp, q = row - b * kp;
+        const I u = anti ? t - half : t;
+        const I row_ = u / vv, b_ = row_ / kp;
+        const int64_t z = u - row_ * vv, b = b_, q = row_ - b_ * kp;
         T x = (T)0;
         if (q < (anti ? nA : nS)) {
             int d, e;
@@ -148,13 +157,13 @@ __global__ __launch_bounds__(kPThreads) void pair_split_plane_kernel(const T *__
 }
 
 // Pair axes (0, 1): R[b,x,y,z] += (US[b][p][z] +- UA[b][p][z]) / 2 with p the pair of (min, max) of (x, y).
-template <typename T>
+template <typename T, typename I>
 __global__ __launch_bounds__(kPThreads) void pair_unfold_row_kernel(const T *__restrict__ Up, int v, int64_t mp,
-                                                                   int64_t half, T *__restrict__ R,
-                                                                   int64_t total) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
-        const int64_t r1 = t / v, z = t - r1 * v, r2 = r1 / v, y = r1 - r2 * v, b = r2 / v, x = r2 - b * v;
+                                                                   int64_t half, T *__restrict__ R, I total) {
+    const I stride = (I)gridDim.x * blockDim.x, vv = (I)v;
+    for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += stride) {
+        const I r1_ = t / vv, r2_ = r1_ / vv, b_ = r2_ / vv;
+        const int64_t z = t - r1_ * vv, y = r1_ - r2_ * vv, b = b_, x = r2_ - b_ * vv;
         const int64_t lo = x < y ? x : y, hi = x < y ? y : x;
         const T *U = Up + b * mp * v;
         T s = U[(row_start(lo, v, false) + (hi - lo)) * v + z];
@@ -226,9 +235,13 @@ int pair_weights(const T *W1, int64_t nvir, int64_t kpad, O *Wp, dcc_stream_t st
     if (int rc = check_sizes(0, nvir, 0, kpad)) return rc;
     if (nvir == 0) return 0;
     DCC_CHECK(W1 != nullptr && Wp != nullptr, "null pointer");
-    const int64_t nS = nvir * (nvir + 1) / 2, nA = nvir * (nvir - 1) / 2;
-    hipLaunchKernelGGL((pair_weights_kernel<T, O>), dim3(grid_for((nS + nA) * kpad)), dim3(kPThreads), 0,
-                       dcc::as_stream(stream), W1, (int)nvir, nS, nA, kpad, Wp, (nS + nA) * kpad);
+    const int64_t nS = nvir * (nvir + 1) / 2, nA = nvir * (nvir - 1) / 2, total = (nS + nA) * kpad;
+    if (total < kIndex32)
+        hipLaunchKernelGGL((pair_weights_kernel<T, O, uint32_t>), dim3(grid_for(total)), dim3(kPThreads), 0,
+                           dcc::as_stream(stream), W1, (int)nvir, nS, nA, (uint32_t)kpad, Wp, (uint32_t)total);
+    else
+        hipLaunchKernelGGL((pair_weights_kernel<T, O, int64_t>), dim3(grid_for(total)), dim3(kPThreads), 0,
+                           dcc::as_stream(stream), W1, (int)nvir, nS, nA, kpad, Wp, total);
     DCC_LAUNCH_CHECK();
     return 0;
 }
@@ -241,8 +254,12 @@ int pair_split(const T *t3, int64_t nblk, int64_t nvir, int32_t axes, int64_t kp
     const int64_t nS = nvir * (nvir + 1) / 2, nA = nvir * (nvir - 1) / 2, half = nblk * kpad * nvir;
     hipStream_t s = dcc::as_stream(stream);
     if (axes == 0) {
-        hipLaunchKernelGGL((pair_split_row_kernel<T, O>), dim3(grid_for(2 * half)), dim3(kPThreads), 0, s, t3,
-                           (int)nvir, nS, nA, kpad, half, Tp);
+        if (2 * half < kIndex32)
+            hipLaunchKernelGGL((pair_split_row_kernel<T, O, uint32_t>), dim3(grid_for(2 * half)), dim3(kPThreads), 0,
+                               s, t3, (int)nvir, nS, nA, (uint32_t)kpad, (uint32_t)half, Tp);
+        else
+            hipLaunchKernelGGL((pair_split_row_kernel<T, O, int64_t>), dim3(grid_for(2 * half)), dim3(kPThreads), 0,
+                               s, t3, (int)nvir, nS, nA, kpad, half, Tp);
     } else {
         int64_t sx, sz;
         plane_strides(axes, nvir, sx, sz);
@@ -266,8 +283,12 @@ int pair_unfold(const T *Up, int64_t nblk, int64_t nvir, int32_t axes, T *R, dcc
     hipStream_t s = dcc::as_stream(stream);
     if (axes == 0) {
         const int64_t total = nblk * nvir * nvir * nvir;
-        hipLaunchKernelGGL((pair_unfold_row_kernel<T>), dim3(grid_for(total)), dim3(kPThreads), 0, s, Up,
-                           (int)nvir, mp, half, R, total);
+        if (total < kIndex32)
+            hipLaunchKernelGGL((pair_unfold_row_kernel<T, uint32_t>), dim3(grid_for(total)), dim3(kPThreads), 0, s,
+                               Up, (int)nvir, mp, half, R, (uint32_t)total);
+        else
+            hipLaunchKernelGGL((pair_unfold_row_kernel<T, int64_t>), dim3(grid_for(total)), dim3(kPThreads), 0, s,
+                               Up, (int)nvir, mp, half, R, total);
     } else {
         int64_t sx, sz;
         plane_strides(axes, nvir, sx, sz);
