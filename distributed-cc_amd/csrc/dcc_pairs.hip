@@ -110,6 +110,44 @@ __global__ __launch_bounds__(kPThreads) void pair_split_row_kernel(const T *__re
     }
 }
 
+template <typename T> struct Vec4;
+template <> struct Vec4<float> { using type = float4; };
+template <> struct Vec4<double> { using type = double4; };
+template <> struct Vec4<uint32_t> { using type = uint4; };
+
+// The row split with four consecutive z per thread as one vector (v % 4 == 0, 16-byte aligned blocks):
+// one pair decode and two vector loads per four elements.  g runs over groups of four.
+template <typename T, typename O, typename I>
+__global__ __launch_bounds__(kPThreads) void pair_split_row4_kernel(const T *__restrict__ t3, int v, int64_t nS,
+                                                                   int64_t nA, I kp, I half4,
+                                                                   O *__restrict__ Tp) {
+    using TV = typename Vec4<T>::type;
+    using OV = typename Vec4<O>::type;
+    const int64_t v3 = (int64_t)v * v * v;
+    const I stride = (I)gridDim.x * blockDim.x, v4 = (I)(v / 4);
+    for (I g = (I)blockIdx.x * blockDim.x + threadIdx.x; g < 2 * half4; g += stride) {
+        const bool anti = g >= half4;
+        const I u = anti ? g - half4 : g;
+        const I row_ = u / v4, b_ = row_ / kp;
+        const int64_t z = 4 * (int64_t)(u - row_ * v4), b = b_, q = row_ - b_ * kp;
+        OV out;
+        if (q < (anti ? nA : nS)) {
+            int d, e;
+            decode_pair(q, v, anti, d, e);
+            const T *blk = t3 + b * v3;
+            const TV p = *reinterpret_cast<const TV *>(blk + ((int64_t)d * v + e) * v + z);
+            const TV m = *reinterpret_cast<const TV *>(blk + ((int64_t)e * v + d) * v + z);
+            out.x = out_value<O>(anti ? p.x - m.x : p.x + m.x);
+            out.y = out_value<O>(anti ? p.y - m.y : p.y + m.y);
+            out.z = out_value<O>(anti ? p.z - m.z : p.z + m.z);
+            out.w = out_value<O>(anti ? p.w - m.w : p.w + m.w);
+        } else {
+            out.x = out.y = out.z = out.w = out_value<O>((T)0);
+        }
+        reinterpret_cast<OV *>(Tp)[g] = out;
+    }
+}
+
 // Pair axes (0, 2) or (1, 2): the pair spans a plane P[x][y] = T[b, x sx + z sz + y] whose columns are
 // contiguous.  One workgroup per (block, z, tile pair dt <= et): it stages the tiles P[dt][et] and
 // P[et][dt] in LDS and writes TS[q(d,e)] = P[d][e] + P[e][d] (d <= e) and TA[q(d,e)] = P[d][e] - P[e][d]
@@ -175,6 +213,32 @@ __global__ __launch_bounds__(kPThreads) void pair_unfold_row_kernel(const T *__r
     }
 }
 
+// The row unfold with four consecutive z per thread as one vector (v % 4 == 0, 16-byte aligned).
+template <typename T, typename I>
+__global__ __launch_bounds__(kPThreads) void pair_unfold_row4_kernel(const T *__restrict__ Up, int v, int64_t mp,
+                                                                    int64_t half, T *__restrict__ R, I total4) {
+    using TV = typename Vec4<T>::type;
+    const I stride = (I)gridDim.x * blockDim.x, v4 = (I)(v / 4), vv = (I)v;
+    for (I g = (I)blockIdx.x * blockDim.x + threadIdx.x; g < total4; g += stride) {
+        const I r1_ = g / v4, r2_ = r1_ / vv, b_ = r2_ / vv;
+        const int64_t z = 4 * (int64_t)(g - r1_ * v4), y = r1_ - r2_ * vv, b = b_, x = r2_ - b_ * vv;
+        const int64_t lo = x < y ? x : y, hi = x < y ? y : x;
+        const T *U = Up + b * mp * v;
+        TV s = *reinterpret_cast<const TV *>(U + (row_start(lo, v, false) + (hi - lo)) * v + z);
+        if (lo < hi) {
+            const TV a = *reinterpret_cast<const TV *>(U + half + (row_start(lo, v, true) + (hi - lo - 1)) * v + z);
+            if (x < y) { s.x += a.x; s.y += a.y; s.z += a.z; s.w += a.w; }
+            else { s.x -= a.x; s.y -= a.y; s.z -= a.z; s.w -= a.w; }
+        }
+        TV r = reinterpret_cast<TV *>(R)[g];
+        r.x += (T)0.5 * s.x;
+        r.y += (T)0.5 * s.y;
+        r.z += (T)0.5 * s.z;
+        r.w += (T)0.5 * s.w;
+        reinterpret_cast<TV *>(R)[g] = r;
+    }
+}
+
 // Pair axes (0, 2) or (1, 2): R[b, x sx + z sz + y] += (US +- UA)[b][z][p] / 2.  One workgroup per
 // (block, z, tile xt, tile yt); elements with x <= y read their pair along y directly, those with x > y
 // come through an LDS tile loaded along x (the contiguous direction of their pairs), so every read and
@@ -222,6 +286,12 @@ inline void plane_strides(int axes, int64_t v, int64_t &sx, int64_t &sz) {
     else { sx = v; sz = v * v; }
 }
 
+// four consecutive z as one vector: rows of nvir elements in multiples of four and a buffer aligned to
+// four elements (every row, block and half offset is then aligned too)
+inline bool vec4_ok(int64_t nvir, const void *p, size_t elem) {
+    return nvir % 4 == 0 && ((uintptr_t)p % (4 * elem)) == 0;
+}
+
 int check_sizes(int64_t nblk, int64_t nvir, int32_t axes, int64_t kpad) {
     DCC_CHECK(nblk >= 0 && nvir >= 0, "negative extent");
     DCC_CHECK(axes >= 0 && axes <= 2, "pair axes must be 0 = (0,1), 1 = (0,2) or 2 = (1,2)");
@@ -254,7 +324,10 @@ int pair_split(const T *t3, int64_t nblk, int64_t nvir, int32_t axes, int64_t kp
     const int64_t nS = nvir * (nvir + 1) / 2, nA = nvir * (nvir - 1) / 2, half = nblk * kpad * nvir;
     hipStream_t s = dcc::as_stream(stream);
     if (axes == 0) {
-        if (2 * half < kIndex32)
+        if (vec4_ok(nvir, t3, sizeof(T)) && vec4_ok(nvir, Tp, sizeof(O)) && 2 * half < kIndex32)
+            hipLaunchKernelGGL((pair_split_row4_kernel<T, O, uint32_t>), dim3(grid_for(half / 2)), dim3(kPThreads),
+                               0, s, t3, (int)nvir, nS, nA, (uint32_t)kpad, (uint32_t)(half / 4), Tp);
+        else if (2 * half < kIndex32)
             hipLaunchKernelGGL((pair_split_row_kernel<T, O, uint32_t>), dim3(grid_for(2 * half)), dim3(kPThreads), 0,
                                s, t3, (int)nvir, nS, nA, (uint32_t)kpad, (uint32_t)half, Tp);
         else
@@ -283,7 +356,10 @@ int pair_unfold(const T *Up, int64_t nblk, int64_t nvir, int32_t axes, T *R, dcc
     hipStream_t s = dcc::as_stream(stream);
     if (axes == 0) {
         const int64_t total = nblk * nvir * nvir * nvir;
-        if (total < kIndex32)
+        if (vec4_ok(nvir, Up, sizeof(T)) && vec4_ok(nvir, R, sizeof(T)) && total < kIndex32)
+            hipLaunchKernelGGL((pair_unfold_row4_kernel<T, uint32_t>), dim3(grid_for(total / 4)), dim3(kPThreads), 0,
+                               s, Up, (int)nvir, mp, half, R, (uint32_t)(total / 4));
+        else if (total < kIndex32)
             hipLaunchKernelGGL((pair_unfold_row_kernel<T, uint32_t>), dim3(grid_for(total)), dim3(kPThreads), 0, s,
                                Up, (int)nvir, mp, half, R, (uint32_t)total);
         else
