@@ -373,6 +373,7 @@ def plan_t2t3(o, v, R, t2, W_ovvv, W_oovo, scratch_bytes, packed=False):
         # one gather pass: target q takes sum_pi X[pi(ijk)]^{pi^-1} over the sources in this chunk
         jobs = []
         for q, t in enumerate(triples):
+            t = tuple(int(x) for x in t)   # Python integers: the offsets pass 2^31 for large chunks
             srcs, coef = [0] * 6, np.zeros(6)
             for pos in pis:
                 src = (t[pos[0]], t[pos[1]], t[pos[2]])
