@@ -1,6 +1,7 @@
-"""Launch plans of large engines checked on the host, where buffer offsets pass 2^31: every GEMM operand
-range (index maps over each batch entry's rows), every S3 job's target and source blocks and every pair
-split/unfold range lies inside the buffer its launch points into, and no offset arithmetic overflows.
+"""Launch plans of large engines (the residual and the T3 -> r1/r2/W chunks) checked on the host, where
+buffer offsets pass 2^31: every GEMM operand range (index maps over each batch entry's rows), every S3
+job's target and source blocks and every pair split/unfold range lies inside the buffer its launch points
+into, and no offset arithmetic overflows.
 A fake device allocator hands out address ranges: nothing is allocated or run."""
 import warnings
 
@@ -52,6 +53,26 @@ def _map(mp, count):
     return q0 * mp.s0 + q1 * mp.s1 + (r0 - q1 * e2) * mp.s2
 
 
+def _check_gemm(g, allocs, isz):
+    """Largest element offset of the launch after checking every operand range against its buffer."""
+    M, N, K = g.args[:3]
+    A, am, ak, B, bk, bn, C, cm, cn = g.args[5:14]
+    assert not any(mp.table for mp in (am, ak, bk, bn, cm, cn)), g.name
+    k_a, k_b, n_b, n_c = _map(ak, K), _map(bk, K), _map(bn, N), _map(cn, N)
+    big = 0
+    for row in g.batches._host:
+        mb, me = int(row["m_begin"]), int(row["m_end"])
+        m_a, m_c = _map(am, me)[mb:], _map(cm, me)[mb:]
+        for what, ptr, off, r1, r2 in (("A", A.value, row["a_off"], m_a, k_a),
+                                       ("B", B.value, row["b_off"], k_b, n_b),
+                                       ("C", C.value, row["c_off"], m_c, n_c)):
+            lo, hi = int(off) + int(r1.min()) + int(r2.min()), int(off) + int(r1.max()) + int(r2.max())
+            s, n = _owner(allocs, ptr)
+            assert lo >= 0 and ptr + (hi + 1) * isz <= s + n, (g.name, what, hi, (s + n - ptr) // isz)
+            big = max(big, hi)
+    return big
+
+
 @pytest.mark.parametrize("o,v,prec,gbytes", [(16, 160, "bf16x3", 16), (16, 160, "fp64", 8), (10, 90, "fp64", 8)])
 def test_residual_plan_in_bounds(monkeypatch, o, v, prec, gbytes):
     allocs = _fake_device(monkeypatch)
@@ -60,25 +81,20 @@ def test_residual_plan_in_bounds(monkeypatch, o, v, prec, gbytes):
     with warnings.catch_warnings():
         warnings.simplefilter("error", RuntimeWarning)     # NumPy integer overflow in offset arithmetic
         eng = TriplesEngine(o, v, ring_fused=True, ring_g_bytes=gbytes << 30, vvvv_pairs=True, **kw)
+        eng._setup_lower()                                   # T3 -> r1/r2/W chunks (Z slices of T3)
     isz = eng.dtype.itemsize
     v3 = v ** 3
     big = 0
+    for ch in eng._lower["chunks"]:
+        jobs = ch[0]._host
+        assert (jobs["dst"] >= 0).all() and (jobs["dst"] + v3 <= eng._lower["Z"].size).all()
+        assert (jobs["src"] >= 0).all() and (jobs["src"] + v3 <= eng.t3.size).all()
+        for g in ch[2] + ch[3]:
+            if isinstance(g, GemmLaunch):
+                big = max(big, _check_gemm(g, allocs, isz))
     for g in eng.launches:
         if isinstance(g, GemmLaunch):
-            M, N, K = g.args[:3]
-            A, am, ak, B, bk, bn, C, cm, cn = g.args[5:14]
-            assert not any(mp.table for mp in (am, ak, bk, bn, cm, cn)), g.name
-            k_a, k_b, n_b, n_c = _map(ak, K), _map(bk, K), _map(bn, N), _map(cn, N)
-            for row in g.batches._host:
-                mb, me = int(row["m_begin"]), int(row["m_end"])
-                m_a, m_c = _map(am, me)[mb:], _map(cm, me)[mb:]
-                for what, ptr, off, r1, r2 in (("A", A.value, row["a_off"], m_a, k_a),
-                                               ("B", B.value, row["b_off"], k_b, n_b),
-                                               ("C", C.value, row["c_off"], m_c, n_c)):
-                    lo, hi = int(off) + int(r1.min()) + int(r2.min()), int(off) + int(r1.max()) + int(r2.max())
-                    s, n = _owner(allocs, ptr)
-                    assert lo >= 0 and ptr + (hi + 1) * isz <= s + n, (g.name, what, hi, (s + n - ptr) // isz)
-                    big = max(big, hi)
+            big = max(big, _check_gemm(g, allocs, isz))
         elif isinstance(g, (S3GatherLaunch, S3Launch)):
             jobs = g.jobs._host
             assert (jobs["dst"] >= 0).all() and (jobs["dst"] + v3 <= g.out.size).all(), g.name
