@@ -10,6 +10,8 @@ Data in HBM (FP64, C order):
   Xt             (i per chunk, o, o, v, v, v)   T2->T3 base term X over ordered (i,j,k)
   pfs            (6, n3, n3)       W_oooo/F_oo prefactors
   W1             (v, v, v, v)      W_vvvv with the F_vv terms folded in
+  Wp             (nS + nA, kpad)   pair-symmetric weights from W1 (vvvv_pairs); Tp / Up, the split T3
+                                   and the products of a chunk of blocks, are views of Xt / Gs
   Xp             (5, targets per chunk, v, v, v)  W_oooo/F_oo terms s = 1..5 before their permutation
   intermediates  shapes of distr_cc/rccsdt.py:380,445,625
 
@@ -19,6 +21,9 @@ batched gather-GEMMs (dcc_contract_f64):
     S3 gather pass: each canonical block adds its six simultaneously (ijk, abc)-permuted sources;
   * F_vv and W_vvvv x T3 (rccsdt.py:400-402,424-426): 3 launches, N = (block, ...); the F_vv terms
     are folded into one copy of W_vvvv with F_vv/2 added on two of its diagonals (prepare_residual);
+    with ``vvvv_pairs`` (a pair-symmetric W_vvvv) each term is instead, per chunk of blocks, a split of
+    T3 into T[de] +- T[ed] over triangular pairs, one GEMM of two batch entries (symmetric and
+    antisymmetric weights, half the flops of the plain product) and an unfold adding (US +- UA)/2 to R;
   * W_ovov/W_ovvo x T3 (rccsdt.py:540-596): with G the part-3 operator
         G[r;pq][a,b,c] = sum_{l,d} ( -W_ovov[l,b,r,d] Y[l][a,d,c] - W_ovov[l,c,r,d] Y[l][a,b,d]
                         + W_ovvo[l,a,d,r] Y[l][d,b,c]
