@@ -25,7 +25,9 @@
 
 namespace {
 
-constexpr int kPT = 32;        // plane tile edge
+// plane tile edge (64 x 64 tiles of 4-byte elements ran the plane passes about 20 % slower: fewer
+// workgroups in flight per CU)
+template <typename T> constexpr int plane_tile() { return 32; }
 constexpr int kPThreads = 256;
 // flat element loops run on 32-bit indices below this extent (the loop index plus one grid stride stays
 // below 2^32)
@@ -153,12 +155,12 @@ __global__ __launch_bounds__(kPThreads) void pair_split_row4_kernel(const T *__r
 // P[et][dt] in LDS and writes TS[q(d,e)] = P[d][e] + P[e][d] (d <= e) and TA[q(d,e)] = P[d][e] - P[e][d]
 // (d < e) along e, so reads and writes both run along contiguous rows.  The extra workgroup per (block,
 // z) (tile index ntp) writes the zero padding of both rows.
-template <typename T, typename O>
+template <typename T, typename O, int PT>
 __global__ __launch_bounds__(kPThreads) void pair_split_plane_kernel(const T *__restrict__ t3, int v, int64_t sx,
                                                                     int64_t sz, int64_t nS, int64_t nA,
                                                                     int64_t kp, int64_t half, int ntile, int ntp,
                                                                     O *__restrict__ Tp) {
-    __shared__ T s1[kPT][kPT + 1], s2[kPT][kPT + 1];
+    __shared__ T s1[PT][PT + 1], s2[PT][PT + 1];
     const int64_t wg = blockIdx.x;
     const int tp = (int)(wg % (ntp + 1));
     const int64_t bz = wg / (ntp + 1);
@@ -176,16 +178,16 @@ __global__ __launch_bounds__(kPThreads) void pair_split_plane_kernel(const T *__
         ++dt;
     }
     const int et = dt + rem;
-    const int d0 = dt * kPT, e0 = et * kPT;
+    const int d0 = dt * PT, e0 = et * PT;
     const T *plane = t3 + b * (int64_t)v * v * v + z * sz;
-    const int c = tid % kPT, r0 = tid / kPT;
-    for (int r = r0; r < kPT; r += kPThreads / kPT) {
+    const int c = tid % PT, r0 = tid / PT;
+    for (int r = r0; r < PT; r += kPThreads / PT) {
         const int x1 = d0 + r, y1 = e0 + c, x2 = e0 + r, y2 = d0 + c;
         s1[r][c] = (x1 < v && y1 < v) ? plane[x1 * sx + y1] : (T)0;
         s2[r][c] = (x2 < v && y2 < v) ? plane[x2 * sx + y2] : (T)0;
     }
     __syncthreads();
-    for (int r = r0; r < kPT; r += kPThreads / kPT) {
+    for (int r = r0; r < PT; r += kPThreads / PT) {
         const int d = d0 + r, e = e0 + c;
         if (d >= v || e >= v || d > e) continue;
         const T p = s1[r][c], m = s2[c][r];
@@ -243,24 +245,24 @@ __global__ __launch_bounds__(kPThreads) void pair_unfold_row4_kernel(const T *__
 // (block, z, tile xt, tile yt); elements with x <= y read their pair along y directly, those with x > y
 // come through an LDS tile loaded along x (the contiguous direction of their pairs), so every read and
 // the read-modify-write of R run along contiguous rows.
-template <typename T>
+template <typename T, int PT>
 __global__ __launch_bounds__(kPThreads) void pair_unfold_plane_kernel(const T *__restrict__ Up, int v, int64_t sx,
                                                                      int64_t sz, int64_t mp, int64_t half,
                                                                      int ntile, T *__restrict__ R) {
-    __shared__ T sl[kPT][kPT + 1];
+    __shared__ T sl[PT][PT + 1];
     const int64_t wg = blockIdx.x;
     const int nt2 = ntile * ntile;
     const int tt = (int)(wg % nt2);
     const int64_t bz = wg / nt2;
     const int64_t b = bz / v, z = bz - b * v;
     const int xt = tt / ntile, yt = tt - xt * ntile;
-    const int x0 = xt * kPT, y0 = yt * kPT;
+    const int x0 = xt * PT, y0 = yt * PT;
     const T *US = Up + bz * mp, *UA = Up + half + bz * mp;
     T *plane = R + b * (int64_t)v * v * v + z * sz;
-    const int tid = threadIdx.x, c = tid % kPT, r0 = tid / kPT;
+    const int tid = threadIdx.x, c = tid % PT, r0 = tid / PT;
     if (xt >= yt) {
         // lower part: pair (y, x) with y = y0 + r, x = x0 + c, y < x
-        for (int r = r0; r < kPT; r += kPThreads / kPT) {
+        for (int r = r0; r < PT; r += kPThreads / PT) {
             const int y = y0 + r, x = x0 + c;
             T val = (T)0;
             if (x < v && y < v && y < x)
@@ -269,7 +271,7 @@ __global__ __launch_bounds__(kPThreads) void pair_unfold_plane_kernel(const T *_
         }
         __syncthreads();
     }
-    for (int r = r0; r < kPT; r += kPThreads / kPT) {
+    for (int r = r0; r < PT; r += kPThreads / PT) {
         const int x = x0 + r, y = y0 + c;
         if (x >= v || y >= v) continue;
         T val;
@@ -336,10 +338,11 @@ int pair_split(const T *t3, int64_t nblk, int64_t nvir, int32_t axes, int64_t kp
     } else {
         int64_t sx, sz;
         plane_strides(axes, nvir, sx, sz);
-        const int ntile = (int)((nvir + kPT - 1) / kPT), ntp = ntile * (ntile + 1) / 2;
+        constexpr int PT = plane_tile<T>();
+        const int ntile = (int)((nvir + PT - 1) / PT), ntp = ntile * (ntile + 1) / 2;
         const int64_t grid = nblk * nvir * (ntp + 1);
         DCC_CHECK(grid < ((int64_t)1 << 31), "grid exceeds 2^31 workgroups");
-        hipLaunchKernelGGL((pair_split_plane_kernel<T, O>), dim3((unsigned)grid), dim3(kPThreads), 0, s, t3,
+        hipLaunchKernelGGL((pair_split_plane_kernel<T, O, PT>), dim3((unsigned)grid), dim3(kPThreads), 0, s, t3,
                            (int)nvir, sx, sz, nS, nA, kpad, half, ntile, ntp, Tp);
     }
     DCC_LAUNCH_CHECK();
@@ -368,10 +371,11 @@ int pair_unfold(const T *Up, int64_t nblk, int64_t nvir, int32_t axes, T *R, dcc
     } else {
         int64_t sx, sz;
         plane_strides(axes, nvir, sx, sz);
-        const int ntile = (int)((nvir + kPT - 1) / kPT);
+        constexpr int PT = plane_tile<T>();
+        const int ntile = (int)((nvir + PT - 1) / PT);
         const int64_t grid = nblk * nvir * ntile * ntile;
         DCC_CHECK(grid < ((int64_t)1 << 31), "grid exceeds 2^31 workgroups");
-        hipLaunchKernelGGL((pair_unfold_plane_kernel<T>), dim3((unsigned)grid), dim3(kPThreads), 0, s, Up,
+        hipLaunchKernelGGL((pair_unfold_plane_kernel<T, PT>), dim3((unsigned)grid), dim3(kPThreads), 0, s, Up,
                            (int)nvir, sx, sz, mp, half, ntile, R);
     }
     DCC_LAUNCH_CHECK();

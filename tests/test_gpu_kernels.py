@@ -382,11 +382,12 @@ _PAIR_PLANE = {0: (0, 3, 1, 2), 1: (0, 2, 1, 3), 2: (0, 1, 2, 3)}   # block -> [
 
 
 @pytest.mark.parametrize("dt", ["f64", "f32", "packed_f32"])
-@pytest.mark.parametrize("v,nb", [(5, 3), (33, 2), (40, 2), (64, 1)])
+@pytest.mark.parametrize("v,nb", [(5, 3), (33, 2), (40, 2), (64, 1), (100, 1)])
 def test_vvvv_pair_kernels(dev, dt, v, nb):
     """dcc_vvvv_pair_weights / _split / _unfold element for element against NumPy for all three pair
     axes (packed words against dcc_pack_bf16x3 of the FP32 values); outputs start as NaN / junk words so
-    padding the kernels leave unwritten shows."""
+    padding the kernels leave unwritten shows.  The plane kernels tile the pair plane 32 x 32: nvir=100
+    has off-diagonal and partial tiles."""
     from distr_cc import _lib as L
     from distr_cc.device import DeviceArray
     from distr_cc.t3_engine import pair_dims
